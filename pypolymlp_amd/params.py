@@ -1,0 +1,190 @@
+"""Model parameters, the params dict, and the polymlp.yaml model format.
+
+The params dict produced here has the schema of
+``PolymlpParamsSingle.as_dict()`` (``src/pypolymlp/core/data_format.py:268-272``)
+restricted to the keys the hot path reads
+(``src/pypolymlp/cxx/src/compute/py_params.cpp:11-60``):
+
+    n_type, print_memory,
+    model.{cutoff, pair_type, feature_type, model_type, max_p, max_l,
+           gtinv.{order, max_l, version, lm_seq, l_comb, lm_coeffs},
+           pair_conditional, pair_params, pair_params_conditional[(j, i)], j <= i}
+
+so that dicts made by pypolymlp itself are accepted unchanged.
+"""
+
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import yaml
+
+from .gtinv import get_gtinv_attrs
+
+MASS = {"H": 1.008, "Al": 26.98, "Mg": 24.31, "O": 16.0, "Fe": 55.85, "Cu": 63.55,
+        "Ag": 107.9, "Au": 197.0, "Ti": 47.87, "Si": 28.09, "A": 1.0, "B": 1.0, "C": 12.01}
+
+
+def set_gaussian_params(params1=(1.0, 1.0, 1), params2=(0.0, 5.0, 7)):
+    """Gaussian (beta, mu) pairs plus the constant function (params_utils.py:72-94)."""
+    g1 = np.linspace(float(params1[0]), float(params1[1]), int(params1[2]))
+    g2 = np.linspace(float(params2[0]), float(params2[1]), int(params2[2]))
+    pair_params = [list(map(float, p)) for p in itertools.product(g1, g2)]
+    pair_params.append([0.0, 0.0])
+    return pair_params
+
+
+def all_pair_conditional(n_type: int, n_params: int):
+    """Every radial function active for every type pair (params_utils.py:97-138)."""
+    return {
+        (i, j): list(range(n_params))
+        for i, j in itertools.combinations_with_replacement(range(n_type), 2)
+    }
+
+
+@dataclass
+class ModelSpec:
+    """Hyperparameters of one polymlp model (PolymlpModelParams subset)."""
+
+    n_type: int
+    cutoff: float
+    feature_type: str = "gtinv"
+    model_type: int = 4
+    max_p: int = 2
+    gtinv_order: int = 2
+    gtinv_maxl: Sequence[int] = (4,)
+    gaussian_params1: Sequence[float] = (1.0, 1.0, 1)
+    gaussian_params2: Sequence[float] = (0.0, 5.0, 7)
+    pair_params: Optional[list] = None
+    pair_params_conditional: Optional[dict] = None
+    elements: Optional[Sequence[str]] = None
+    gtinv_version: int = 1
+    extra: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        if self.pair_params is None:
+            self.pair_params = set_gaussian_params(self.gaussian_params1, self.gaussian_params2)
+        if self.pair_params_conditional is None:
+            self.pair_params_conditional = all_pair_conditional(self.n_type, len(self.pair_params))
+        if self.elements is None:
+            self.elements = [chr(ord("A") + i) for i in range(self.n_type)]
+        if self.feature_type == "pair":
+            self.gtinv_order = 0
+            self.gtinv_maxl = ()
+
+    @property
+    def max_l(self) -> int:
+        return int(max(self.gtinv_maxl)) if self.feature_type == "gtinv" and self.gtinv_maxl else 0
+
+    def as_dict(self) -> dict:
+        """Params dict with the schema of PolymlpParamsSingle.as_dict()."""
+        if self.feature_type == "gtinv":
+            l_comb, lm_seq, lm_coeffs = get_gtinv_attrs(
+                self.gtinv_order, self.gtinv_maxl, self.gtinv_version
+            )
+        else:
+            l_comb, lm_seq, lm_coeffs = [], [], []
+        gtinv = {
+            "order": self.gtinv_order,
+            "max_l": list(self.gtinv_maxl),
+            "n_type": self.n_type,
+            "version": self.gtinv_version,
+            "lm_seq": lm_seq,
+            "l_comb": l_comb,
+            "lm_coeffs": lm_coeffs,
+        }
+        model = {
+            "cutoff": float(self.cutoff),
+            "model_type": int(self.model_type),
+            "max_p": int(self.max_p),
+            "max_l": self.max_l,
+            "feature_type": self.feature_type,
+            "gtinv": gtinv,
+            "pair_type": "gaussian",
+            "pair_conditional": True,
+            "pair_params": [list(map(float, p)) for p in self.pair_params],
+            "pair_params_conditional": {
+                tuple(k): list(v) for k, v in self.pair_params_conditional.items()
+            },
+        }
+        return {
+            "n_type": self.n_type,
+            "elements": list(self.elements),
+            "print_memory": False,
+            "include_force": True,
+            "include_stress": True,
+            "model": model,
+        }
+
+
+def save_mlp_yaml(spec: ModelSpec, coeffs, scales, filename: str):
+    """Write polymlp.yaml in the reference layout (io_polymlp_yaml.py:17-86).
+
+    The stored coefficients are ``coeffs / scales`` printed with ``%.15e``.
+    """
+    coeffs = np.asarray(coeffs, dtype=float)
+    scales = np.asarray(scales, dtype=float)
+    lines = []
+    lines.append("elements:      [" + ", ".join(spec.elements) + "]")
+    lines.append("enable_spins:  " + str([0 for _ in spec.elements]))
+    lines.append(f"cutoff:        {float(spec.cutoff)}")
+    lines.append("pair_type:     gaussian")
+    lines.append(f"feature_type:  {spec.feature_type}")
+    lines.append(f"model_type:    {spec.model_type}")
+    lines.append(f"max_p:         {spec.max_p}")
+    lines.append(f"max_l:         {spec.max_l}")
+    lines.append("")
+    if spec.feature_type == "gtinv":
+        lines.append(f"gtinv_order:   {spec.gtinv_order}")
+        lines.append(f"gtinv_maxl:    {list(map(int, spec.gtinv_maxl))}")
+        lines.append(f"gtinv_sym:     {[0 for _ in spec.gtinv_maxl]}")
+        lines.append(f"gtinv_version: {spec.gtinv_version}")
+        lines.append("")
+    lines.append("electrostatic: 0")
+    lines.append("mass:          " + str([MASS.get(e, 1.0) for e in spec.elements]))
+    lines.append("")
+    lines.append(f"n_pair_params: {len(spec.pair_params)}")
+    lines.append("pair_params:")
+    for p in spec.pair_params:
+        lines.append("- " + str([float(p[0]), float(p[1])]))
+    lines.append("")
+    lines.append(f"n_type_pairs: {len(spec.pair_params_conditional)}")
+    lines.append("type_pairs:")
+    for k, v in spec.pair_params_conditional.items():
+        lines.append("- atom_type_pair:      " + str(list(k)))
+        lines.append("  pair_params_indices: " + str(list(v)))
+    lines.append("")
+    lines.append("type_full:    1")
+    lines.append("type_indices: " + str(list(range(spec.n_type))))
+    lines.append("")
+    c = coeffs / scales
+    lines.append(f"n_coeffs: {len(c)}")
+    lines.append("coeffs: [" + ", ".join(f"{v:.15e}" for v in c) + "]")
+    with open(filename, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def load_mlp_yaml(filename: str):
+    """Read polymlp.yaml (io_polymlp_yaml.py:89-154). Returns (ModelSpec, coeffs)."""
+    yml = yaml.safe_load(open(filename))
+    elements = list(yml["elements"])
+    n_type = len(elements)
+    cond = {tuple(tp["atom_type_pair"]): list(tp["pair_params_indices"]) for tp in yml["type_pairs"]}
+    ft = yml["feature_type"]
+    spec = ModelSpec(
+        n_type=n_type,
+        cutoff=float(yml["cutoff"]),
+        feature_type=ft,
+        model_type=int(yml["model_type"]),
+        max_p=int(yml["max_p"]),
+        gtinv_order=int(yml["gtinv_order"]) if ft == "gtinv" else 0,
+        gtinv_maxl=tuple(yml["gtinv_maxl"]) if ft == "gtinv" else (),
+        pair_params=[list(map(float, p)) for p in yml["pair_params"]],
+        pair_params_conditional=cond,
+        elements=elements,
+        gtinv_version=int(yml.get("gtinv_version", 1)) if ft == "gtinv" else 1,
+    )
+    return spec, np.array(yml["coeffs"], dtype=float)
