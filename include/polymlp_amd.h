@@ -1,0 +1,141 @@
+/*
+ * polymlp_amd: C-ABI of the MI355X (gfx950) hot path of pypolymlp.
+ *
+ * Plain pointers and sizes only. Every function returns 0 on success and a
+ * non-zero code on failure; plm_last_error() then describes the failure
+ * (thread-local). Nothing here calls exit().
+ *
+ * Array conventions (as the reference's pybind11 module receives them):
+ *   axis   : 3x3 row-major, lattice vectors as columns (axis[i][col])
+ *   pos_c  : Cartesian positions, 3 x n row-major (component-major), i.e.
+ *            axis @ fractional positions, not rewrapped
+ *   types  : n int32 atom types (0-based, model element order)
+ * Batches concatenate these per structure: axes (n_st*9), pos (sum 3*n_s,
+ * each structure's 3 x n_s block contiguous), types (sum n_s), n_atoms (n_st).
+ */
+#ifndef POLYMLP_AMD_H
+#define POLYMLP_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLM_OK 0
+#define PLM_ERR_ARG 1
+#define PLM_ERR_MODEL 2
+#define PLM_ERR_DEVICE 3
+#define PLM_ERR_MEMORY 4
+#define PLM_ERR_SOLVE 5
+
+/* Precision of the feature stream / Gram accumulation. */
+#define PLM_FP64 0  /* plain FP64 features, FP64 MFMA X^T X (the drop-in form) */
+#define PLM_MIXED 1 /* FP32 features, split-bf16 x3 MFMA, FP64 block sums */
+
+typedef struct plm_model plm_model;
+typedef struct plm_batch plm_batch;
+typedef struct plm_normal plm_normal;
+
+/* Model description: the contents of the params dict the reference's
+ * convert_params_dict_to_feature_params reads
+ * (src/cxx/src/compute/py_params.cpp:11-60). */
+typedef struct plm_model_desc {
+    int n_type;
+    int feature_type;     /* 0 = pair, 1 = gtinv */
+    int model_type;       /* 1..4 */
+    int max_p;            /* 1..3 */
+    int max_l;
+    double cutoff;
+    int n_params;
+    const double* params; /* n_params x 2: (beta, mu) */
+    /* pair_params_conditional for type pairs (i <= j) in the order
+     * (0,0),(0,1),...,(0,n-1),(1,1),...: counts then concatenated indices */
+    const int* cond_count;
+    const int* cond_index;
+    /* invariant tables (Readgtinv output, lm = l*l + l + m) */
+    int n_lcomb;
+    const int* lcomb_order;   /* n_lcomb */
+    const int* lcomb_l;       /* sum(order) */
+    const int* lcomb_nterm;   /* n_lcomb */
+    const int* lm;            /* sum(order_i * nterm_i) */
+    const double* lm_coeff;   /* sum(nterm) */
+} plm_model_desc;
+
+const char* plm_last_error(void);
+int plm_device_count(int* n);
+
+/* Model setup: Mapping / ModelParams / Features / FeaturesPoly
+ * (src/cxx/src/polymlp/polymlp_features.cpp:27-59). Uploads the tables. */
+int plm_model_create(const plm_model_desc* desc, plm_model** out);
+void plm_model_destroy(plm_model* m);
+/* PolymlpAPI::get_n_variables (polymlp_api.cpp:190-197) */
+int plm_model_n_features(const plm_model* m, int* n);
+/* Per-type sizes: order-parameter components, non-conjugate ones, linear features. */
+int plm_model_type_sizes(const plm_model* m, int type, int* n_nlm, int* n_noconj, int* n_linear);
+/* Potential fold for evaluation (PolymlpEval ctor, polymlp_eval.cpp:12-20;
+ * Potential, polymlp_potential.cpp:25-156). coeffs = coeffs / scales. */
+int plm_model_set_coeffs(plm_model* m, const double* coeffs, int n);
+
+/* NeighborCell (src/cxx/src/compute/neighbor_cell.cpp:11-257): lattice
+ * translations (n_trans x 3) and the positions used for the search (3 x n,
+ * rewrapped only for reduced cells). Two-call: trans == NULL returns n_trans. */
+int plm_cell_translations(const double* axis, const double* pos_c, int n, double cutoff,
+                          double* trans, int max_trans, double* pos_rev, int* n_trans);
+
+/* NeighborFull / NeighborHalf (neighbor_full.cpp:10-76, neighbor_half.cpp:10-100)
+ * on the GPU. offset (n+1), neigh (M), trans_id (M), d (M x 3, r_j - r_i + T).
+ * Two-call: with capacity < M only n_pairs is set. */
+int plm_neighbor(const double* axis, const double* pos_c, int n, double cutoff, int half,
+                 int* offset, int* neigh, int* trans_id, double* d, long capacity, long* n_pairs);
+
+/* Device-resident batch of structures (positions, translations, full
+ * neighbour list). */
+int plm_batch_create(const plm_model* m, int n_st, const double* axes, const double* pos,
+                     const int* types, const int* n_atoms, plm_batch** out);
+void plm_batch_destroy(plm_batch* b);
+/* rows of X for the batch with forces and stresses: sum(7 + 3 n_s) */
+int plm_batch_n_rows(const plm_batch* b, long* rows);
+
+/* PotentialModel.get_x (py_model.cpp:10-54): X (rows x F, row-major) with the
+ * reference's row layout [n_st energy rows | 6 stress rows per structure |
+ * 3N force rows per structure], all structures with forces. precision =
+ * PLM_FP64 or PLM_MIXED (FP32 stream, written to FP64). */
+int plm_features_x(plm_model* m, plm_batch* b, int precision, double* x_out);
+
+/* Fused X + X^T X accumulation (data_sequential.py:25-155 with sum_xtx,
+ * utils_sequential.py:94-109). row_w / row_y: per-row weights and weighted
+ * targets in the batch's row order (plm_batch_row_order). X^T X accumulates
+ * (w x)(w x)^T over rows; X^T y accumulates (w x) * y; the raw (unweighted)
+ * energy rows feed xe_sum and xe_sq_sum as in the reference. */
+int plm_normal_create(const plm_model* m, plm_normal** out);
+void plm_normal_destroy(plm_normal* acc);
+int plm_normal_accumulate(plm_normal* acc, plm_batch* b, const double* row_w, const double* y_w,
+                          int precision);
+/* xtx: F x F full symmetric (row-major); xty, xe_sum, xe_sq_sum: F. */
+int plm_normal_result(plm_normal* acc, double* xtx, double* xty, double* xe_sum,
+                      double* xe_sq_sum, long* n_rows);
+/* Row permutation: for each row of the batch in the reference layout, the
+ * batch-internal row index (batch rows are [e, 6 s, 3N f] per structure). */
+int plm_batch_row_order(const plm_batch* b, long* order);
+
+/* Ridge regression on the normal equations (solvers.py:9-84): for each alpha
+ * (ascending), Cholesky of (xtx + alpha I) in FP64 on the GPU, solve, and one
+ * step of iterative refinement. xtx: F x F symmetric; coeffs: n_alpha x F.
+ * info[k] != 0 marks a failed factorization (the reference then returns
+ * coefficients of 1e30). */
+int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
+              double* coeffs, int* info);
+
+/* PotentialPropertiesFast.eval (py_properties_fast.cpp:19-28): energy,
+ * forces (n x 3) and stress (xx, yy, zz, xy, yz, zx) in eV/cell. */
+int plm_eval(plm_model* m, const double* axis, const double* pos_c, const int* types, int n,
+             int precision, double* energy, double* forces, double* stress);
+
+/* Evaluation of a device-resident batch (eval_multiple, py_properties_fast.cpp:30-77). */
+int plm_eval_batch(plm_model* m, plm_batch* b, int precision, double* energies, double* forces,
+                   double* stresses);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POLYMLP_AMD_H */

@@ -1,0 +1,291 @@
+"""ctypes binding of the C-ABI (include/polymlp_amd.h).
+
+The HIP library ``libpolymlp_amd.so`` is built in-tree by ``pypolymlp_amd.build``.
+Importing this module does not touch the GPU; device work starts at the first
+batch, neighbour-list or evaluation call. A missing library raises at load
+time: there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpolymlp_amd.so")
+
+PLM_FP64 = 0
+PLM_MIXED = 1
+
+EXPORTS = [
+    "plm_last_error", "plm_device_count", "plm_model_create", "plm_model_destroy",
+    "plm_model_n_features", "plm_model_type_sizes", "plm_model_set_coeffs",
+    "plm_cell_translations", "plm_neighbor", "plm_batch_create", "plm_batch_destroy",
+    "plm_batch_n_rows", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
+    "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
+    "plm_eval", "plm_eval_batch",
+]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_type", ctypes.c_int), ("feature_type", ctypes.c_int),
+        ("model_type", ctypes.c_int), ("max_p", ctypes.c_int), ("max_l", ctypes.c_int),
+        ("cutoff", ctypes.c_double), ("n_params", ctypes.c_int), ("params", _dp),
+        ("cond_count", _ip), ("cond_index", _ip), ("n_lcomb", ctypes.c_int),
+        ("lcomb_order", _ip), ("lcomb_l", _ip), ("lcomb_nterm", _ip), ("lm", _ip),
+        ("lm_coeff", _dp),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"HIP library not built: {LIB_PATH} (run python -m pypolymlp_amd.build)"
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.plm_last_error.restype = ctypes.c_char_p
+        L.plm_model_create.argtypes = [ctypes.POINTER(ModelDesc), ctypes.POINTER(vp)]
+        L.plm_model_destroy.argtypes = [vp]
+        L.plm_model_destroy.restype = None
+        L.plm_model_n_features.argtypes = [vp, _ip]
+        L.plm_model_type_sizes.argtypes = [vp, ctypes.c_int, _ip, _ip, _ip]
+        L.plm_model_set_coeffs.argtypes = [vp, _dp, ctypes.c_int]
+        L.plm_cell_translations.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_double, _dp,
+                                            ctypes.c_int, _dp, _ip]
+        L.plm_neighbor.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                   _ip, _ip, _ip, _dp, ctypes.c_long, _lp]
+        L.plm_batch_create.argtypes = [vp, ctypes.c_int, _dp, _dp, _ip, _ip, ctypes.POINTER(vp)]
+        L.plm_batch_destroy.argtypes = [vp]
+        L.plm_batch_destroy.restype = None
+        L.plm_batch_n_rows.argtypes = [vp, _lp]
+        L.plm_batch_row_order.argtypes = [vp, _lp]
+        L.plm_features_x.argtypes = [vp, vp, ctypes.c_int, _dp]
+        L.plm_normal_create.argtypes = [vp, ctypes.POINTER(vp)]
+        L.plm_normal_destroy.argtypes = [vp]
+        L.plm_normal_destroy.restype = None
+        L.plm_normal_accumulate.argtypes = [vp, vp, _dp, _dp, ctypes.c_int]
+        L.plm_normal_result.argtypes = [vp, _dp, _dp, _dp, _dp, _lp]
+        L.plm_ridge.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _ip]
+        L.plm_eval.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.plm_eval_batch.argtypes = [vp, vp, ctypes.c_int, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().plm_last_error().decode())
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _pd(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+class Model:
+    """plm_model built from a params dict (PolymlpParamsSingle.as_dict() schema)."""
+
+    def __init__(self, params_dict: dict):
+        m = params_dict["model"]
+        n_type = int(params_dict["n_type"])
+        keep = []
+        d = ModelDesc()
+        d.n_type = n_type
+        d.feature_type = 1 if m["feature_type"] == "gtinv" else 0
+        d.model_type = int(m["model_type"])
+        d.max_p = int(m["max_p"])
+        d.max_l = int(m["max_l"])
+        d.cutoff = float(m["cutoff"])
+        params = _d(np.array(m["pair_params"], dtype=float).reshape(-1, 2))
+        keep.append(params)
+        d.n_params = params.shape[0]
+        d.params = _pd(params)
+        counts, idx = [], []
+        cond = m["pair_params_conditional"]
+        for i in range(n_type):
+            for j in range(i, n_type):
+                v = cond[(i, j)] if (i, j) in cond else cond[str((i, j))]
+                counts.append(len(v))
+                idx.extend(int(x) for x in v)
+        cc, ci = _i(counts), _i(idx if idx else [0])
+        keep += [cc, ci]
+        d.cond_count, d.cond_index = _pi(cc), _pi(ci)
+        if d.feature_type == 1:
+            g = m["gtinv"]
+            l_comb, lm_seq, coeffs = g["l_comb"], g["lm_seq"], g["lm_coeffs"]
+            orders = _i([len(lc) for lc in l_comb])
+            l_flat = _i([v for lc in l_comb for v in lc])
+            nterm = _i([len(c) for c in coeffs])
+            lm_flat = _i([v for seq in lm_seq for row in seq for v in row])
+            c_flat = _d([v for c in coeffs for v in c])
+            keep += [orders, l_flat, nterm, lm_flat, c_flat]
+            d.n_lcomb = len(l_comb)
+            d.lcomb_order, d.lcomb_l, d.lcomb_nterm = _pi(orders), _pi(l_flat), _pi(nterm)
+            d.lm, d.lm_coeff = _pi(lm_flat), _pd(c_flat)
+        h = ctypes.c_void_p()
+        _check(lib().plm_model_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.n_type = n_type
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.plm_model_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n_features(self) -> int:
+        n = ctypes.c_int()
+        _check(lib().plm_model_n_features(self._h, ctypes.byref(n)))
+        return n.value
+
+    def type_sizes(self, t: int):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().plm_model_type_sizes(self._h, t, ctypes.byref(a), ctypes.byref(b),
+                                          ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_coeffs(self, coeffs):
+        c = _d(coeffs)
+        _check(lib().plm_model_set_coeffs(self._h, _pd(c), len(c)))
+
+
+def _pack_structures(axes: Sequence, positions_c: Sequence, types: Sequence):
+    axes_a = _d(np.stack([np.asarray(a, dtype=float) for a in axes]))
+    pos_a = _d(np.concatenate([_d(p).ravel() for p in positions_c]))
+    types_a = _i(np.concatenate([np.asarray(t) for t in types]))
+    n_atoms = _i([len(t) for t in types])
+    return axes_a, pos_a, types_a, n_atoms
+
+
+class Batch:
+    """Device-resident batch of structures with its full neighbour list."""
+
+    def __init__(self, model: Model, axes, positions_c, types):
+        self.model = model
+        a, p, t, n = _pack_structures(axes, positions_c, types)
+        self.n_atoms = n
+        h = ctypes.c_void_p()
+        _check(lib().plm_batch_create(model.handle, len(n), _pd(a), _pd(p), _pi(t), _pi(n),
+                                      ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.plm_batch_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n_rows(self) -> int:
+        r = ctypes.c_long()
+        _check(lib().plm_batch_n_rows(self._h, ctypes.byref(r)))
+        return r.value
+
+    def row_order(self) -> np.ndarray:
+        """Batch-internal row index of each row in the reference layout."""
+        o = np.zeros(self.n_rows, dtype=np.int64)
+        _check(lib().plm_batch_row_order(self._h, o.ctypes.data_as(_lp)))
+        return o
+
+
+def features_x(model: Model, batch: Batch, precision: int = PLM_FP64) -> np.ndarray:
+    """X (rows x F) in the reference's PotentialModel layout."""
+    X = np.zeros((batch.n_rows, model.n_features))
+    _check(lib().plm_features_x(model.handle, batch.handle, precision, _pd(X)))
+    return X
+
+
+class Normal:
+    """Device accumulator of X^T X, X^T y, xe_sum and xe_sq_sum."""
+
+    def __init__(self, model: Model):
+        self.model = model
+        h = ctypes.c_void_p()
+        _check(lib().plm_normal_create(model.handle, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.plm_normal_destroy(self._h)
+            self._h = None
+
+    def accumulate(self, batch: Batch, w_ref, y_ref, precision: int = PLM_FP64):
+        """w_ref / y_ref: weights and weighted targets in the reference row layout."""
+        order = batch.row_order()
+        w = np.zeros(batch.n_rows)
+        y = np.zeros(batch.n_rows)
+        w[order] = w_ref
+        y[order] = y_ref
+        _check(lib().plm_normal_accumulate(self._h, batch.handle, _pd(w), _pd(y), precision))
+
+    def result(self):
+        F = self.model.n_features
+        xtx = np.zeros((F, F))
+        xty = np.zeros(F)
+        s = np.zeros(F)
+        s2 = np.zeros(F)
+        n = ctypes.c_long()
+        _check(lib().plm_normal_result(self._h, _pd(xtx), _pd(xty), _pd(s), _pd(s2),
+                                       ctypes.byref(n)))
+        return xtx, xty, s, s2, n.value
+
+
+def cell_translations(axis, positions_c, cutoff: float):
+    axis, pos = _d(axis), _d(positions_c)
+    n = pos.shape[1]
+    nt = ctypes.c_int()
+    _check(lib().plm_cell_translations(_pd(axis), _pd(pos), n, cutoff, None, 0, None,
+                                       ctypes.byref(nt)))
+    tr = np.zeros((nt.value, 3))
+    prev = np.zeros((3, n))
+    _check(lib().plm_cell_translations(_pd(axis), _pd(pos), n, cutoff, _pd(tr), nt.value,
+                                       _pd(prev), ctypes.byref(nt)))
+    return tr, prev
+
+
+def neighbor(axis, positions_c, cutoff: float, half: bool = False):
+    """(offset[N+1], neigh[M], trans_id[M], d[M,3]) computed on the GPU."""
+    axis, pos = _d(axis), _d(positions_c)
+    n = pos.shape[1]
+    m = ctypes.c_long()
+    _check(lib().plm_neighbor(_pd(axis), _pd(pos), n, cutoff, int(half), None, None, None,
+                              None, 0, ctypes.byref(m)))
+    off = np.zeros(n + 1, dtype=np.int32)
+    nb = np.zeros(m.value, dtype=np.int32)
+    tid = np.zeros(m.value, dtype=np.int32)
+    d = np.zeros((m.value, 3))
+    _check(lib().plm_neighbor(_pd(axis), _pd(pos), n, cutoff, int(half), _pi(off), _pi(nb),
+                              _pi(tid), _pd(d), m.value, ctypes.byref(m)))
+    return off, nb, tid, d

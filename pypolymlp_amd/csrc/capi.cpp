@@ -1,0 +1,767 @@
+// C-ABI implementation: model and batch device tables, X rows, fused X^T X.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/polymlp_amd.h"
+#include "capi_internal.hpp"
+#include "cell.hpp"
+#include "kernels.hpp"
+#include "model.hpp"
+
+namespace plm {
+
+thread_local std::string g_last_error;
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr, bytes = 0;
+}
+void DevBuf::alloc(size_t n) {
+    if (n <= bytes && ptr) return;
+    release();
+    if (n == 0) n = 16;
+    check_hip(hipMalloc(&ptr, n), "hipMalloc");
+    bytes = n;
+}
+
+Device& device() {
+    static Device d;
+    if (!d.init) {
+        int n = 0;
+        check_hip(hipGetDeviceCount(&n), "hipGetDeviceCount");
+        if (n < 1) throw DeviceError("no HIP device");
+        check_hip(hipSetDevice(0), "hipSetDevice");
+        check_hip(hipStreamCreate(&d.stream), "hipStreamCreate");
+        d.init = true;
+    }
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// model tables
+// ---------------------------------------------------------------------------
+ModelHandle::ModelHandle(const ModelInput& in) : host(in) {}
+
+template <class V>
+const typename V::value_type* up(ModelHandle& h, const V& v) {
+    return h.store.upload(v.data(), v.size() * sizeof(typename V::value_type));
+}
+
+void ModelHandle::build_device() {
+    const Model& M = host;
+    const int nt = M.n_type(), ntp = M.n_tp(), nfn = M.n_fn();
+    const bool gt = M.input().gtinv;
+    md = ModelDev{};
+    md.n_type = nt, md.n_fn = nfn, md.n_tp = ntp;
+    md.max_l = gt ? M.input().max_l : -1;
+    md.n_lmh = gt ? M.n_lm_half() : 1;
+    md.cutoff = M.input().cutoff;
+    md.n_features = M.n_features();
+    md.n_linear = M.n_linear();
+    if (nt > kMaxTypes) throw std::invalid_argument("too many atom types");
+    if (md.n_lmh > kMaxLmHalf) throw std::invalid_argument("max_l too large");
+    for (int a = 0; a < nt; ++a)
+        for (int b = 0; b < nt; ++b) md.tp_of[a * nt + b] = M.type_pair(a, b);
+    std::vector<double> rad((size_t)ntp * nfn * 2, 0.0);
+    std::vector<int> rcount(ntp);
+    for (int tp = 0; tp < ntp; ++tp) {
+        const auto& ns = M.tp_to_n()[tp];
+        rcount[tp] = (int)ns.size();
+        for (size_t k = 0; k < ns.size(); ++k) {
+            rad[((size_t)tp * nfn + k) * 2] = M.input().params[ns[k]][0];
+            rad[((size_t)tp * nfn + k) * 2 + 1] = M.input().params[ns[k]][1];
+        }
+    }
+    md.rad = up(*this, rad);
+    md.rad_count = up(*this, rcount);
+    max_nlm = 0, max_noconj = 0;
+    for (int t = 0; t < nt; ++t) {
+        const auto& T = M.type(t);
+        TypeDev& D = md.t[t];
+        D = TypeDev{};
+        // order-parameter components
+        std::vector<int> src, tpo(ntp + 1, 0), head, nid, ylmk;
+        std::vector<float> sign;
+        std::vector<unsigned char> conj;
+        std::vector<std::vector<std::array<int, 3>>> by_tp(ntp);
+        if (gt) {
+            D.n_nlm = (int)T.nlm.size();
+            D.n_noconj = (int)T.noconj.size();
+            for (const auto& a : T.nlm) {
+                if (a.conj) {
+                    src.push_back(T.local_to_noconj[a.ilocal_conj]);
+                    sign.push_back((float)a.cc);
+                    conj.push_back(1);
+                } else {
+                    src.push_back(T.local_to_noconj[a.ilocal]);
+                    sign.push_back(1.f);
+                    conj.push_back(0);
+                }
+            }
+            for (size_t k = 0; k < T.noconj.size(); ++k) {
+                const auto& a = T.nlm[T.noconj[k]];
+                by_tp[a.tp].push_back({(int)k, a.n_id, a.ylm_key});
+            }
+        } else {
+            D.n_nlm = D.n_noconj = (int)T.ntp.size();
+            for (int k = 0; k < D.n_nlm; ++k) {
+                src.push_back(k), sign.push_back(1.f), conj.push_back(0);
+                by_tp[T.ntp[k][2]].push_back({k, T.ntp[k][1], -1});
+            }
+        }
+        for (int tp = 0; tp < ntp; ++tp) {
+            tpo[tp + 1] = tpo[tp] + (int)by_tp[tp].size();
+            for (const auto& e : by_tp[tp]) head.push_back(e[0]), nid.push_back(e[1]), ylmk.push_back(e[2]);
+        }
+        max_nlm = std::max(max_nlm, D.n_nlm);
+        max_noconj = std::max(max_noconj, D.n_noconj);
+        D.nlm_src = up(*this, src);
+        D.nlm_sign = up(*this, sign);
+        D.nlm_conj = up(*this, conj);
+        D.nc_tp_off = up(*this, tpo);
+        D.nc_tp_head = up(*this, head);
+        D.nc_tp_nid = up(*this, nid);
+        D.nc_tp_ylm = up(*this, ylmk);
+        // linear feature values
+        const int L = (int)T.features.size();
+        D.n_lin = L;
+        std::vector<int> fvo(L + 1, 0), fvk;
+        std::vector<double> fvc;
+        for (int f = 0; f < L; ++f) {
+            for (const auto& pc : T.feat_val[f]) {
+                const auto& keys = T.prod[pc.first];
+                if (keys.size() > 4) throw std::invalid_argument("invariant order above 4");
+                for (int k = 0; k < 4; ++k) fvk.push_back(k < (int)keys.size() ? keys[k] : -1);
+                fvc.push_back(pc.second);
+            }
+            fvo[f + 1] = (int)fvc.size();
+        }
+        D.fv_off = up(*this, fvo);
+        D.fv_coeff = up(*this, fvc);
+        D.fv_keys = up(*this, fvk);
+        // derivative weights W'[f, noconj head]
+        std::vector<int> wzo((size_t)L * ntp + 1, 0), wzn, wzy, wto(1, 0), wtk;
+        std::vector<double> wtc;
+        std::vector<unsigned char> wtj;
+        for (int f = 0; f < L; ++f) {
+            // (tp, head) -> terms
+            std::map<std::pair<int, int>, std::vector<std::tuple<double, int, unsigned char>>> nz;
+            for (const auto& d : T.feat_deriv[f]) {
+                int h0, tp;
+                double c = d.coeff;
+                unsigned char cj = 0;
+                if (gt) {
+                    const auto& a = T.nlm[d.head];
+                    if (a.conj) {
+                        h0 = T.local_to_noconj[a.ilocal_conj];
+                        c *= a.cc;
+                        cj = 1;
+                    } else {
+                        h0 = T.local_to_noconj[d.head];
+                    }
+                    tp = a.tp;
+                } else {
+                    h0 = d.head;
+                    tp = T.ntp[d.head][2];
+                }
+                nz[{tp, h0}].push_back({c, d.prod, cj});
+            }
+            int cur_tp = 0;
+            for (const auto& kv : nz) {
+                const int tp = kv.first.first, h0 = kv.first.second;
+                while (cur_tp < tp) wzo[(size_t)f * ntp + (++cur_tp)] = (int)wzn.size();
+                if (gt) {
+                    const auto& a = T.nlm[T.noconj[h0]];
+                    wzn.push_back(a.n_id), wzy.push_back(a.ylm_key);
+                } else {
+                    wzn.push_back(T.ntp[h0][1]), wzy.push_back(-1);
+                }
+                for (const auto& term : kv.second) {
+                    const auto& keys = T.prod_deriv[std::get<1>(term)];
+                    if (keys.size() > 3) throw std::invalid_argument("invariant order above 4");
+                    for (int k = 0; k < 3; ++k) wtk.push_back(k < (int)keys.size() ? keys[k] : -1);
+                    wtc.push_back(std::get<0>(term));
+                    wtj.push_back(std::get<2>(term));
+                }
+                wto.push_back((int)wtc.size());
+            }
+            while (cur_tp < ntp) wzo[(size_t)f * ntp + (++cur_tp)] = (int)wzn.size();
+        }
+        D.wz_off = up(*this, wzo);
+        D.wz_nid = up(*this, wzn);
+        D.wz_ylm = up(*this, wzy);
+        D.wz_term_off = up(*this, wto);
+        D.wt_coeff = up(*this, wtc);
+        D.wt_keys = up(*this, wtk);
+        D.wt_conj = up(*this, wtj);
+        D.lin_global = up(*this, T.linear_global);
+        host_wz_off.push_back(wzo);
+    }
+    built = true;
+}
+
+// Column recipes and per-(type, block) tables for a block width CB.
+const ColumnTables& ModelHandle::columns(int CB) {
+    auto it = col_cache.find(CB);
+    if (it != col_cache.end()) return *it->second;
+    const Model& M = host;
+    const int nt = M.n_type(), F = M.n_features(), nlin = M.n_linear(), ntp = M.n_tp();
+    auto ct = std::make_unique<ColumnTables>();
+    std::vector<unsigned char> nfac(F);
+    std::vector<std::array<int, 3>> base(F, {-1, -1, -1});
+    for (int c = 0; c < F; ++c) {
+        if (c < nlin) nfac[c] = 1, base[c] = {c, -1, -1};
+        else if (c < nlin + (int)M.comb2().size()) {
+            const auto& cb = M.comb2()[c - nlin];
+            nfac[c] = 2, base[c] = {cb[0], cb[1], -1};
+        } else {
+            const auto& cb = M.comb3()[c - nlin - M.comb2().size()];
+            nfac[c] = 3, base[c] = {cb[0], cb[1], cb[2]};
+        }
+    }
+    // global linear id -> local feature id per type
+    std::vector<std::vector<int>> g2l(nt, std::vector<int>(nlin, -1));
+    for (int t = 0; t < nt; ++t) {
+        const auto& lg = M.type(t).linear_global;
+        for (size_t k = 0; k < lg.size(); ++k) g2l[t][lg[k]] = (int)k;
+    }
+    const int nblk = (F + CB - 1) / CB;
+    std::vector<int> c0(nblk + 1);
+    for (int b = 0; b <= nblk; ++b) c0[b] = std::min(F, b * CB);
+    std::vector<int> need_off((size_t)nt * (nblk + 1), 0), need, col_need((size_t)nt * F * 3, -1);
+    std::vector<int> bw_off((size_t)nt * (nblk + 1), 0), bw_nz, need_wz;
+    int need_max = 1, wz_max = 1;
+    for (int t = 0; t < nt; ++t) {
+        const auto& wzo = host_wz_off[t];
+        for (int b = 0; b < nblk; ++b) {
+            need_off[(size_t)t * (nblk + 1) + b] = (int)need.size();
+            bw_off[(size_t)t * (nblk + 1) + b] = (int)bw_nz.size();
+            std::set<int> s;
+            for (int c = c0[b]; c < c0[b + 1]; ++c) {
+                bool ok = true;
+                for (int k = 0; k < nfac[c]; ++k) ok = ok && g2l[t][base[c][k]] >= 0;
+                if (!ok) continue;
+                for (int k = 0; k < nfac[c]; ++k) s.insert(g2l[t][base[c][k]]);
+            }
+            std::vector<int> lst(s.begin(), s.end());
+            std::map<int, int> pos;
+            int nzb = 0;
+            for (size_t k = 0; k < lst.size(); ++k) {
+                pos[lst[k]] = (int)k;
+                const int f = lst[k];
+                need.push_back(f);
+                need_wz.push_back(nzb);
+                for (int g = wzo[(size_t)f * ntp]; g < wzo[(size_t)(f + 1) * ntp]; ++g)
+                    bw_nz.push_back(g), ++nzb;
+            }
+            need_max = std::max(need_max, (int)lst.size());
+            wz_max = std::max(wz_max, nzb);
+            for (int c = c0[b]; c < c0[b + 1]; ++c) {
+                bool ok = true;
+                for (int k = 0; k < nfac[c]; ++k) ok = ok && g2l[t][base[c][k]] >= 0;
+                if (!ok) continue;
+                for (int k = 0; k < nfac[c]; ++k)
+                    col_need[((size_t)t * F + c) * 3 + k] = pos[g2l[t][base[c][k]]];
+            }
+        }
+        need_off[(size_t)t * (nblk + 1) + nblk] = (int)need.size();
+        bw_off[(size_t)t * (nblk + 1) + nblk] = (int)bw_nz.size();
+    }
+    ColumnsDev& cd = ct->cd;
+    cd.n_cols = F;
+    cd.nfac = ct->store.upload(nfac.data(), nfac.size());
+    std::vector<int> bl((size_t)nt * F * 3, -1);
+    for (int t = 0; t < nt; ++t)
+        for (int c = 0; c < F; ++c)
+            for (int k = 0; k < nfac[c]; ++k)
+                bl[((size_t)t * F + c) * 3 + k] = g2l[t][base[c][k]];
+    cd.base_local = ct->store.upload(bl.data(), bl.size() * 4);
+    cd.n_blocks = nblk;
+    cd.blk_c0 = ct->store.upload(c0.data(), c0.size() * 4);
+    cd.need_off = ct->store.upload(need_off.data(), need_off.size() * 4);
+    cd.need_feat = ct->store.upload(need.data(), need.size() * 4);
+    cd.col_need = ct->store.upload(col_need.data(), col_need.size() * 4);
+    ct->bw.off = ct->store.upload(bw_off.data(), bw_off.size() * 4);
+    ct->bw.nz = ct->store.upload(bw_nz.data(), bw_nz.size() * 4);
+    ct->bw.need_wz_off = ct->store.upload(need_wz.data(), need_wz.size() * 4);
+    ct->need_max = need_max;
+    ct->wz_max = wz_max;
+    ct->CB = CB;
+    auto& ref = *ct;
+    col_cache[CB] = std::move(ct);
+    return ref;
+}
+
+// ---------------------------------------------------------------------------
+// batch
+// ---------------------------------------------------------------------------
+BatchHandle::BatchHandle(ModelHandle& m, int n_st_, const double* axes, const double* pos,
+                         const int* types_in, const int* n_atoms_in)
+    : model(&m), n_st(n_st_) {
+    Device& dev = device();
+    atom_off.assign(n_st + 1, 0);
+    for (int s = 0; s < n_st; ++s) {
+        if (n_atoms_in[s] <= 0) throw std::invalid_argument("structure with no atoms");
+        atom_off[s + 1] = atom_off[s] + n_atoms_in[s];
+    }
+    n_atoms = atom_off[n_st];
+    row_off.assign(n_st + 1, 0);
+    for (int s = 0; s < n_st; ++s) row_off[s + 1] = row_off[s] + 7 + 3 * n_atoms_in[s];
+    std::vector<double> pos_dev((size_t)n_atoms * 3), trans;
+    std::vector<int> tr_off(n_st + 1, 0), atom_st(n_atoms);
+    const double cutoff = m.host.input().cutoff;
+    size_t poff = 0;
+    max_atoms = 0;
+    for (int s = 0; s < n_st; ++s) {
+        const int n = n_atoms_in[s];
+        max_atoms = std::max(max_atoms, n);
+        std::vector<double> tr, prev;
+        cell_translations(axes + 9 * (size_t)s, pos + poff, n, cutoff, tr, prev);
+        for (int j = 0; j < n; ++j)
+            for (int a = 0; a < 3; ++a) pos_dev[3 * (size_t)(atom_off[s] + j) + a] = prev[(size_t)a * n + j];
+        trans.insert(trans.end(), tr.begin(), tr.end());
+        tr_off[s + 1] = tr_off[s] + (int)(tr.size() / 3);
+        for (int j = 0; j < n; ++j) atom_st[atom_off[s] + j] = s;
+        poff += 3 * (size_t)n;
+    }
+    types.assign(types_in, types_in + n_atoms);
+    for (int t : types)
+        if (t < 0 || t >= m.host.n_type()) throw std::invalid_argument("atom type out of range");
+    d_types.alloc(n_atoms * 4);
+    d_pos.alloc(pos_dev.size() * 8);
+    d_trans.alloc(std::max<size_t>(trans.size(), 3) * 8);
+    d_tr_off.alloc((n_st + 1) * 4);
+    d_atom_st.alloc(n_atoms * 4);
+    d_atom_off.alloc((n_st + 1) * 4);
+    d_row_off.alloc((n_st + 1) * 4);
+    hipStream_t st = dev.stream;
+    check_hip(hipMemcpyAsync(d_types.ptr, types.data(), n_atoms * 4, hipMemcpyHostToDevice, st), "copy");
+    check_hip(hipMemcpyAsync(d_pos.ptr, pos_dev.data(), pos_dev.size() * 8, hipMemcpyHostToDevice, st), "copy");
+    if (!trans.empty())
+        check_hip(hipMemcpyAsync(d_trans.ptr, trans.data(), trans.size() * 8, hipMemcpyHostToDevice, st), "copy");
+    check_hip(hipMemcpyAsync(d_tr_off.ptr, tr_off.data(), (n_st + 1) * 4, hipMemcpyHostToDevice, st), "copy");
+    check_hip(hipMemcpyAsync(d_atom_st.ptr, atom_st.data(), n_atoms * 4, hipMemcpyHostToDevice, st), "copy");
+    check_hip(hipMemcpyAsync(d_atom_off.ptr, atom_off.data(), (n_st + 1) * 4, hipMemcpyHostToDevice, st), "copy");
+    check_hip(hipMemcpyAsync(d_row_off.ptr, row_off.data(), (n_st + 1) * 4, hipMemcpyHostToDevice, st), "copy");
+    // neighbour list: count, scan on host, fill
+    const double cut2 = cutoff * cutoff;
+    const double tol = 1e-10, tol2 = tol * tol;
+    DevBuf cnt;
+    cnt.alloc(n_atoms * 4);
+    launch_neighbor_full(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr, (double*)d_pos.ptr,
+                         (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2, tol2, (int*)cnt.ptr,
+                         nullptr, nullptr, nullptr, nullptr);
+    check_hip(hipGetLastError(), "neighbor count launch");
+    std::vector<int> c(n_atoms);
+    check_hip(hipMemcpyAsync(c.data(), cnt.ptr, n_atoms * 4, hipMemcpyDeviceToHost, st), "copy");
+    check_hip(hipStreamSynchronize(st), "sync");
+    nb_off.assign(n_atoms + 1, 0);
+    for (int i = 0; i < n_atoms; ++i) nb_off[i + 1] = nb_off[i] + c[i];
+    n_pairs = nb_off[n_atoms];
+    d_nb_off.alloc((n_atoms + 1) * 4);
+    d_nb_j.alloc(std::max<long>(n_pairs, 1) * 4);
+    d_nb_t.alloc(std::max<long>(n_pairs, 1) * 4);
+    d_nb_d.alloc(std::max<long>(n_pairs, 1) * 24);
+    check_hip(hipMemcpyAsync(d_nb_off.ptr, nb_off.data(), (n_atoms + 1) * 4, hipMemcpyHostToDevice, st), "copy");
+    launch_neighbor_full(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr, (double*)d_pos.ptr,
+                         (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2, tol2, nullptr,
+                         (int*)d_nb_off.ptr, (int*)d_nb_j.ptr, (int*)d_nb_t.ptr, (double*)d_nb_d.ptr);
+    check_hip(hipGetLastError(), "neighbor fill launch");
+    check_hip(hipStreamSynchronize(st), "sync");
+    max_pairs_atom = 0;
+    for (int i = 0; i < n_atoms; ++i) max_pairs_atom = std::max(max_pairs_atom, c[i]);
+}
+
+BatchDev BatchHandle::view() const {
+    BatchDev b;
+    b.n_st = n_st;
+    b.n_atoms_total = n_atoms;
+    b.st_atom_off = (const int*)d_atom_off.ptr;
+    b.st_row_off = (const int*)d_row_off.ptr;
+    b.types = (const int*)d_types.ptr;
+    b.pos = (const double*)d_pos.ptr;
+    b.nb_off = (const int*)d_nb_off.ptr;
+    b.nb_j = (const int*)d_nb_j.ptr;
+    b.nb_d = (const double*)d_nb_d.ptr;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// X build for a range of structures into a row-major buffer
+// ---------------------------------------------------------------------------
+static constexpr size_t kLdsBytes = 160 * 1024;
+
+XPlan plan_x(ModelHandle& m, int rows_max, bool fp64) {
+    const int elem = fp64 ? 8 : 4;
+    XPlan p{};
+    const int lmax = m.md.max_l;
+    for (int CB : {64, 48, 32, 24, 16, 8}) {
+        for (int NP : {16, 8, 4}) {
+            const auto& ct = m.columns(CB);
+            XBlockParams P{};
+            P.CB = CB, P.NP = NP, P.need_max = ct.need_max, P.wz_max = ct.wz_max;
+            P.rows_max = rows_max, P.lmax = lmax;
+            const size_t sm = xblock_smem(m.md, P, elem);
+            if (sm <= kLdsBytes - 1024) {
+                p.P = P;
+                p.ct = &ct;
+                p.smem = sm;
+                return p;
+            }
+        }
+    }
+    throw std::runtime_error("structure too large for the LDS row tile");
+}
+
+void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
+                  const double* roww, double* xe_sum, double* xe_sq, DevBuf& a_buf) {
+    Device& dev = device();
+    hipStream_t st = dev.stream;
+    const int a0 = b.atom_off[s0], a1 = b.atom_off[s1];
+    const int na = a1 - a0;
+    const int elem = fp64 ? 8 : 4;
+    const int a_stride = m.max_nlm;
+    a_buf.alloc((size_t)na * a_stride * 2 * elem);
+    BatchDev bd = b.view();
+    launch_anlm(st, fp64, m.md, bd, a_buf.ptr, a_stride, a0, na, m.max_noconj);
+    check_hip(hipGetLastError(), "anlm launch");
+    int rows_max = 0;
+    for (int s = s0; s < s1; ++s) rows_max = std::max(rows_max, 7 + 3 * (b.atom_off[s + 1] - b.atom_off[s]));
+    XPlan pl = plan_x(m, rows_max, fp64);
+    pl.P.ld = ld;
+    pl.P.st0 = s0;
+    pl.P.row_base = b.row_off[s0];
+    launch_xblock(st, fp64, m.md, pl.ct->cd, bd, pl.ct->bw, pl.P, s1 - s0, a_buf.ptr, a_stride, a0,
+                  X, roww, xe_sum, xe_sq);
+    check_hip(hipGetLastError(), "xblock launch");
+}
+
+}  // namespace plm
+
+using namespace plm;
+
+#define PLM_TRY(...)                                         \
+    try {                                                    \
+        __VA_ARGS__;                                         \
+        return PLM_OK;                                       \
+    } catch (const DeviceError& e) {                         \
+        g_last_error = e.what();                             \
+        return PLM_ERR_DEVICE;                               \
+    } catch (const std::bad_alloc& e) {                      \
+        g_last_error = "out of host memory";                 \
+        return PLM_ERR_MEMORY;                               \
+    } catch (const std::invalid_argument& e) {               \
+        g_last_error = e.what();                             \
+        return PLM_ERR_ARG;                                  \
+    } catch (const std::exception& e) {                      \
+        g_last_error = e.what();                             \
+        return PLM_ERR_MODEL;                                \
+    }
+
+extern "C" {
+
+const char* plm_last_error(void) { return g_last_error.c_str(); }
+
+int plm_device_count(int* n) {
+    PLM_TRY({
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+        *n = c;
+    });
+}
+
+int plm_model_create(const plm_model_desc* d, plm_model** out) {
+    PLM_TRY({
+        if (!d || !out) throw std::invalid_argument("null argument");
+        ModelInput in;
+        in.n_type = d->n_type;
+        in.gtinv = d->feature_type == 1;
+        in.model_type = d->model_type;
+        in.max_p = d->max_p;
+        in.max_l = d->max_l;
+        in.cutoff = d->cutoff;
+        for (int k = 0; k < d->n_params; ++k) in.params.push_back({d->params[2 * k], d->params[2 * k + 1]});
+        in.cond.assign(in.n_type, std::vector<std::vector<int>>(in.n_type));
+        int tp = 0, off = 0;
+        for (int i = 0; i < in.n_type; ++i)
+            for (int j = i; j < in.n_type; ++j, ++tp) {
+                const int c = d->cond_count[tp];
+                in.cond[i][j].assign(d->cond_index + off, d->cond_index + off + c);
+                off += c;
+            }
+        if (in.gtinv) {
+            int pl = 0, pm = 0, pc = 0;
+            for (int k = 0; k < d->n_lcomb; ++k) {
+                const int o = d->lcomb_order[k], nt = d->lcomb_nterm[k];
+                in.l_comb.emplace_back(d->lcomb_l + pl, d->lcomb_l + pl + o);
+                pl += o;
+                std::vector<std::vector<int>> lm(nt);
+                for (int t = 0; t < nt; ++t) lm[t].assign(d->lm + pm + t * o, d->lm + pm + (t + 1) * o);
+                pm += o * nt;
+                in.lm_seq.push_back(lm);
+                in.lm_coeffs.emplace_back(d->lm_coeff + pc, d->lm_coeff + pc + nt);
+                pc += nt;
+            }
+        }
+        auto h = std::make_unique<ModelHandle>(in);
+        *out = reinterpret_cast<plm_model*>(h.release());
+    });
+}
+
+void plm_model_destroy(plm_model* m) { delete reinterpret_cast<ModelHandle*>(m); }
+
+int plm_model_n_features(const plm_model* m, int* n) {
+    PLM_TRY({ *n = reinterpret_cast<const ModelHandle*>(m)->host.n_features(); });
+}
+
+int plm_model_type_sizes(const plm_model* m, int type, int* n_nlm, int* n_noconj, int* n_linear) {
+    PLM_TRY({
+        const auto& M = reinterpret_cast<const ModelHandle*>(m)->host;
+        if (type < 0 || type >= M.n_type()) throw std::invalid_argument("type out of range");
+        const auto& T = M.type(type);
+        *n_nlm = M.input().gtinv ? (int)T.nlm.size() : (int)T.ntp.size();
+        *n_noconj = M.input().gtinv ? (int)T.noconj.size() : (int)T.ntp.size();
+        *n_linear = (int)T.features.size();
+    });
+}
+
+int plm_cell_translations(const double* axis, const double* pos_c, int n, double cutoff,
+                          double* trans, int max_trans, double* pos_rev, int* n_trans) {
+    PLM_TRY({
+        std::vector<double> tr, prev;
+        cell_translations(axis, pos_c, n, cutoff, tr, prev);
+        *n_trans = (int)(tr.size() / 3);
+        if (trans) std::memcpy(trans, tr.data(), sizeof(double) * 3 * std::min(*n_trans, max_trans));
+        if (pos_rev) std::memcpy(pos_rev, prev.data(), sizeof(double) * prev.size());
+    });
+}
+
+int plm_neighbor(const double* axis, const double* pos_c, int n, double cutoff, int half,
+                 int* offset, int* neigh, int* trans_id, double* d, long capacity, long* n_pairs) {
+    PLM_TRY({
+        Device& dev = device();
+        hipStream_t st = dev.stream;
+        std::vector<double> tr, prev;
+        cell_translations(axis, pos_c, n, cutoff, tr, prev);
+        std::vector<double> pd((size_t)3 * n);
+        for (int j = 0; j < n; ++j)
+            for (int a = 0; a < 3; ++a) pd[3 * (size_t)j + a] = prev[(size_t)a * n + j];
+        const int ntr = (int)(tr.size() / 3);
+        DevBuf dpos, dtr, dto, dst, dao, cnt, doff, dj, dt, dd;
+        dpos.alloc(pd.size() * 8), dtr.alloc(std::max(1, ntr) * 24), dto.alloc(8);
+        dst.alloc((size_t)n * 4), dao.alloc(8), cnt.alloc((size_t)n * 4);
+        std::vector<int> zeros(n, 0), to = {0, ntr}, ao = {0, n};
+        check_hip(hipMemcpyAsync(dpos.ptr, pd.data(), pd.size() * 8, hipMemcpyHostToDevice, st), "copy");
+        if (ntr) check_hip(hipMemcpyAsync(dtr.ptr, tr.data(), tr.size() * 8, hipMemcpyHostToDevice, st), "copy");
+        check_hip(hipMemcpyAsync(dto.ptr, to.data(), 8, hipMemcpyHostToDevice, st), "copy");
+        check_hip(hipMemcpyAsync(dst.ptr, zeros.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "copy");
+        check_hip(hipMemcpyAsync(dao.ptr, ao.data(), 8, hipMemcpyHostToDevice, st), "copy");
+        const double cut2 = cutoff * cutoff, tol = 1e-10, tol2 = tol * tol;
+        auto run = [&](int* c, int* o, int* j, int* t, double* dv) {
+            if (half)
+                launch_neighbor_half(st, n, (int*)dst.ptr, (int*)dao.ptr, (double*)dpos.ptr, (int*)dto.ptr,
+                                     (double*)dtr.ptr, cut2, tol2, tol, c, o, j, t, dv);
+            else
+                launch_neighbor_full(st, n, (int*)dst.ptr, (int*)dao.ptr, (double*)dpos.ptr, (int*)dto.ptr,
+                                     (double*)dtr.ptr, cut2, tol2, c, o, j, t, dv);
+            check_hip(hipGetLastError(), "neighbor launch");
+        };
+        run((int*)cnt.ptr, nullptr, nullptr, nullptr, nullptr);
+        std::vector<int> c(n), off(n + 1, 0);
+        check_hip(hipMemcpyAsync(c.data(), cnt.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, st), "copy");
+        check_hip(hipStreamSynchronize(st), "sync");
+        for (int i = 0; i < n; ++i) off[i + 1] = off[i] + c[i];
+        const long M = off[n];
+        *n_pairs = M;
+        if (capacity < M || !offset) return PLM_OK;
+        doff.alloc((size_t)(n + 1) * 4), dj.alloc(std::max<long>(M, 1) * 4);
+        dt.alloc(std::max<long>(M, 1) * 4), dd.alloc(std::max<long>(M, 1) * 24);
+        check_hip(hipMemcpyAsync(doff.ptr, off.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice, st), "copy");
+        run(nullptr, (int*)doff.ptr, (int*)dj.ptr, (int*)dt.ptr, (double*)dd.ptr);
+        std::memcpy(offset, off.data(), (size_t)(n + 1) * 4);
+        if (neigh) check_hip(hipMemcpyAsync(neigh, dj.ptr, M * 4, hipMemcpyDeviceToHost, st), "copy");
+        if (trans_id) check_hip(hipMemcpyAsync(trans_id, dt.ptr, M * 4, hipMemcpyDeviceToHost, st), "copy");
+        if (d) check_hip(hipMemcpyAsync(d, dd.ptr, M * 24, hipMemcpyDeviceToHost, st), "copy");
+        check_hip(hipStreamSynchronize(st), "sync");
+    });
+}
+
+int plm_batch_create(const plm_model* m, int n_st, const double* axes, const double* pos,
+                     const int* types, const int* n_atoms, plm_batch** out) {
+    PLM_TRY({
+        if (!m || !out || n_st <= 0) throw std::invalid_argument("bad batch arguments");
+        auto* mh = const_cast<ModelHandle*>(reinterpret_cast<const ModelHandle*>(m));
+        if (!mh->built) mh->build_device();
+        auto b = std::make_unique<BatchHandle>(*mh, n_st, axes, pos, types, n_atoms);
+        *out = reinterpret_cast<plm_batch*>(b.release());
+    });
+}
+
+void plm_batch_destroy(plm_batch* b) { delete reinterpret_cast<BatchHandle*>(b); }
+
+int plm_batch_n_rows(const plm_batch* b, long* rows) {
+    PLM_TRY({ *rows = reinterpret_cast<const BatchHandle*>(b)->row_off.back(); });
+}
+
+int plm_batch_row_order(const plm_batch* bp, long* order) {
+    PLM_TRY({
+        const auto* b = reinterpret_cast<const BatchHandle*>(bp);
+        long r = 0;
+        for (int s = 0; s < b->n_st; ++s) order[r++] = b->row_off[s];
+        for (int s = 0; s < b->n_st; ++s)
+            for (int k = 0; k < 6; ++k) order[r++] = b->row_off[s] + 1 + k;
+        for (int s = 0; s < b->n_st; ++s) {
+            const int n = b->atom_off[s + 1] - b->atom_off[s];
+            for (int k = 0; k < 3 * n; ++k) order[r++] = b->row_off[s] + 7 + k;
+        }
+    });
+}
+
+int plm_features_x(plm_model* mp, plm_batch* bp, int precision, double* x_out) {
+    PLM_TRY({
+        auto* m = reinterpret_cast<ModelHandle*>(mp);
+        auto* b = reinterpret_cast<BatchHandle*>(bp);
+        if (!m->built) m->build_device();
+        const bool fp64 = precision == PLM_FP64;
+        const int F = m->host.n_features();
+        const long R = b->row_off.back();
+        const int elem = fp64 ? 8 : 4;
+        Device& dev = device();
+        hipStream_t st = dev.stream;
+        DevBuf xb, ab;
+        std::vector<double> hx((size_t)F);
+        std::vector<long> order(R);
+        plm_batch_row_order(bp, order.data());
+        std::vector<long> inv(R);
+        for (long r = 0; r < R; ++r) inv[order[r]] = r;
+        // process structures in chunks bounded by ~1 GiB of X
+        const size_t row_bytes = (size_t)F * elem;
+        int s0 = 0;
+        std::vector<char> hbuf;
+        while (s0 < b->n_st) {
+            int s1 = s0 + 1;
+            while (s1 < b->n_st && (size_t)(b->row_off[s1 + 1] - b->row_off[s0]) * row_bytes < (1ull << 30)) ++s1;
+            const long r0 = b->row_off[s0], r1 = b->row_off[s1];
+            xb.alloc((size_t)(r1 - r0) * row_bytes);
+            build_x_rows(*m, *b, s0, s1, fp64, xb.ptr, F, nullptr, nullptr, nullptr, ab);
+            hbuf.resize((size_t)(r1 - r0) * row_bytes);
+            check_hip(hipMemcpyAsync(hbuf.data(), xb.ptr, hbuf.size(), hipMemcpyDeviceToHost, st), "copy");
+            check_hip(hipStreamSynchronize(st), "sync");
+            for (long r = r0; r < r1; ++r) {
+                double* dst = x_out + (size_t)inv[r] * F;
+                if (fp64) std::memcpy(dst, hbuf.data() + (size_t)(r - r0) * row_bytes, row_bytes);
+                else {
+                    const float* src = reinterpret_cast<const float*>(hbuf.data() + (size_t)(r - r0) * row_bytes);
+                    for (int c = 0; c < F; ++c) dst[c] = src[c];
+                }
+            }
+            s0 = s1;
+        }
+    });
+}
+
+int plm_normal_create(const plm_model* mp, plm_normal** out) {
+    PLM_TRY({
+        auto* m = const_cast<ModelHandle*>(reinterpret_cast<const ModelHandle*>(mp));
+        if (!m->built) m->build_device();
+        auto acc = std::make_unique<NormalHandle>();
+        acc->model = m;
+        const int F = m->host.n_features();
+        acc->F = F;
+        acc->F_pad = (F + 127) / 128 * 128;
+        acc->xtx.alloc((size_t)acc->F_pad * acc->F_pad * 8);
+        acc->vec.alloc((size_t)acc->F_pad * 3 * 8);
+        Device& dev = device();
+        check_hip(hipMemsetAsync(acc->xtx.ptr, 0, acc->xtx.bytes, dev.stream), "memset");
+        check_hip(hipMemsetAsync(acc->vec.ptr, 0, acc->vec.bytes, dev.stream), "memset");
+        check_hip(hipStreamSynchronize(dev.stream), "sync");
+        *out = reinterpret_cast<plm_normal*>(acc.release());
+    });
+}
+
+void plm_normal_destroy(plm_normal* a) { delete reinterpret_cast<NormalHandle*>(a); }
+
+int plm_normal_accumulate(plm_normal* ap, plm_batch* bp, const double* row_w, const double* y_w,
+                          int precision) {
+    PLM_TRY({
+        auto* acc = reinterpret_cast<NormalHandle*>(ap);
+        auto* b = reinterpret_cast<BatchHandle*>(bp);
+        normal_accumulate(*acc, *b, row_w, y_w, precision == PLM_FP64);
+    });
+}
+
+int plm_normal_result(plm_normal* ap, double* xtx, double* xty, double* xe_sum, double* xe_sq_sum,
+                      long* n_rows) {
+    PLM_TRY({
+        auto* acc = reinterpret_cast<NormalHandle*>(ap);
+        Device& dev = device();
+        hipStream_t st = dev.stream;
+        const int F = acc->F, Fp = acc->F_pad;
+        check_hip(hipStreamSynchronize(st), "sync");
+        if (xtx) {
+            std::vector<double> full((size_t)Fp * Fp);
+            check_hip(hipMemcpy(full.data(), acc->xtx.ptr, full.size() * 8, hipMemcpyDeviceToHost), "copy");
+            // upper block triangle is valid; mirror by blocks of the Gram tile (128)
+            const int T = 128;
+            for (int i = 0; i < F; ++i)
+                for (int j = 0; j < F; ++j) {
+                    const bool upper = (i / T) <= (j / T);
+                    xtx[(size_t)i * F + j] = upper ? full[(size_t)i * Fp + j] : full[(size_t)j * Fp + i];
+                }
+        }
+        std::vector<double> v((size_t)Fp * 3);
+        check_hip(hipMemcpy(v.data(), acc->vec.ptr, v.size() * 8, hipMemcpyDeviceToHost), "copy");
+        if (xty) std::memcpy(xty, v.data(), (size_t)F * 8);
+        if (xe_sum) std::memcpy(xe_sum, v.data() + Fp, (size_t)F * 8);
+        if (xe_sq_sum) std::memcpy(xe_sq_sum, v.data() + 2 * (size_t)Fp, (size_t)F * 8);
+        if (n_rows) *n_rows = acc->n_rows;
+    });
+}
+
+int plm_model_set_coeffs(plm_model* mp, const double* coeffs, int n) {
+    PLM_TRY({
+        auto* m = reinterpret_cast<ModelHandle*>(mp);
+        m->host.set_potential(std::vector<double>(coeffs, coeffs + n));
+        if (!m->built) m->build_device();
+        upload_potential(*m);
+    });
+}
+
+int plm_eval(plm_model* mp, const double* axis, const double* pos_c, const int* types, int n,
+             int precision, double* energy, double* forces, double* stress) {
+    PLM_TRY({
+        auto* m = reinterpret_cast<ModelHandle*>(mp);
+        eval_single(*m, axis, pos_c, types, n, precision == PLM_FP64, energy, forces, stress);
+    });
+}
+
+int plm_eval_batch(plm_model* mp, plm_batch* bp, int precision, double* energies, double* forces,
+                   double* stresses) {
+    PLM_TRY({
+        auto* m = reinterpret_cast<ModelHandle*>(mp);
+        auto* b = reinterpret_cast<BatchHandle*>(bp);
+        eval_batch(*m, *b, precision == PLM_FP64, energies, forces, stresses);
+    });
+}
+
+int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
+              double* coeffs, int* info) {
+    PLM_TRY({ ridge_solve(F, xtx, xty, alphas, n_alpha, coeffs, info); });
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// Internal handles behind the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device_tables.hpp"
+#include "kernels.hpp"
+#include "model.hpp"
+
+namespace plm {
+
+extern thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check_hip(hipError_t e, const char* what);
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf();
+    void alloc(size_t n);
+    void release();
+};
+
+struct Device {
+    bool init = false;
+    hipStream_t stream = nullptr;
+};
+Device& device();
+
+// Owns many small device arrays uploaded once.
+struct Store {
+    std::vector<std::unique_ptr<DevBuf>> bufs;
+    template <class T>
+    const T* upload(const T* host, size_t bytes) {
+        auto b = std::make_unique<DevBuf>();
+        b->alloc(bytes ? bytes : 16);
+        if (bytes) check_hip(hipMemcpy(b->ptr, host, bytes, hipMemcpyHostToDevice), "upload");
+        const T* p = reinterpret_cast<const T*>(b->ptr);
+        bufs.push_back(std::move(b));
+        return p;
+    }
+};
+
+struct ColumnTables {
+    Store store;
+    ColumnsDev cd{};
+    BlockWz bw{};
+    int need_max = 0, wz_max = 0, CB = 0;
+};
+
+struct ModelHandle {
+    Model host;
+    Store store;
+    ModelDev md{};
+    bool built = false;
+    int max_nlm = 0, max_noconj = 0;
+    std::vector<std::vector<int>> host_wz_off;
+    std::map<int, std::unique_ptr<ColumnTables>> col_cache;
+    // evaluation tables
+    Store pot_store;
+    bool pot_uploaded = false;
+    explicit ModelHandle(const ModelInput& in);
+    void build_device();
+    const ColumnTables& columns(int CB);
+};
+
+struct BatchHandle {
+    ModelHandle* model;
+    int n_st = 0, n_atoms = 0, max_atoms = 0, max_pairs_atom = 0;
+    long n_pairs = 0;
+    std::vector<int> atom_off, row_off, types, nb_off;
+    DevBuf d_types, d_pos, d_trans, d_tr_off, d_atom_st, d_atom_off, d_row_off;
+    DevBuf d_nb_off, d_nb_j, d_nb_t, d_nb_d;
+    BatchHandle(ModelHandle& m, int n_st, const double* axes, const double* pos, const int* types,
+                const int* n_atoms);
+    BatchDev view() const;
+};
+
+struct XPlan {
+    XBlockParams P;
+    const ColumnTables* ct;
+    size_t smem;
+};
+XPlan plan_x(ModelHandle& m, int rows_max, bool fp64);
+
+void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
+                  const double* roww, double* xe_sum, double* xe_sq, DevBuf& a_buf);
+
+struct NormalHandle {
+    ModelHandle* model = nullptr;
+    int F = 0, F_pad = 0;
+    long n_rows = 0;
+    DevBuf xtx;   // F_pad x F_pad, upper block triangle accumulated
+    DevBuf vec;   // [xty | xe_sum | xe_sq_sum], each F_pad
+    DevBuf xbuf, abuf, wbuf, ybuf;
+};
+
+void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
+                       bool fp64);
+
+void upload_potential(ModelHandle& m);
+void eval_single(ModelHandle& m, const double* axis, const double* pos_c, const int* types, int n,
+                 bool fp64, double* energy, double* forces, double* stress);
+void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, double* forces,
+                double* stresses);
+void ridge_solve(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
+                 double* coeffs, int* info);
+
+}  // namespace plm

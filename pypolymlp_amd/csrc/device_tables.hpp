@@ -1,0 +1,92 @@
+// Plain-old-data views of the model and structure-batch tables in device
+// memory. Passed by value to kernels. All index arrays are int32.
+#pragma once
+
+#include <cstdint>
+
+namespace plm {
+
+constexpr int kMaxTypes = 8;
+constexpr int kMaxFn = 64;      // radial functions per type pair
+constexpr int kMaxLmHalf = 120; // (lmax+1)(lmax+2)/2 for lmax <= 14
+
+// Per-type tables (offsets into the flat arrays of ModelDev).
+struct TypeDev {
+    int n_nlm;        // local order-parameter components (incl. conjugates)
+    int n_noconj;     // non-conjugate components
+    int n_lin;        // linear features valid for this type
+    // a_full fill: local component k = conj ? cc * conj(a_nc[src]) : a_nc[src]
+    const int* nlm_src;        // [n_nlm] noconj index of the source
+    const float* nlm_sign;     // [n_nlm] +1 plain, cc for conjugates (sign of imag flips)
+    const unsigned char* nlm_conj;  // [n_nlm]
+    // anlm accumulation grouped by type pair: heads with tp == p
+    const int* nc_tp_off;      // [n_tp+1]
+    const int* nc_tp_head;     // noconj index
+    const int* nc_tp_nid;      // radial n_id
+    const int* nc_tp_ylm;      // ylm key
+    // feature values: x_f = sum coeff * Re(prod a[keys])
+    const int* fv_off;         // [n_lin+1]
+    const double* fv_coeff;
+    const int* fv_keys;        // [nterm*4], -1 padded
+    // derivative weights W'[f, h]: nonzeros sorted by (f, tp)
+    const int* wz_off;         // [n_lin*n_tp + 1]
+    const int* wz_nid;         // radial n_id of the head
+    const int* wz_ylm;         // ylm key of the head
+    const int* wz_term_off;    // [nnz+1]
+    const double* wt_coeff;    // folded coefficient (cc applied for conjugate heads)
+    const int* wt_keys;        // [nterm*3], -1 padded: product of all but the head
+    const unsigned char* wt_conj;  // product enters conjugated
+    const int* lin_global;     // [n_lin] global linear feature id
+    // eval: potential terms per noconj head
+    const int* pot_off;        // [n_noconj+1]
+    const double* pot_ce;
+    const double* pot_cf;
+    const int* pot_prod;       // index into prod_deriv
+    const int* pot_pf;         // index into prod_features
+    const int* pd_keys;        // [n_prod_deriv*3] -1 padded (eval products)
+    int n_prod_deriv;
+    const int* pf_keys;        // [n_prod_features*3] -1 padded: products of features
+    int n_prod_features;
+    // eval: noconj heads grouped by tp with (n_id, ylm, m==0, sign_j, jlocal)
+    const int* ev_jlocal;      // [n_noconj] jlocal_noconj of each noconj head (per tp list)
+    const float* ev_signj;     // [n_noconj]
+    const unsigned char* ev_m0;// [n_noconj]
+};
+
+struct ModelDev {
+    int n_type, n_fn, n_tp, max_l, n_lmh;
+    double cutoff;
+    int n_features, n_linear;
+    int tp_of[kMaxTypes * kMaxTypes];
+    const double* rad;         // [n_tp * n_fn * 2] (beta, mu) by (tp, n_id)
+    const int* rad_count;      // [n_tp] active radial functions
+    TypeDev t[kMaxTypes];
+};
+
+// Column recipes of X: column c = prod_{k < nfac} x_{base_k}.
+struct ColumnsDev {
+    int n_cols;
+    const unsigned char* nfac;        // [F]
+    const int* base_local;            // [n_type][F][3] local feature ids or -1 (invalid)
+    // column blocks
+    int n_blocks;
+    const int* blk_c0;                // [n_blocks+1] column ranges
+    const int* need_off;              // [n_type][n_blocks+1]
+    const int* need_feat;             // local feature ids needed by the block
+    const int* col_need;              // [n_type][F][3] index into the block's need list
+};
+
+// A batch of structures resident in device memory.
+struct BatchDev {
+    int n_st;
+    int n_atoms_total;
+    const int* st_atom_off;   // [n_st+1]
+    const int* st_row_off;    // [n_st+1] row offsets of X (7 + 3N per structure)
+    const int* types;         // [A]
+    const double* pos;        // [A*3] (possibly rewrapped) Cartesian positions
+    const int* nb_off;        // [A+1] full neighbour list
+    const int* nb_j;          // [M] global atom index
+    const double* nb_d;       // [M*3] r_j - r_i + T
+};
+
+}  // namespace plm

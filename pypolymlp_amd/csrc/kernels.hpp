@@ -1,0 +1,57 @@
+// Kernel launchers (host-callable) of the HIP hot path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_tables.hpp"
+
+namespace plm {
+
+struct XBlockParams;
+struct BlockWz;
+
+void launch_neighbor_full(hipStream_t st, int n, const int* atom_st, const int* st_atom_off,
+                          const double* pos, const int* tr_off, const double* trans, double cut2,
+                          double tol2, int* count, const int* nb_off, int* nb_j, int* nb_t,
+                          double* nb_d);
+void launch_neighbor_half(hipStream_t st, int n, const int* atom_st, const int* st_atom_off,
+                          const double* pos, const int* tr_off, const double* trans, double cut2,
+                          double tol2, double tol, int* count, const int* nb_off, int* nb_j,
+                          int* nb_t, double* nb_d);
+
+size_t anlm_smem(const ModelDev& md, int elem, int nthreads, int max_noconj);
+void launch_anlm(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, void* a_full,
+                 int a_stride, int atom0, int n_atoms, int max_noconj);
+
+struct XBlockParams {
+    int CB;
+    int NP;
+    int need_max;
+    int wz_max;
+    int rows_max;
+    int lmax;
+    int ld;
+    int st0;
+    int row_base;
+};
+
+struct BlockWz {
+    const int* off;
+    const int* nz;
+    const int* need_wz_off;
+};
+
+size_t xblock_smem(const ModelDev& md, const XBlockParams& P, int elem);
+void launch_xblock(hipStream_t st, bool fp64, const ModelDev& md, const ColumnsDev& cd,
+                   const BatchDev& bd, const BlockWz& bw, const XBlockParams& P, int n_st,
+                   const void* a_full, int a_stride, int atom_base, void* X, const double* roww,
+                   double* xe_sum, double* xe_sq);
+
+int gram_tile(bool fp64);
+int gram_kstep(bool fp64);
+void launch_gram(hipStream_t st, bool fp64, const void* X, int ld, int rows, int F_pad, double* C,
+                 int ldc, int chunk_rows);
+void launch_xty(hipStream_t st, bool fp64, const void* X, int ld, int rows, int F, const double* y,
+                double* xty);
+
+}  // namespace plm

@@ -1,0 +1,63 @@
+// Fused X -> X^T X accumulation over a device-resident batch.
+//
+// Structures are processed in sub-batches whose X rows (bounded to a few GiB)
+// are built by the column-block kernel straight into a row-major buffer with
+// the row weights applied, then folded into the FP64 Gram matrix and X^T y.
+// X for the whole dataset is never materialised. Mirrors the per-batch work of
+// _compute_products_single_batch (src/mlp_dev/core/data_sequential.py:97-155).
+#include <algorithm>
+#include <cstring>
+
+#include "capi_internal.hpp"
+
+namespace plm {
+
+static constexpr size_t kXBudget = 3ull << 30;   // bytes of X per sub-batch
+static constexpr size_t kABudget = 2ull << 30;   // bytes of order parameters per sub-batch
+static constexpr int kChunkRows = 4096;          // FP32 accumulation span of the bf16 Gram
+
+void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
+                       bool fp64) {
+    ModelHandle& m = *acc.model;
+    Device& dev = device();
+    hipStream_t st = dev.stream;
+    const int elem = fp64 ? 8 : 4;
+    const int F = acc.F, ld = acc.F_pad;
+    const int kstep = 32;
+    const size_t row_bytes = (size_t)ld * elem;
+    double* xty = reinterpret_cast<double*>(acc.vec.ptr);
+    double* xe_sum = xty + acc.F_pad;
+    double* xe_sq = xe_sum + acc.F_pad;
+    int s0 = 0;
+    while (s0 < b.n_st) {
+        int s1 = s0 + 1;
+        auto fits = [&](int s_end) {
+            const size_t rows = (size_t)(b.row_off[s_end] - b.row_off[s0]);
+            const size_t atoms = (size_t)(b.atom_off[s_end] - b.atom_off[s0]);
+            return (rows + kstep) * row_bytes <= kXBudget &&
+                   atoms * m.max_nlm * 2 * elem <= kABudget;
+        };
+        while (s1 < b.n_st && fits(s1 + 1)) ++s1;
+        const long r0 = b.row_off[s0], r1 = b.row_off[s1];
+        const long rows = r1 - r0;
+        const long rows_pad = (rows + kstep - 1) / kstep * kstep;
+        acc.xbuf.alloc((size_t)rows_pad * row_bytes);
+        acc.wbuf.alloc((size_t)rows * 8);
+        acc.ybuf.alloc((size_t)rows * 8);
+        check_hip(hipMemsetAsync(acc.xbuf.ptr, 0, (size_t)rows_pad * row_bytes, st), "memset");
+        check_hip(hipMemcpyAsync(acc.wbuf.ptr, row_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
+        check_hip(hipMemcpyAsync(acc.ybuf.ptr, y_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
+        build_x_rows(m, b, s0, s1, fp64, acc.xbuf.ptr, ld, (const double*)acc.wbuf.ptr, xe_sum, xe_sq,
+                     acc.abuf);
+        launch_gram(st, fp64, acc.xbuf.ptr, ld, (int)rows_pad, acc.F_pad, (double*)acc.xtx.ptr,
+                    acc.F_pad, kChunkRows);
+        check_hip(hipGetLastError(), "gram launch");
+        launch_xty(st, fp64, acc.xbuf.ptr, ld, (int)rows, F, (const double*)acc.ybuf.ptr, xty);
+        check_hip(hipGetLastError(), "xty launch");
+        acc.n_rows += rows;
+        s0 = s1;
+    }
+    check_hip(hipStreamSynchronize(st), "sync");
+}
+
+}  // namespace plm

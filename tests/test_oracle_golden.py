@@ -1,0 +1,103 @@
+"""The oracle (reference C++ built from sources, oracle/_ref) against the
+golden vectors held by the reference's own tests.
+
+Sources of the expected values:
+  tests/test_calc/test_properties_MgO.py:9-95      MgO energies, forces, stresses
+  tests/test_calc/test_compute_features.py:45-90   MgO feature sums
+  tests/test_cxx/test_neighbor.py:62-120           rocksalt half neighbour lists
+  tests/test_cxx/test_gtinv.py:10-38               invariant tables
+"""
+
+import numpy as np
+import pytest
+
+from oracle import ref
+from pypolymlp_amd.gtinv import get_gtinv_attrs
+from pypolymlp_amd.structure import volume_gpa_factor
+from tests.helpers import poscar, ref_file
+
+pytestmark = pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+
+
+MGO_PAIR_F = [
+    [-0.03962958, -0.01188776, -0.07928375], [-0.00459307, 0.00331611, 0.02210267],
+    [0.01105381, -0.00137797, 0.022104], [0.01105096, 0.00331545, -0.00918516],
+    [0.00978188, 0.00177061, 0.01180386], [0.00590284, 0.00293358, 0.01180553],
+    [0.00589926, 0.00176979, 0.01958533], [0.0005339, 0.00016018, 0.00106752],
+]
+MGO_PAIR_S = [-0.17379186, -0.17521681, -0.16909401, 0.00004465, 0.00008926, 0.00029751]
+MGO_GTINV_F = [
+    [-0.03882777, -0.0116472, -0.07768031], [-0.00302382, 0.00324809, 0.02165033],
+    [0.01082712, -0.00090718, 0.02165147], [0.01082468, 0.00324753, -0.00604679],
+    [0.00925154, 0.00182258, 0.01215086], [0.00607581, 0.00277448, 0.01215188],
+    [0.00607362, 0.00182207, 0.0185246], [-0.00120118, -0.00036037, -0.00240204],
+]
+MGO_GTINV_S = [-2.78575100e-02, -2.91029804e-02, -2.37508486e-02, 1.32744242e-05,
+               2.65114887e-05, 8.83338671e-05]
+
+
+@pytest.mark.parametrize(
+    "model,e_true,f_true,s_true",
+    [
+        ("polymlp.yaml.pair.MgO", -40.22469744315832, MGO_PAIR_F, MGO_PAIR_S),
+        ("polymlp.yaml.gtinv.MgO", -40.223320043232334, MGO_GTINV_F, MGO_GTINV_S),
+    ],
+)
+def test_oracle_eval_mgo(model, e_true, f_true, s_true):
+    st = poscar("POSCAR.RS.MgO")
+    m = ref.RefModel(ref_file(model), with_eval=True)
+    e, f, s = m.eval(st.axis, st.positions_cartesian, st.types)
+    assert e == pytest.approx(e_true, rel=1e-8)
+    np.testing.assert_allclose(f, f_true, atol=1e-6)
+    np.testing.assert_allclose(s * volume_gpa_factor(st), s_true, atol=1e-5)
+
+
+@pytest.mark.parametrize(
+    "model,F,total,diff",
+    [
+        ("polymlp.yaml.pair.MgO", 324, 997193.0146734761, -6.428600229143143),
+        ("polymlp.yaml.gtinv.MgO", 1899, 237100.3979091199, -1.9717588279337908),
+    ],
+)
+def test_oracle_features_mgo(model, F, total, diff):
+    sts = [poscar("POSCAR-00001.MgO"), poscar("POSCAR-00002.MgO")]
+    m = ref.RefModel(ref_file(model))
+    x = m.x_batch([s.axis for s in sts], [s.positions_cartesian for s in sts],
+                  [s.types for s in sts], force=False)
+    assert x.shape == (2, F)
+    assert np.sum(x) == pytest.approx(total, rel=1e-6)
+    assert np.sum(x[0] - x[1]) == pytest.approx(diff, rel=1e-6)
+    xf = m.x_batch([s.axis for s in sts], [s.positions_cartesian for s in sts],
+                   [s.types for s in sts], force=True)
+    assert xf.shape == (398, F)
+    # energy rows of the force path (compute_anlmtp_d) and the energy-only path
+    # (compute_anlmtp) agree to rounding
+    np.testing.assert_allclose(xf[:2], x, rtol=1e-10, atol=0)
+
+
+def test_oracle_neighbor_half_rocksalt():
+    st = poscar("POSCAR-rocksalt")
+    off, nb, d = ref.neighbor_list(st.axis, st.positions_cartesian, 6.0, half=True)
+    sizes = np.diff(off)
+    assert list(sizes) == [9, 21, 33, 45, 47, 59, 71, 83]
+    sums = [int(nb[off[i]:off[i + 1]].sum()) for i in range(8)]
+    assert sums == [0, 9, 30, 63, 90, 149, 220, 303]
+    np.testing.assert_allclose(d[off[2] + 5], [-2.0, 4.0, 2.0])
+
+
+def test_gtinv_tables_match_oracle_and_golden():
+    l_comb, lm_seq, coeffs = get_gtinv_attrs(3, [12, 12])
+    assert len(l_comb) == 154
+    assert tuple(l_comb[23]) == (0, 9, 9)
+    assert tuple(l_comb[145]) == (8, 12, 12)
+    assert tuple(lm_seq[45][2]) == (4, 51, 97)
+    assert coeffs[45][2] == pytest.approx(-0.176928656723561)
+    assert coeffs[90][2] == pytest.approx(0.113929892405108)
+    assert (l_comb, lm_seq, coeffs) == ref.gtinv(3, [12, 12])
+    l6, lm6, c6 = get_gtinv_attrs(6, [16, 12, 8, 1, 1])
+    assert len(l6) == 254
+    assert tuple(l6[145]) == (8, 8, 12)
+    assert tuple(lm6[45][2]) == (4, 27, 61)
+    assert c6[45][2] == pytest.approx(-0.18156825980064)
+    for order, maxl in [(2, [4]), (3, [8, 4]), (4, [8, 4, 2])]:
+        assert get_gtinv_attrs(order, maxl) == ref.gtinv(order, maxl)
