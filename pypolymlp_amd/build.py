@@ -33,7 +33,10 @@ SOURCES = [
     ("cell.cpp", ["-ffp-contract=off"]),
     ("capi.cpp", []),
     ("normal.cpp", []),
-    ("eval_stub.cpp", []),
+    ("ridge.cpp", []),
+    ("kernels_solve.hip", []),
+    ("eval.cpp", []),
+    ("kernels_eval.hip", []),
     ("kernels_x.hip", []),
     ("kernels_gram.hip", []),
 ]

@@ -289,3 +289,41 @@ def neighbor(axis, positions_c, cutoff: float, half: bool = False):
     _check(lib().plm_neighbor(_pd(axis), _pd(pos), n, cutoff, int(half), _pi(off), _pi(nb),
                               _pi(tid), _pd(d), m.value, ctypes.byref(m)))
     return off, nb, tid, d
+
+
+def eval_structure(model: Model, axis, positions_c, types, precision: int = PLM_FP64):
+    """(energy, forces (N, 3), stress (6,)) of one structure (PotentialPropertiesFast.eval)."""
+    axis, pos = _d(axis), _d(positions_c)
+    t = _i(types)
+    n = len(t)
+    e = np.zeros(1)
+    f = np.zeros((n, 3))
+    s = np.zeros(6)
+    _check(lib().plm_eval(model.handle, _pd(axis), _pd(pos), _pi(t), n, precision, _pd(e), _pd(f),
+                          _pd(s)))
+    return float(e[0]), f, s
+
+
+def eval_batch(model: Model, batch: Batch, precision: int = PLM_FP64):
+    """(energies (n_st,), forces list of (N_s, 3), stresses (n_st, 6)) of a batch."""
+    n_st = len(batch.n_atoms)
+    A = int(batch.n_atoms.sum())
+    e = np.zeros(n_st)
+    f = np.zeros((A, 3))
+    s = np.zeros((n_st, 6))
+    _check(lib().plm_eval_batch(model.handle, batch.handle, precision, _pd(e), _pd(f), _pd(s)))
+    off = np.concatenate([[0], np.cumsum(batch.n_atoms)])
+    return e, [f[off[k]:off[k + 1]] for k in range(n_st)], s
+
+
+def ridge(xtx, xty, alphas) -> np.ndarray:
+    """Coefficients (n_alpha x F) of (xtx + alpha I) c = xty on the GPU (plm_ridge)."""
+    A = _d(xtx)
+    b = _d(xty)
+    al = _d(alphas)
+    F = A.shape[0]
+    out = np.zeros((len(al), F))
+    info = np.zeros(len(al), dtype=np.int32)
+    _check(lib().plm_ridge(F, _pd(A), _pd(b), _pd(al), len(al), _pd(out), _pi(info)))
+    out[info != 0] = 1e30
+    return out

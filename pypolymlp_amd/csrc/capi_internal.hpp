@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "device_tables.hpp"
+#include "eval_tables.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
 
@@ -72,6 +73,8 @@ struct ModelHandle {
     // evaluation tables
     Store pot_store;
     bool pot_uploaded = false;
+    EvalDev ed{};
+    int max_heads = 0, max_lin = 0;
     explicit ModelHandle(const ModelInput& in);
     void build_device();
     const ColumnTables& columns(int CB);
@@ -84,9 +87,14 @@ struct BatchHandle {
     std::vector<int> atom_off, row_off, types, nb_off;
     DevBuf d_types, d_pos, d_trans, d_tr_off, d_atom_st, d_atom_off, d_row_off;
     DevBuf d_nb_off, d_nb_j, d_nb_t, d_nb_d;
+    // half neighbour list (evaluation), built on first use
+    bool has_half = false;
+    long n_half = 0;
+    DevBuf d_hb_off, d_hb_j, d_hb_t, d_hb_d;
     BatchHandle(ModelHandle& m, int n_st, const double* axes, const double* pos, const int* types,
                 const int* n_atoms);
     BatchDev view() const;
+    void ensure_half();
 };
 
 struct XPlan {

@@ -1,0 +1,144 @@
+// Evaluation (PolymlpEval) orchestration: potential tables, half neighbour
+// list, order parameters, per-atom prod_sum and the half-pair loop.
+#include <algorithm>
+#include <cstring>
+
+#include "capi_internal.hpp"
+
+namespace plm {
+
+void upload_potential(ModelHandle& m) {
+    const Model& M = m.host;
+    if (!M.has_potential()) throw std::runtime_error("coefficients not set");
+    m.pot_store.bufs.clear();
+    const int nt = M.n_type(), ntp = M.n_tp();
+    const bool gt = M.input().gtinv;
+    m.max_heads = 0, m.max_lin = 0;
+    for (int t = 0; t < nt; ++t) {
+        const auto& T = M.type(t);
+        EvalTypeDev& E = m.ed.t[t];
+        E.n_heads = (int)T.pot.size();
+        m.max_heads = std::max(m.max_heads, E.n_heads);
+        m.max_lin = std::max(m.max_lin, (int)T.features.size());
+        std::vector<int> off(E.n_heads + 1, 0), pd, pf;
+        std::vector<double> ce, cf;
+        for (int h = 0; h < E.n_heads; ++h) {
+            for (const auto& pt : T.pot[h]) {
+                const auto& k3 = T.prod_deriv[pt.prod];
+                const auto& k2 = T.prod_features[pt.prod_features];
+                if (k3.size() > 3 || k2.size() > 2) throw std::invalid_argument("term order too high");
+                for (int a = 0; a < 3; ++a) pd.push_back(a < (int)k3.size() ? k3[a] : -1);
+                for (int a = 0; a < 2; ++a) pf.push_back(a < (int)k2.size() ? k2[a] : -1);
+                ce.push_back(pt.coeff_e), cf.push_back(pt.coeff_f);
+            }
+            off[h + 1] = (int)ce.size();
+        }
+        std::vector<std::vector<std::array<int, 6>>> by_tp(ntp);
+        if (gt) {
+            for (size_t k = 0; k < T.noconj.size(); ++k) {
+                const auto& a = T.nlm[T.noconj[k]];
+                by_tp[a.tp].push_back({a.n_id, a.ylm_key, (int)k, a.jlocal_noconj,
+                                       a.sign_j < 0 ? -1 : 1, a.m == 0 ? 1 : 0});
+            }
+        } else {
+            for (size_t k = 0; k < T.ntp.size(); ++k)
+                by_tp[T.ntp[k][2]].push_back({T.ntp[k][1], -1, (int)k, T.ntp[k][3], 1, 0});
+        }
+        std::vector<int> hoff(ntp + 1, 0), nid, ylm, hi, hj;
+        std::vector<float> sj;
+        std::vector<unsigned char> m0;
+        for (int tp = 0; tp < ntp; ++tp) {
+            for (const auto& e : by_tp[tp]) {
+                nid.push_back(e[0]), ylm.push_back(e[1]), hi.push_back(e[2]), hj.push_back(e[3]);
+                sj.push_back((float)e[4]), m0.push_back((unsigned char)e[5]);
+            }
+            hoff[tp + 1] = (int)nid.size();
+        }
+        auto upv = [&](const auto& v) {
+            using V = typename std::decay_t<decltype(v)>::value_type;
+            return m.pot_store.upload(v.data(), v.size() * sizeof(V));
+        };
+        E.pot_off = upv(off), E.pot_ce = upv(ce), E.pot_cf = upv(cf);
+        E.pot_pd = upv(pd), E.pot_pf = upv(pf);
+        E.hd_off = upv(hoff), E.hd_nid = upv(nid), E.hd_ylm = upv(ylm);
+        E.hd_i = upv(hi), E.hd_j = upv(hj), E.hd_signj = upv(sj), E.hd_m0 = upv(m0);
+    }
+    m.pot_uploaded = true;
+}
+
+void BatchHandle::ensure_half() {
+    if (has_half) return;
+    Device& dev = device();
+    hipStream_t st = dev.stream;
+    const double cutoff = model->host.input().cutoff;
+    const double cut2 = cutoff * cutoff, tol = 1e-10, tol2 = tol * tol;
+    DevBuf cnt;
+    cnt.alloc((size_t)n_atoms * 4);
+    launch_neighbor_half(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr, (double*)d_pos.ptr,
+                         (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2, tol2, tol, (int*)cnt.ptr,
+                         nullptr, nullptr, nullptr, nullptr);
+    check_hip(hipGetLastError(), "half count launch");
+    std::vector<int> c(n_atoms), off(n_atoms + 1, 0);
+    check_hip(hipMemcpyAsync(c.data(), cnt.ptr, (size_t)n_atoms * 4, hipMemcpyDeviceToHost, st), "copy");
+    check_hip(hipStreamSynchronize(st), "sync");
+    for (int i = 0; i < n_atoms; ++i) off[i + 1] = off[i] + c[i];
+    n_half = off[n_atoms];
+    d_hb_off.alloc((size_t)(n_atoms + 1) * 4);
+    d_hb_j.alloc(std::max<long>(n_half, 1) * 4);
+    d_hb_t.alloc(std::max<long>(n_half, 1) * 4);
+    d_hb_d.alloc(std::max<long>(n_half, 1) * 24);
+    check_hip(hipMemcpyAsync(d_hb_off.ptr, off.data(), (size_t)(n_atoms + 1) * 4, hipMemcpyHostToDevice, st), "copy");
+    launch_neighbor_half(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr, (double*)d_pos.ptr,
+                         (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2, tol2, tol, nullptr,
+                         (int*)d_hb_off.ptr, (int*)d_hb_j.ptr, (int*)d_hb_t.ptr, (double*)d_hb_d.ptr);
+    check_hip(hipGetLastError(), "half fill launch");
+    check_hip(hipStreamSynchronize(st), "sync");
+    has_half = true;
+}
+
+void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, double* forces,
+                double* stresses) {
+    if (!m.built) m.build_device();
+    if (!m.pot_uploaded) upload_potential(m);
+    b.ensure_half();
+    Device& dev = device();
+    hipStream_t st = dev.stream;
+    const int elem = fp64 ? 8 : 4;
+    const int A = b.n_atoms;
+    DevBuf a_buf, pse, psf, out;
+    a_buf.alloc((size_t)A * m.max_nlm * 2 * elem);
+    pse.alloc((size_t)A * std::max(1, m.max_heads) * 2 * elem);
+    psf.alloc((size_t)A * std::max(1, m.max_heads) * 2 * elem);
+    const size_t n_out = (size_t)b.n_st + 3 * (size_t)A + 6 * (size_t)b.n_st;
+    out.alloc(n_out * 8);
+    check_hip(hipMemsetAsync(out.ptr, 0, n_out * 8, st), "memset");
+    double* d_e = (double*)out.ptr;
+    double* d_f = d_e + b.n_st;
+    double* d_s = d_f + 3 * (size_t)A;
+    BatchDev bd = b.view();
+    const int pair_model = m.host.input().gtinv ? 0 : 1;
+    launch_anlm(st, fp64, m.md, bd, a_buf.ptr, m.max_nlm, 0, A, m.max_noconj, 1);
+    check_hip(hipGetLastError(), "eval anlm launch");
+    launch_prodsum(st, fp64, m.md, m.ed, bd, a_buf.ptr, m.max_nlm, pse.ptr, psf.ptr,
+                   std::max(1, m.max_heads), m.max_lin, pair_model);
+    check_hip(hipGetLastError(), "prodsum launch");
+    launch_pair(st, fp64, m.md, m.ed, bd, (const int*)b.d_atom_st.ptr, (const int*)b.d_hb_off.ptr,
+                (const int*)b.d_hb_j.ptr, (const double*)b.d_hb_d.ptr, pse.ptr, psf.ptr,
+                std::max(1, m.max_heads), d_e, d_f, d_s, pair_model);
+    check_hip(hipGetLastError(), "pair launch");
+    std::vector<double> h(n_out);
+    check_hip(hipMemcpyAsync(h.data(), out.ptr, n_out * 8, hipMemcpyDeviceToHost, st), "copy");
+    check_hip(hipStreamSynchronize(st), "sync");
+    if (energies) std::memcpy(energies, h.data(), (size_t)b.n_st * 8);
+    if (forces) std::memcpy(forces, h.data() + b.n_st, 3 * (size_t)A * 8);
+    if (stresses) std::memcpy(stresses, h.data() + b.n_st + 3 * (size_t)A, 6 * (size_t)b.n_st * 8);
+}
+
+void eval_single(ModelHandle& m, const double* axis, const double* pos_c, const int* types, int n,
+                 bool fp64, double* energy, double* forces, double* stress) {
+    if (!m.built) m.build_device();
+    BatchHandle b(m, 1, axis, pos_c, types, &n);
+    eval_batch(m, b, fp64, energy, forces, stress);
+}
+
+}  // namespace plm

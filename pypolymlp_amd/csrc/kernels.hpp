@@ -21,7 +21,7 @@ void launch_neighbor_half(hipStream_t st, int n, const int* atom_st, const int* 
 
 size_t anlm_smem(const ModelDev& md, int elem, int nthreads, int max_noconj);
 void launch_anlm(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, void* a_full,
-                 int a_stride, int atom0, int n_atoms, int max_noconj);
+                 int a_stride, int atom0, int n_atoms, int max_noconj, int negate = 0);
 
 struct XBlockParams {
     int CB;
@@ -46,6 +46,15 @@ void launch_xblock(hipStream_t st, bool fp64, const ModelDev& md, const ColumnsD
                    const BatchDev& bd, const BlockWz& bw, const XBlockParams& P, int n_st,
                    const void* a_full, int a_stride, int atom_base, void* X, const double* roww,
                    double* xe_sum, double* xe_sq);
+
+struct EvalDev;
+void launch_prodsum(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,
+                    const BatchDev& bd, const void* a_full, int a_stride, void* ps_e, void* ps_f,
+                    int ps_stride, int max_lin, int pair_model);
+void launch_pair(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,
+                 const BatchDev& bd, const int* atom_st, const int* hb_off, const int* hb_j,
+                 const double* hb_d, const void* ps_e, const void* ps_f, int ps_stride,
+                 double* energy, double* forces, double* stress, int pair_model);
 
 int gram_tile(bool fp64);
 int gram_kstep(bool fp64);

@@ -181,7 +181,7 @@ constexpr int kAnlmPairs = 16;
 template <class T>
 __global__ void __launch_bounds__(256) anlm_kernel(ModelDev md, BatchDev bd,
                                                    Cpx<T>* __restrict__ a_full, int a_stride,
-                                                   int atom0, int n_atoms) {
+                                                   int atom0, int n_atoms, int negate) {
     extern __shared__ unsigned char smem_raw[];
     const int ia = blockIdx.x;
     if (ia >= n_atoms) return;
@@ -202,8 +202,12 @@ __global__ void __launch_bounds__(256) anlm_kernel(ModelDev md, BatchDev bd,
             const int k = c0 + threadIdx.x;
             const int j = bd.nb_j[k];
             const int tp = md.tp_of[t1 * md.n_type + bd.types[j]];
-            compute_pair<T>(md, L, pairs + (size_t)threadIdx.x * L.stride, bd.nb_d + 3 * (size_t)k,
-                            tp, 0, false, lmax);
+            // X path: d = r_j - r_i + T (local.cpp:149); evaluation: the
+            // reversed vector r_i - r_j - T (polymlp_eval.cpp:376-379)
+            const double* dk = bd.nb_d + 3 * (size_t)k;
+            const double sg = negate ? -1.0 : 1.0;
+            const double dd[3] = {sg * dk[0], sg * dk[1], sg * dk[2]};
+            compute_pair<T>(md, L, pairs + (size_t)threadIdx.x * L.stride, dd, tp, 0, false, lmax);
         }
         __syncthreads();
         for (int p = 0; p < np; ++p) {
@@ -496,19 +500,19 @@ size_t anlm_smem(const ModelDev& md, int elem, int nthreads, int max_noconj) {
 
 template <class T>
 void launch_anlm_t(hipStream_t st, const ModelDev& md, const BatchDev& bd, void* a_full,
-                   int a_stride, int atom0, int n_atoms, int max_noconj) {
+                   int a_stride, int atom0, int n_atoms, int max_noconj, int negate) {
     const int B = 64;
     const size_t sm = anlm_smem(md, sizeof(T), kAnlmPairs, max_noconj);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_kernel<T>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     anlm_kernel<T><<<n_atoms, B, sm, st>>>(md, bd, reinterpret_cast<Cpx<T>*>(a_full), a_stride,
-                                           atom0, n_atoms);
+                                           atom0, n_atoms, negate);
 }
 
 void launch_anlm(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, void* a_full,
-                 int a_stride, int atom0, int n_atoms, int max_noconj) {
-    if (fp64) launch_anlm_t<double>(st, md, bd, a_full, a_stride, atom0, n_atoms, max_noconj);
-    else launch_anlm_t<float>(st, md, bd, a_full, a_stride, atom0, n_atoms, max_noconj);
+                 int a_stride, int atom0, int n_atoms, int max_noconj, int negate) {
+    if (fp64) launch_anlm_t<double>(st, md, bd, a_full, a_stride, atom0, n_atoms, max_noconj, negate);
+    else launch_anlm_t<float>(st, md, bd, a_full, a_stride, atom0, n_atoms, max_noconj, negate);
 }
 
 size_t xblock_smem(const ModelDev& md, const XBlockParams& P, int elem) {

@@ -1,0 +1,221 @@
+"""Regression on the GPU: weights, fused X^T X / X^T y, scales, ridge, RMSE.
+
+Host-side mirror of pypolymlp's sequential regression:
+  calc_xtx_xty                 src/pypolymlp/mlp_dev/core/data_sequential.py:25-94
+  _compute_products_single_batch                                     :97-155
+  apply_weights and helpers    src/pypolymlp/mlp_dev/core/utils_weights.py:10-98
+  compute_scales / round_scales src/pypolymlp/mlp_dev/core/utils_scales.py:6-40
+  solver_ridge                 src/pypolymlp/mlp_dev/standard/solvers.py:48-84
+  compute_mse / compute_rmse   src/pypolymlp/mlp_dev/core/utils_model_selection.py:37-89
+  fit                          src/pypolymlp/mlp_dev/standard/fit.py:12-63
+
+The per-row weights and weighted targets are computed here exactly as the
+reference computes them; X, X^T X, X^T y and the energy-row sums come from
+the HIP library (pypolymlp_amd.capi); the ridge solve runs on the GPU
+(plm_ridge) when available and otherwise through LAPACK posv like the
+reference.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import capi
+from .datasets import Dataset
+
+
+# --- weights (utils_weights.py) ---------------------------------------------
+def _weight_energy(energy, total_n_atoms, min_e=None):
+    e_per_atom = energy / total_n_atoms
+    if min_e is None:
+        min_e = np.min(e_per_atom)
+    e_th1, e_th2 = min_e * 0.75, min_e * 0.50
+    w = np.ones(len(energy))
+    w[e_per_atom > e_th1] = 0.5
+    w[e_per_atom > e_th2] = 0.3
+    w[e_per_atom > 0.0] = 0.1
+    return w
+
+
+def _weight_force(forces, tol=1e-12):
+    w = np.abs(forces)
+    w[w < tol] = tol
+    w = np.reciprocal(w)
+    w[w > 1.0] = 1.0
+    return w
+
+
+def _weight_stress(stress, weight_stress, tol=1e-12):
+    nonzero = np.abs(stress) > tol
+    log1 = np.ones(len(stress)) * np.log10(tol)
+    log1[nonzero] = np.log10(np.abs(stress)[nonzero])
+    w = np.power(5, -log1)
+    w[w > 1.0] = 1.0
+    w *= weight_stress
+    return w
+
+
+def row_weights(ds: Dataset, min_e: float, weight_stress: float = 0.1):
+    """(w, y_w) over the rows of one dataset batch in the reference layout
+    [energies | stresses (6 per structure) | forces (3N per structure)]."""
+    ne = len(ds.energies)
+    n_s = 6 * ne
+    forces = ds.forces_flat
+    stresses = ds.stresses_flat
+    n = ne + n_s + len(forces)
+    w = np.ones(n)
+    y = np.zeros(n)
+    we = _weight_energy(ds.energies, ds.total_n_atoms, min_e=min_e) * ds.weight
+    w[:ne] = we
+    y[:ne] = we * ds.energies
+    if ds.include_force:
+        wf = _weight_force(forces) * ds.weight
+        w[ne + n_s:] = wf
+        y[ne + n_s:] = wf * forces
+    if ds.include_stress:
+        ws = _weight_stress(stresses, weight_stress * ds.weight)
+        w[ne:ne + n_s] = ws
+        y[ne:ne + n_s] = ws * stresses
+    elif ds.include_force:
+        w[ne:ne + n_s] = 0.0
+        y[ne:ne + n_s] = 0.0
+    return w, y
+
+
+# --- scales (utils_scales.py) ------------------------------------------------
+def round_scales(scales, include_force=True, threshold=1e-10):
+    zero = np.abs(scales) < (threshold if include_force else threshold * threshold)
+    scales[zero] = 1.0
+    return scales, zero
+
+
+def compute_scales(scales, xe_sum, xe_sq_sum, n_data, include_force=True, threshold=1e-10):
+    if scales is None:
+        var = xe_sq_sum / n_data - np.square(xe_sum / n_data)
+        var[var < 0.0] = 1.0
+        scales = np.sqrt(var)
+    return round_scales(scales, include_force=include_force, threshold=threshold)
+
+
+def get_min_energy(datasets: Sequence[Dataset]) -> float:
+    return float(min(np.min(d.energies / d.total_n_atoms) for d in datasets))
+
+
+@dataclass
+class NormalEquations:
+    xtx: np.ndarray
+    xty: np.ndarray
+    y_sq_norm: float
+    total_n_data: int
+    scales: np.ndarray
+    min_energy: float
+    xe_sum: Optional[np.ndarray] = None
+    xe_sq_sum: Optional[np.ndarray] = None
+
+
+def calc_xtx_xty(model: capi.Model, datasets: Sequence[Dataset], precision=capi.PLM_FP64,
+                 scales: Optional[np.ndarray] = None, min_energy: Optional[float] = None,
+                 weight_stress: float = 0.1, batch_size: int = 4096,
+                 scale_threshold: float = 1e-10) -> NormalEquations:
+    """Fused X -> (X^T X, X^T y) on the GPU over datasets in structure batches."""
+    if min_energy is None:
+        min_energy = get_min_energy(datasets)
+    acc = capi.Normal(model)
+    y_sq = 0.0
+    for ds in datasets:
+        ds.sort_by_atoms()
+        n = len(ds.structures)
+        for b0 in range(0, n, batch_size):
+            b1 = min(n, b0 + batch_size)
+            sub = Dataset(structures=ds.structures[b0:b1], energies=ds.energies[b0:b1],
+                          forces=ds.forces[b0:b1], stresses=ds.stresses[b0:b1], name=ds.name,
+                          weight=ds.weight, include_force=ds.include_force,
+                          include_stress=ds.include_stress)
+            w, yw = row_weights(sub, min_energy, weight_stress)
+            sts = sub.structures
+            batch = capi.Batch(model, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                               [s.types for s in sts])
+            acc.accumulate(batch, w, yw, precision)
+            y_sq += float(yw @ yw)
+    xtx, xty, xe_sum, xe_sq, n_rows = acc.result()
+    n_data = sum(len(d.energies) for d in datasets)
+    scales, zero = compute_scales(scales, xe_sum, xe_sq, n_data,
+                                  include_force=datasets[0].include_force,
+                                  threshold=scale_threshold)
+    xtx[zero] = 0.0
+    xtx[:, zero] = 0.0
+    xty[zero] = 0.0
+    xtx /= scales[:, None]
+    xtx /= scales[None, :]
+    xty /= scales
+    return NormalEquations(xtx=xtx, xty=xty, y_sq_norm=y_sq, total_n_data=n_rows, scales=scales,
+                           min_energy=min_energy, xe_sum=xe_sum, xe_sq_sum=xe_sq)
+
+
+def solve_linear_equation(A, b):
+    """LAPACK posv (upper) as solvers.py:9-45; 1e30 on a failed factorization."""
+    from scipy.linalg.lapack import get_lapack_funcs
+
+    (posv,) = get_lapack_funcs(("posv",), (A, b))
+    _, x, info = posv(A, b, lower=False, overwrite_a=False, overwrite_b=False)
+    if info != 0:
+        x = np.ones(x.shape) * 1e30
+    return x
+
+
+def solver_ridge(xtx, xty, alphas, use_gpu: bool = True):
+    """Coefficients (F x n_alpha) for each alpha (solvers.py:48-84)."""
+    F = xtx.shape[0]
+    if use_gpu:
+        try:
+            return capi.ridge(xtx, xty, alphas).T
+        except RuntimeError as e:
+            if "not available" not in str(e):
+                raise
+    coefs = np.zeros((F, len(alphas)))
+    A = xtx.copy()
+    prev = 0.0
+    for i, a in enumerate(alphas):
+        A.flat[:: F + 1] += a - prev
+        coefs[:, i] = solve_linear_equation(A, xty)
+        prev = a
+    return coefs
+
+
+def compute_rmse(coefs_array, ne: NormalEquations):
+    out = []
+    for c in coefs_array.T:
+        mse = (c @ (ne.xtx @ c) - 2 * c @ ne.xty + ne.y_sq_norm) / ne.total_n_data
+        out.append(np.sqrt(mse) if mse >= 0 else 1e10)
+    return np.array(out)
+
+
+@dataclass
+class FitResult:
+    coeffs: np.ndarray
+    scales: np.ndarray
+    alpha: float
+    rmse_train: float
+    rmse_test: float
+
+    @property
+    def coeffs_unscaled(self):
+        """Coefficients as stored in polymlp.yaml and passed to evaluation."""
+        return self.coeffs / self.scales
+
+
+def fit(model: capi.Model, train: List[Dataset], test: List[Dataset], alphas,
+        precision=capi.PLM_FP64, use_gpu_solver: bool = True) -> FitResult:
+    """Sequential ridge fit with model selection on the test RMSE (fit.py:12-63)."""
+    tr = calc_xtx_xty(model, train, precision=precision)
+    coefs = solver_ridge(tr.xtx, tr.xty, alphas, use_gpu=use_gpu_solver)
+    rmse_train = compute_rmse(coefs, tr)
+    te = calc_xtx_xty(model, test, precision=precision, scales=tr.scales.copy(),
+                      min_energy=tr.min_energy)
+    rmse_test = compute_rmse(coefs, te)
+    k = int(np.argmin(rmse_test))
+    return FitResult(coeffs=coefs[:, k], scales=tr.scales, alpha=float(alphas[k]),
+                     rmse_train=float(rmse_train[k]), rmse_test=float(rmse_test[k]))

@@ -1,0 +1,81 @@
+"""GPU evaluation (PolymlpEval) against the oracle and the reference's golden values.
+
+Golden: tests/test_calc/test_properties_MgO.py:9-95 (energy rel 1e-8, forces
+atol 1e-6 eV/A, stress atol 1e-5 GPa). Oracle: PolymlpEval compiled from the
+reference sources, energy rel 1e-10, forces and stress normwise 1e-10.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import ref
+from pypolymlp_amd import capi, datasets
+from pypolymlp_amd.structure import volume_gpa_factor
+from tests.helpers import load_model, poscar, ref_file, write_model_yaml
+from tests.test_oracle_golden import MGO_GTINV_F, MGO_GTINV_S, MGO_PAIR_F, MGO_PAIR_S
+
+pytestmark = [pytest.mark.gpu]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize(
+    "model,e_true,f_true,s_true",
+    [
+        ("polymlp.yaml.pair.MgO", -40.22469744315832, MGO_PAIR_F, MGO_PAIR_S),
+        ("polymlp.yaml.gtinv.MgO", -40.223320043232334, MGO_GTINV_F, MGO_GTINV_S),
+    ],
+)
+def test_eval_mgo_golden(model, e_true, f_true, s_true):
+    spec, coeffs = load_model(model)
+    m = capi.Model(spec.as_dict())
+    m.set_coeffs(coeffs)
+    st = poscar("POSCAR.RS.MgO")
+    say(f"eval model={model}")
+    e, f, s = capi.eval_structure(m, st.axis, st.positions_cartesian, st.types)
+    assert e == pytest.approx(e_true, rel=1e-8)
+    np.testing.assert_allclose(f, f_true, atol=1e-6)
+    np.testing.assert_allclose(s * volume_gpa_factor(st), s_true, atol=1e-5)
+    if ref.available():
+        er, fr, sr = ref.RefModel(ref_file(model), with_eval=True).eval(
+            st.axis, st.positions_cartesian, st.types)
+        assert e == pytest.approx(er, rel=1e-10)
+        assert np.linalg.norm(f - fr) <= 1e-10 * np.linalg.norm(fr)
+        assert np.linalg.norm(s - sr) <= 1e-10 * np.linalg.norm(sr)
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+@pytest.mark.parametrize("cfg", ["cfg1", "cfg2"])
+def test_eval_configs_vs_oracle(cfg, tmp_path):
+    if cfg == "cfg1":
+        spec, ds = datasets.config1_model(), datasets.config1_dataset(3)
+    else:
+        spec, ds = datasets.config2_model(), datasets.config2_dataset(2, targets=False)
+    m = capi.Model(spec.as_dict())
+    rng = np.random.default_rng(7)
+    coeffs = rng.normal(size=m.n_features) * 1e-3
+    m.set_coeffs(coeffs)
+    path = write_model_yaml(spec, str(tmp_path / "m.yaml"), coeffs)
+    R = ref.RefModel(path, with_eval=True)
+    sts = ds.structures
+    b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                   [s.types for s in sts])
+    say(f"eval batch {cfg} FP64")
+    E, F, S = capi.eval_batch(m, b, capi.PLM_FP64)
+    say(f"eval batch {cfg} MIXED")
+    Em, Fm, Sm = capi.eval_batch(m, b, capi.PLM_MIXED)
+    for k, st in enumerate(sts):
+        er, fr, sr = R.eval(st.axis, st.positions_cartesian, st.types)
+        assert E[k] == pytest.approx(er, rel=1e-10)
+        assert np.linalg.norm(F[k] - fr) <= 1e-10 * np.linalg.norm(fr)
+        assert np.linalg.norm(S[k] - sr) <= 1e-10 * np.linalg.norm(sr)
+        n = st.n_total
+        say(f"  mixed: dE/atom {abs(Em[k] - er) / n * 1e3:.3e} meV, "
+            f"max dF {np.abs(Fm[k] - fr).max() * 1e3:.3e} meV/A")
+        assert abs(Em[k] - er) / n < 1e-4
+        assert np.abs(Fm[k] - fr).max() < 1e-3

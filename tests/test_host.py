@@ -1,0 +1,100 @@
+"""Host-side logic on the CPU: the C-ABI library, model setup and lattice
+translations against the oracle, and the regression numerics against closed
+forms. No GPU work is launched here."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import ref
+from pypolymlp_amd import capi, datasets
+from pypolymlp_amd import mlp_dev
+from tests.helpers import load_model, random_cell_structure, write_model_yaml
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_header_symbol():
+    header = open(os.path.join(ROOT, "include", "polymlp_amd.h")).read()
+    declared = set(re.findall(r"\b(plm_[a-z_]+)\s*\(", header))
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared == set(capi.EXPORTS)
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+@pytest.mark.parametrize("name", ["polymlp.yaml.gtinv.MgO", "polymlp.yaml.pair.MgO",
+                                  "polymlp.yaml.gtinv.Al", "polymlp.yaml.MgO"])
+def test_feature_count_reference_models(name):
+    spec, coeffs = load_model(name)
+    m = capi.Model(spec.as_dict())
+    assert m.n_features == len(coeffs)
+    assert m.n_features == ref.RefModel(os.path.join(ROOT, "tests", "golden", "ref_files", name)).n_features
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+@pytest.mark.parametrize("cfg,F", [("1", 2277), ("2", 13020), ("3", 25428)])
+def test_feature_count_configs(cfg, F, tmp_path):
+    spec = getattr(datasets, f"config{cfg}_model")()
+    m = capi.Model(spec.as_dict())
+    assert m.n_features == F
+    path = write_model_yaml(spec, str(tmp_path / "m.yaml"), np.zeros(F))
+    assert ref.RefModel(path).n_features == F
+
+
+def test_feature_count_known_answers():
+    # PolymlpAPI n_variables of the MgO models (tests/test_cxx/test_polymlp_api.py:17,
+    # tests/test_cxx/test_polymlp_api_pair.py:19) and the per-type layout of the
+    # synthetic-anlm test (450 components, 270 non-conjugate, 891 features).
+    spec, _ = load_model("polymlp.yaml.MgO")
+    m = capi.Model(spec.as_dict())
+    assert m.n_features == 1899
+    assert m.type_sizes(0) == (450, 270, 891)
+    assert m.type_sizes(1) == (450, 270, 891)
+    spec, _ = load_model("polymlp.yaml.pair.MgO")
+    assert capi.Model(spec.as_dict()).n_features == 324
+
+
+def test_invalid_model_is_an_error_not_an_exit():
+    spec = datasets.config1_model()
+    d = spec.as_dict()
+    d["model"]["max_p"] = 4
+    with pytest.raises(RuntimeError, match="max_p"):
+        capi.Model(d)
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+def test_cell_translations_bit_exact():
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        ax, pos, _ = random_cell_structure(rng, n_atoms=5, skew=2.5)
+        rc = float(rng.uniform(4.0, 8.0))
+        t1, p1 = capi.cell_translations(ax, pos, rc)
+        t2, p2, _ = ref.neighbor_cell(ax, pos, rc)
+        assert np.array_equal(t1, t2)
+        assert np.array_equal(p1, p2)
+
+
+def test_weights_closed_forms():
+    ds = datasets.Dataset(structures=[], energies=np.array([-5.0, -2.0, 1.0]), forces=[],
+                          stresses=np.zeros((3, 6)))
+    # thresholds 0.75 * min_e = -3.75 and 0.5 * min_e = -2.5 (utils_weights.py:10-22)
+    we = mlp_dev._weight_energy(np.array([-5.0, -3.0, -2.0, 1.0]), np.ones(4), min_e=-5.0)
+    assert list(we) == [1.0, 0.5, 0.3, 0.1]
+    wf = mlp_dev._weight_force(np.array([0.5, 2.0, 0.0, -4.0]))
+    np.testing.assert_allclose(wf, [1.0, 0.5, 1.0, 0.25])
+    ws = mlp_dev._weight_stress(np.array([0.01, 10.0, 100.0]), 0.1)
+    np.testing.assert_allclose(ws, [0.1, 0.1 * 5 ** -1, 0.1 * 5 ** -2])
+    del ds
+
+
+def test_scales_closed_form_and_zero_rule():
+    xe = np.array([[1.0, 2.0, 5.0], [3.0, 2.0, 5.0 + 1e-12]])
+    s, zero = mlp_dev.compute_scales(None, xe.sum(0), (xe ** 2).sum(0), 2)
+    np.testing.assert_allclose(s[0], 1.0)
+    assert list(zero) == [False, True, True]
+    assert s[1] == 1.0 and s[2] == 1.0
