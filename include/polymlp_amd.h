@@ -134,6 +134,14 @@ int plm_eval(plm_model* m, const double* axis, const double* pos_c, const int* t
 int plm_eval_batch(plm_model* m, plm_batch* b, int precision, double* energies, double* forces,
                    double* stresses);
 
+/* Instrumentation (no counterpart in the reference): per-kernel-class timing
+ * with HIP events recorded on the library's own stream, for roofline
+ * figures. Names: "neighbor", "anlm", "xblock", "gram", "xty", "eval_anlm",
+ * "eval_prodsum", "eval_pair", "cholesky", "chol_solve". */
+int plm_profile_enable(int on);
+int plm_profile_reset(void);
+int plm_profile_get(const char* name, double* total_ms, long* launches);
+
 #ifdef __cplusplus
 }
 #endif

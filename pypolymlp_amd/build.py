@@ -34,6 +34,7 @@ SOURCES = [
     ("capi.cpp", []),
     ("normal.cpp", []),
     ("ridge.cpp", []),
+    ("profile.cpp", []),
     ("kernels_solve.hip", []),
     ("eval.cpp", []),
     ("kernels_eval.hip", []),

@@ -26,7 +26,7 @@ EXPORTS = [
     "plm_cell_translations", "plm_neighbor", "plm_batch_create", "plm_batch_destroy",
     "plm_batch_n_rows", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
-    "plm_eval", "plm_eval_batch",
+    "plm_eval", "plm_eval_batch", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -82,6 +82,8 @@ def lib():
         L.plm_ridge.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _ip]
         L.plm_eval.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.plm_eval_batch.argtypes = [vp, vp, ctypes.c_int, _dp, _dp, _dp]
+        L.plm_profile_enable.argtypes = [ctypes.c_int]
+        L.plm_profile_get.argtypes = [ctypes.c_char_p, _dp, _lp]
         _lib = L
     return _lib
 
@@ -327,3 +329,19 @@ def ridge(xtx, xty, alphas) -> np.ndarray:
     _check(lib().plm_ridge(F, _pd(A), _pd(b), _pd(al), len(al), _pd(out), _pi(info)))
     out[info != 0] = 1e30
     return out
+
+
+def profile_enable(on: bool = True):
+    _check(lib().plm_profile_enable(int(on)))
+
+
+def profile_reset():
+    _check(lib().plm_profile_reset())
+
+
+def profile_get(name: str):
+    """(total milliseconds, launches) of a kernel class since the last reset."""
+    ms = ctypes.c_double()
+    n = ctypes.c_long()
+    _check(lib().plm_profile_get(name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value

@@ -16,6 +16,7 @@
 #include "cell.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
+#include "profile.hpp"
 
 namespace plm {
 
@@ -438,7 +439,10 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     const int a_stride = m.max_nlm;
     a_buf.alloc((size_t)na * a_stride * 2 * elem);
     BatchDev bd = b.view();
-    launch_anlm(st, fp64, m.md, bd, a_buf.ptr, a_stride, a0, na, m.max_noconj);
+    {
+        ProfScope ps("anlm", st);
+        launch_anlm(st, fp64, m.md, bd, a_buf.ptr, a_stride, a0, na, m.max_noconj);
+    }
     check_hip(hipGetLastError(), "anlm launch");
     int rows_max = 0;
     for (int s = s0; s < s1; ++s) rows_max = std::max(rows_max, 7 + 3 * (b.atom_off[s + 1] - b.atom_off[s]));
@@ -446,8 +450,11 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     pl.P.ld = ld;
     pl.P.st0 = s0;
     pl.P.row_base = b.row_off[s0];
-    launch_xblock(st, fp64, m.md, pl.ct->cd, bd, pl.ct->bw, pl.P, s1 - s0, a_buf.ptr, a_stride, a0,
-                  X, roww, xe_sum, xe_sq);
+    {
+        ProfScope ps("xblock", st);
+        launch_xblock(st, fp64, m.md, pl.ct->cd, bd, pl.ct->bw, pl.P, s1 - s0, a_buf.ptr, a_stride,
+                      a0, X, roww, xe_sum, xe_sq);
+    }
     check_hip(hipGetLastError(), "xblock launch");
 }
 

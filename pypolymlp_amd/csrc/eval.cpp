@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "capi_internal.hpp"
+#include "profile.hpp"
 
 namespace plm {
 
@@ -117,14 +118,24 @@ void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, dou
     double* d_s = d_f + 3 * (size_t)A;
     BatchDev bd = b.view();
     const int pair_model = m.host.input().gtinv ? 0 : 1;
-    launch_anlm(st, fp64, m.md, bd, a_buf.ptr, m.max_nlm, 0, A, m.max_noconj, 1);
+    {
+        ProfScope ps("eval_anlm", st);
+        launch_anlm(st, fp64, m.md, bd, a_buf.ptr, m.max_nlm, 0, A, m.max_noconj, 1);
+    }
     check_hip(hipGetLastError(), "eval anlm launch");
-    launch_prodsum(st, fp64, m.md, m.ed, bd, a_buf.ptr, m.max_nlm, pse.ptr, psf.ptr,
-                   std::max(1, m.max_heads), m.max_lin, pair_model);
+    {
+        ProfScope ps("eval_prodsum", st);
+        launch_prodsum(st, fp64, m.md, m.ed, bd, a_buf.ptr, m.max_nlm, pse.ptr, psf.ptr,
+                       std::max(1, m.max_heads), m.max_lin, pair_model);
+    }
     check_hip(hipGetLastError(), "prodsum launch");
-    launch_pair(st, fp64, m.md, m.ed, bd, (const int*)b.d_atom_st.ptr, (const int*)b.d_hb_off.ptr,
-                (const int*)b.d_hb_j.ptr, (const double*)b.d_hb_d.ptr, pse.ptr, psf.ptr,
-                std::max(1, m.max_heads), d_e, d_f, d_s, pair_model);
+    {
+        ProfScope ps("eval_pair", st);
+        launch_pair(st, fp64, m.md, m.ed, bd, (const int*)b.d_atom_st.ptr,
+                    (const int*)b.d_hb_off.ptr, (const int*)b.d_hb_j.ptr,
+                    (const double*)b.d_hb_d.ptr, pse.ptr, psf.ptr, std::max(1, m.max_heads), d_e,
+                    d_f, d_s, pair_model);
+    }
     check_hip(hipGetLastError(), "pair launch");
     std::vector<double> h(n_out);
     check_hip(hipMemcpyAsync(h.data(), out.ptr, n_out * 8, hipMemcpyDeviceToHost, st), "copy");

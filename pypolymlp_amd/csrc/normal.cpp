@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "capi_internal.hpp"
+#include "profile.hpp"
 
 namespace plm {
 
@@ -49,10 +50,16 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
         check_hip(hipMemcpyAsync(acc.ybuf.ptr, y_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
         build_x_rows(m, b, s0, s1, fp64, acc.xbuf.ptr, ld, (const double*)acc.wbuf.ptr, xe_sum, xe_sq,
                      acc.abuf);
-        launch_gram(st, fp64, acc.xbuf.ptr, ld, (int)rows_pad, acc.F_pad, (double*)acc.xtx.ptr,
-                    acc.F_pad, kChunkRows);
+        {
+            ProfScope ps("gram", st);
+            launch_gram(st, fp64, acc.xbuf.ptr, ld, (int)rows_pad, acc.F_pad, (double*)acc.xtx.ptr,
+                        acc.F_pad, kChunkRows);
+        }
         check_hip(hipGetLastError(), "gram launch");
-        launch_xty(st, fp64, acc.xbuf.ptr, ld, (int)rows, F, (const double*)acc.ybuf.ptr, xty);
+        {
+            ProfScope ps("xty", st);
+            launch_xty(st, fp64, acc.xbuf.ptr, ld, (int)rows, F, (const double*)acc.ybuf.ptr, xty);
+        }
         check_hip(hipGetLastError(), "xty launch");
         acc.n_rows += rows;
         s0 = s1;

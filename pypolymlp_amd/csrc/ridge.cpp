@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "capi_internal.hpp"
+#include "profile.hpp"
 
 namespace plm {
 
@@ -39,7 +40,10 @@ void ridge_solve(int F, const double* xtx, const double* xty, const double* alph
         check_hip(hipMemcpyAsync(Af.ptr, A0.ptr, mat, hipMemcpyDeviceToDevice, st), "copy");
         launch_add_diag(st, (double*)Af.ptr, np, F, a);
         check_hip(hipMemsetAsync(di.ptr, 0, 4, st), "memset");
-        launch_cholesky(st, (double*)Af.ptr, np, np, (int*)di.ptr);
+        {
+            ProfScope ps("cholesky", st);
+            launch_cholesky(st, (double*)Af.ptr, np, np, (int*)di.ptr);
+        }
         check_hip(hipGetLastError(), "cholesky launch");
         check_hip(hipMemcpyAsync(x.ptr, b.ptr, np * 8, hipMemcpyDeviceToDevice, st), "copy");
         launch_chol_solve(st, (double*)Af.ptr, np, np, (double*)x.ptr);
