@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -311,9 +312,9 @@ const ColumnTables& ModelHandle::columns(int CB) {
 // ---------------------------------------------------------------------------
 // batch
 // ---------------------------------------------------------------------------
-BatchHandle::BatchHandle(ModelHandle& m, int n_st_, const double* axes, const double* pos,
-                         const int* types_in, const int* n_atoms_in)
-    : model(&m), n_st(n_st_) {
+BatchHandle::BatchHandle(ModelHandle* m, double cutoff_, int n_st_, const double* axes,
+                         const double* pos, const int* types_in, const int* n_atoms_in)
+    : model(m), cutoff(cutoff_), n_st(n_st_) {
     Device& dev = device();
     atom_off.assign(n_st + 1, 0);
     for (int s = 0; s < n_st; ++s) {
@@ -325,7 +326,6 @@ BatchHandle::BatchHandle(ModelHandle& m, int n_st_, const double* axes, const do
     for (int s = 0; s < n_st; ++s) row_off[s + 1] = row_off[s] + 7 + 3 * n_atoms_in[s];
     std::vector<double> pos_dev((size_t)n_atoms * 3), trans;
     std::vector<int> tr_off(n_st + 1, 0), atom_st(n_atoms);
-    const double cutoff = m.host.input().cutoff;
     size_t poff = 0;
     max_atoms = 0;
     for (int s = 0; s < n_st; ++s) {
@@ -341,8 +341,9 @@ BatchHandle::BatchHandle(ModelHandle& m, int n_st_, const double* axes, const do
         poff += 3 * (size_t)n;
     }
     types.assign(types_in, types_in + n_atoms);
-    for (int t : types)
-        if (t < 0 || t >= m.host.n_type()) throw std::invalid_argument("atom type out of range");
+    if (m)
+        for (int t : types)
+            if (t < 0 || t >= m->host.n_type()) throw std::invalid_argument("atom type out of range");
     d_types.alloc(n_atoms * 4);
     d_pos.alloc(pos_dev.size() * 8);
     d_trans.alloc(std::max<size_t>(trans.size(), 3) * 8);
@@ -359,33 +360,111 @@ BatchHandle::BatchHandle(ModelHandle& m, int n_st_, const double* axes, const do
     check_hip(hipMemcpyAsync(d_atom_st.ptr, atom_st.data(), n_atoms * 4, hipMemcpyHostToDevice, st), "copy");
     check_hip(hipMemcpyAsync(d_atom_off.ptr, atom_off.data(), (n_st + 1) * 4, hipMemcpyHostToDevice, st), "copy");
     check_hip(hipMemcpyAsync(d_row_off.ptr, row_off.data(), (n_st + 1) * 4, hipMemcpyHostToDevice, st), "copy");
-    // neighbour list: count, scan on host, fill
+    // cell-list search for one large structure whose cell spans >= 3 bins of
+    // width rc along every lattice direction
+    if (n_st == 1 && n_atoms >= 1024) {
+        CellInfo info;
+        std::vector<double> tr, prev;
+        cell_translations(axes, pos, n_atoms, cutoff, tr, prev, &info);
+        double inv[9];
+        {
+            const double* a = info.axis;
+            const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                               a[2] * (a[3] * a[7] - a[4] * a[6]);
+            inv[0] = (a[4] * a[8] - a[5] * a[7]) / det, inv[1] = -(a[1] * a[8] - a[2] * a[7]) / det;
+            inv[2] = (a[1] * a[5] - a[2] * a[4]) / det, inv[3] = -(a[3] * a[8] - a[5] * a[6]) / det;
+            inv[4] = (a[0] * a[8] - a[2] * a[6]) / det, inv[5] = -(a[0] * a[5] - a[2] * a[3]) / det;
+            inv[6] = (a[3] * a[7] - a[4] * a[6]) / det, inv[7] = -(a[0] * a[7] - a[1] * a[6]) / det;
+            inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+        }
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            const double h = 1.0 / std::sqrt(inv[3 * a] * inv[3 * a] + inv[3 * a + 1] * inv[3 * a + 1] +
+                                             inv[3 * a + 2] * inv[3 * a + 2]);
+            grid.nb[a] = (int)std::floor(h / cutoff);
+            grid.mexp[a] = info.mexp[a];
+            if (grid.nb[a] < 3) ok = false;
+        }
+        if (ok) {
+            binned = true;
+            for (int a = 0; a < 9; ++a) grid.inv[a] = inv[a];
+            d_tindex.alloc(info.tindex.size() * 4);
+            check_hip(hipMemcpy(d_tindex.ptr, info.tindex.data(), info.tindex.size() * 4,
+                                hipMemcpyHostToDevice), "copy");
+            grid.tindex = (const int*)d_tindex.ptr;
+        }
+    }
+    n_pairs = build_list(false, d_nb_off, d_nb_j, d_nb_t, d_nb_d, nb_off);
+    max_pairs_atom = 0;
+    for (int i = 0; i < n_atoms; ++i) max_pairs_atom = std::max(max_pairs_atom, nb_off[i + 1] - nb_off[i]);
+}
+
+void BatchHandle::prepare_bins() {
+    Device& dev = device();
+    hipStream_t st = dev.stream;
+    const int nbins = grid.nb[0] * grid.nb[1] * grid.nb[2];
+    d_bin.alloc((size_t)n_atoms * 4), d_wrap.alloc((size_t)n_atoms * 12);
+    d_bin_start.alloc((size_t)(nbins + 1) * 4), d_bin_sorted.alloc((size_t)n_atoms * 4);
+    DevBuf cnt, cur;
+    cnt.alloc((size_t)nbins * 4), cur.alloc((size_t)nbins * 4);
+    check_hip(hipMemsetAsync(cnt.ptr, 0, (size_t)nbins * 4, st), "memset");
+    check_hip(hipMemsetAsync(cur.ptr, 0, (size_t)nbins * 4, st), "memset");
+    launch_bin_assign(st, n_atoms, (const double*)d_pos.ptr, grid, (int*)d_bin.ptr, (int*)d_wrap.ptr,
+                      (int*)cnt.ptr);
+    check_hip(hipGetLastError(), "bin assign launch");
+    std::vector<int> c(nbins), start(nbins + 1, 0);
+    check_hip(hipMemcpyAsync(c.data(), cnt.ptr, (size_t)nbins * 4, hipMemcpyDeviceToHost, st), "copy");
+    check_hip(hipStreamSynchronize(st), "sync");
+    for (int b = 0; b < nbins; ++b) start[b + 1] = start[b] + c[b];
+    check_hip(hipMemcpyAsync(d_bin_start.ptr, start.data(), (size_t)(nbins + 1) * 4,
+                             hipMemcpyHostToDevice, st), "copy");
+    launch_bin_scatter(st, n_atoms, (const int*)d_bin.ptr, (const int*)d_bin_start.ptr, (int*)cur.ptr,
+                       (int*)d_bin_sorted.ptr);
+    check_hip(hipGetLastError(), "bin scatter launch");
+    check_hip(hipStreamSynchronize(st), "sync");
+}
+
+long BatchHandle::build_list(bool half, DevBuf& off, DevBuf& nj, DevBuf& nt, DevBuf& nd,
+                             std::vector<int>& host_off) {
+    Device& dev = device();
+    hipStream_t st = dev.stream;
     const double cut2 = cutoff * cutoff;
     const double tol = 1e-10, tol2 = tol * tol;
+    if (binned && !d_bin.ptr) prepare_bins();
     DevBuf cnt;
-    cnt.alloc(n_atoms * 4);
-    launch_neighbor_full(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr, (double*)d_pos.ptr,
-                         (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2, tol2, (int*)cnt.ptr,
-                         nullptr, nullptr, nullptr, nullptr);
-    check_hip(hipGetLastError(), "neighbor count launch");
+    cnt.alloc((size_t)n_atoms * 4);
+    auto run = [&](int* c, const int* o, int* j, int* t, double* d) {
+        if (binned)
+            launch_bin_pairs(st, n_atoms, half ? 1 : 0, (const double*)d_pos.ptr, (const int*)d_bin.ptr,
+                             (const int*)d_wrap.ptr, (const int*)d_bin_start.ptr,
+                             (const int*)d_bin_sorted.ptr, grid, (const double*)d_trans.ptr, cut2,
+                             tol2, tol, c, o, j, t, d);
+        else if (half)
+            launch_neighbor_half(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr,
+                                 (double*)d_pos.ptr, (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2,
+                                 tol2, tol, c, o, j, t, d);
+        else
+            launch_neighbor_full(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr,
+                                 (double*)d_pos.ptr, (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2,
+                                 tol2, c, o, j, t, d);
+        check_hip(hipGetLastError(), "neighbor launch");
+    };
+    run((int*)cnt.ptr, nullptr, nullptr, nullptr, nullptr);
     std::vector<int> c(n_atoms);
-    check_hip(hipMemcpyAsync(c.data(), cnt.ptr, n_atoms * 4, hipMemcpyDeviceToHost, st), "copy");
+    check_hip(hipMemcpyAsync(c.data(), cnt.ptr, (size_t)n_atoms * 4, hipMemcpyDeviceToHost, st), "copy");
     check_hip(hipStreamSynchronize(st), "sync");
-    nb_off.assign(n_atoms + 1, 0);
-    for (int i = 0; i < n_atoms; ++i) nb_off[i + 1] = nb_off[i] + c[i];
-    n_pairs = nb_off[n_atoms];
-    d_nb_off.alloc((n_atoms + 1) * 4);
-    d_nb_j.alloc(std::max<long>(n_pairs, 1) * 4);
-    d_nb_t.alloc(std::max<long>(n_pairs, 1) * 4);
-    d_nb_d.alloc(std::max<long>(n_pairs, 1) * 24);
-    check_hip(hipMemcpyAsync(d_nb_off.ptr, nb_off.data(), (n_atoms + 1) * 4, hipMemcpyHostToDevice, st), "copy");
-    launch_neighbor_full(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr, (double*)d_pos.ptr,
-                         (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2, tol2, nullptr,
-                         (int*)d_nb_off.ptr, (int*)d_nb_j.ptr, (int*)d_nb_t.ptr, (double*)d_nb_d.ptr);
-    check_hip(hipGetLastError(), "neighbor fill launch");
+    host_off.assign(n_atoms + 1, 0);
+    for (int i = 0; i < n_atoms; ++i) host_off[i + 1] = host_off[i] + c[i];
+    const long M = host_off[n_atoms];
+    off.alloc((size_t)(n_atoms + 1) * 4);
+    nj.alloc(std::max<long>(M, 1) * 4);
+    nt.alloc(std::max<long>(M, 1) * 4);
+    nd.alloc(std::max<long>(M, 1) * 24);
+    check_hip(hipMemcpyAsync(off.ptr, host_off.data(), (size_t)(n_atoms + 1) * 4,
+                             hipMemcpyHostToDevice, st), "copy");
+    run(nullptr, (const int*)off.ptr, (int*)nj.ptr, (int*)nt.ptr, (double*)nd.ptr);
     check_hip(hipStreamSynchronize(st), "sync");
-    max_pairs_atom = 0;
-    for (int i = 0; i < n_atoms; ++i) max_pairs_atom = std::max(max_pairs_atom, c[i]);
+    return M;
 }
 
 BatchDev BatchHandle::view() const {
@@ -561,49 +640,23 @@ int plm_cell_translations(const double* axis, const double* pos_c, int n, double
 int plm_neighbor(const double* axis, const double* pos_c, int n, double cutoff, int half,
                  int* offset, int* neigh, int* trans_id, double* d, long capacity, long* n_pairs) {
     PLM_TRY({
-        Device& dev = device();
-        hipStream_t st = dev.stream;
-        std::vector<double> tr, prev;
-        cell_translations(axis, pos_c, n, cutoff, tr, prev);
-        std::vector<double> pd((size_t)3 * n);
-        for (int j = 0; j < n; ++j)
-            for (int a = 0; a < 3; ++a) pd[3 * (size_t)j + a] = prev[(size_t)a * n + j];
-        const int ntr = (int)(tr.size() / 3);
-        DevBuf dpos, dtr, dto, dst, dao, cnt, doff, dj, dt, dd;
-        dpos.alloc(pd.size() * 8), dtr.alloc(std::max(1, ntr) * 24), dto.alloc(8);
-        dst.alloc((size_t)n * 4), dao.alloc(8), cnt.alloc((size_t)n * 4);
-        std::vector<int> zeros(n, 0), to = {0, ntr}, ao = {0, n};
-        check_hip(hipMemcpyAsync(dpos.ptr, pd.data(), pd.size() * 8, hipMemcpyHostToDevice, st), "copy");
-        if (ntr) check_hip(hipMemcpyAsync(dtr.ptr, tr.data(), tr.size() * 8, hipMemcpyHostToDevice, st), "copy");
-        check_hip(hipMemcpyAsync(dto.ptr, to.data(), 8, hipMemcpyHostToDevice, st), "copy");
-        check_hip(hipMemcpyAsync(dst.ptr, zeros.data(), (size_t)n * 4, hipMemcpyHostToDevice, st), "copy");
-        check_hip(hipMemcpyAsync(dao.ptr, ao.data(), 8, hipMemcpyHostToDevice, st), "copy");
-        const double cut2 = cutoff * cutoff, tol = 1e-10, tol2 = tol * tol;
-        auto run = [&](int* c, int* o, int* j, int* t, double* dv) {
-            if (half)
-                launch_neighbor_half(st, n, (int*)dst.ptr, (int*)dao.ptr, (double*)dpos.ptr, (int*)dto.ptr,
-                                     (double*)dtr.ptr, cut2, tol2, tol, c, o, j, t, dv);
-            else
-                launch_neighbor_full(st, n, (int*)dst.ptr, (int*)dao.ptr, (double*)dpos.ptr, (int*)dto.ptr,
-                                     (double*)dtr.ptr, cut2, tol2, c, o, j, t, dv);
-            check_hip(hipGetLastError(), "neighbor launch");
-        };
-        run((int*)cnt.ptr, nullptr, nullptr, nullptr, nullptr);
-        std::vector<int> c(n), off(n + 1, 0);
-        check_hip(hipMemcpyAsync(c.data(), cnt.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, st), "copy");
-        check_hip(hipStreamSynchronize(st), "sync");
-        for (int i = 0; i < n; ++i) off[i + 1] = off[i] + c[i];
-        const long M = off[n];
+        std::vector<int> types(n, 0);
+        BatchHandle b(nullptr, cutoff, 1, axis, pos_c, types.data(), &n);
+        DevBuf off, nj, nt, nd;
+        std::vector<int> host_off;
+        const long M = half ? b.build_list(true, off, nj, nt, nd, host_off) : b.n_pairs;
+        if (!half) host_off = b.nb_off;
         *n_pairs = M;
         if (capacity < M || !offset) return PLM_OK;
-        doff.alloc((size_t)(n + 1) * 4), dj.alloc(std::max<long>(M, 1) * 4);
-        dt.alloc(std::max<long>(M, 1) * 4), dd.alloc(std::max<long>(M, 1) * 24);
-        check_hip(hipMemcpyAsync(doff.ptr, off.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice, st), "copy");
-        run(nullptr, (int*)doff.ptr, (int*)dj.ptr, (int*)dt.ptr, (double*)dd.ptr);
-        std::memcpy(offset, off.data(), (size_t)(n + 1) * 4);
-        if (neigh) check_hip(hipMemcpyAsync(neigh, dj.ptr, M * 4, hipMemcpyDeviceToHost, st), "copy");
-        if (trans_id) check_hip(hipMemcpyAsync(trans_id, dt.ptr, M * 4, hipMemcpyDeviceToHost, st), "copy");
-        if (d) check_hip(hipMemcpyAsync(d, dd.ptr, M * 24, hipMemcpyDeviceToHost, st), "copy");
+        Device& dev = device();
+        hipStream_t st = dev.stream;
+        const DevBuf& J = half ? nj : b.d_nb_j;
+        const DevBuf& T = half ? nt : b.d_nb_t;
+        const DevBuf& D = half ? nd : b.d_nb_d;
+        std::memcpy(offset, host_off.data(), (size_t)(n + 1) * 4);
+        if (neigh) check_hip(hipMemcpyAsync(neigh, J.ptr, M * 4, hipMemcpyDeviceToHost, st), "copy");
+        if (trans_id) check_hip(hipMemcpyAsync(trans_id, T.ptr, M * 4, hipMemcpyDeviceToHost, st), "copy");
+        if (d) check_hip(hipMemcpyAsync(d, D.ptr, M * 24, hipMemcpyDeviceToHost, st), "copy");
         check_hip(hipStreamSynchronize(st), "sync");
     });
 }
@@ -614,7 +667,8 @@ int plm_batch_create(const plm_model* m, int n_st, const double* axes, const dou
         if (!m || !out || n_st <= 0) throw std::invalid_argument("bad batch arguments");
         auto* mh = const_cast<ModelHandle*>(reinterpret_cast<const ModelHandle*>(m));
         if (!mh->built) mh->build_device();
-        auto b = std::make_unique<BatchHandle>(*mh, n_st, axes, pos, types, n_atoms);
+        auto b = std::make_unique<BatchHandle>(mh, mh->host.input().cutoff, n_st, axes, pos, types,
+                                               n_atoms);
         *out = reinterpret_cast<plm_batch*>(b.release());
     });
 }

@@ -81,7 +81,8 @@ struct ModelHandle {
 };
 
 struct BatchHandle {
-    ModelHandle* model;
+    ModelHandle* model;  // may be null for neighbour-list-only use
+    double cutoff = 0.0;
     int n_st = 0, n_atoms = 0, max_atoms = 0, max_pairs_atom = 0;
     long n_pairs = 0;
     std::vector<int> atom_off, row_off, types, nb_off;
@@ -91,10 +92,18 @@ struct BatchHandle {
     bool has_half = false;
     long n_half = 0;
     DevBuf d_hb_off, d_hb_j, d_hb_t, d_hb_d;
-    BatchHandle(ModelHandle& m, int n_st, const double* axes, const double* pos, const int* types,
-                const int* n_atoms);
+    // cell-list search (single large structure)
+    bool binned = false;
+    BinGridHost grid{};
+    DevBuf d_tindex, d_bin, d_wrap, d_bin_start, d_bin_sorted;
+    BatchHandle(ModelHandle* m, double cutoff, int n_st, const double* axes, const double* pos,
+                const int* types, const int* n_atoms);
     BatchDev view() const;
     void ensure_half();
+    // neighbour list (full or half) into the given buffers; returns pair count
+    long build_list(bool half, DevBuf& off, DevBuf& nj, DevBuf& nt, DevBuf& nd,
+                    std::vector<int>& host_off);
+    void prepare_bins();
 };
 
 struct XPlan {

@@ -112,7 +112,7 @@ struct Cell {
 }  // namespace
 
 void cell_translations(const double* axis9, const double* pos_c, int n, double cutoff,
-                       std::vector<double>& trans, std::vector<double>& pos_out) {
+                       std::vector<double>& trans, std::vector<double>& pos_out, CellInfo* info) {
     Cell c;
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) c.ax[i][j] = axis9[3 * i + j];
@@ -144,13 +144,26 @@ void cell_translations(const double* axis9, const double* pos_c, int n, double c
                 if (maxlen < d) maxlen = d;
             }
     trans.clear();
+    if (info) {
+        info->mexp[0] = m0, info->mexp[1] = m1, info->mexp[2] = m2;
+        info->tindex.assign((size_t)(2 * m0 + 1) * (2 * m1 + 1) * (2 * m2 + 1), -1);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) info->axis[3 * a + b] = c.ax[a][b];
+    }
+    int t = 0;
     for (int i = -m0; i < m0 + 1; ++i)
         for (int j = -m1; j < m1 + 1; ++j)
             for (int k = -m2; k < m2 + 1; ++k) {
                 double v[3];
                 c.cart(i, j, k, v);
                 const double d = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                if (d < maxlen + cutoff) trans.insert(trans.end(), v, v + 3);
+                if (d < maxlen + cutoff) {
+                    trans.insert(trans.end(), v, v + 3);
+                    if (info)
+                        info->tindex[((size_t)(i + m0) * (2 * m1 + 1) + (j + m1)) * (2 * m2 + 1) +
+                                     (k + m2)] = t;
+                    ++t;
+                }
             }
 }
 

@@ -69,31 +69,8 @@ void upload_potential(ModelHandle& m) {
 
 void BatchHandle::ensure_half() {
     if (has_half) return;
-    Device& dev = device();
-    hipStream_t st = dev.stream;
-    const double cutoff = model->host.input().cutoff;
-    const double cut2 = cutoff * cutoff, tol = 1e-10, tol2 = tol * tol;
-    DevBuf cnt;
-    cnt.alloc((size_t)n_atoms * 4);
-    launch_neighbor_half(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr, (double*)d_pos.ptr,
-                         (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2, tol2, tol, (int*)cnt.ptr,
-                         nullptr, nullptr, nullptr, nullptr);
-    check_hip(hipGetLastError(), "half count launch");
-    std::vector<int> c(n_atoms), off(n_atoms + 1, 0);
-    check_hip(hipMemcpyAsync(c.data(), cnt.ptr, (size_t)n_atoms * 4, hipMemcpyDeviceToHost, st), "copy");
-    check_hip(hipStreamSynchronize(st), "sync");
-    for (int i = 0; i < n_atoms; ++i) off[i + 1] = off[i] + c[i];
-    n_half = off[n_atoms];
-    d_hb_off.alloc((size_t)(n_atoms + 1) * 4);
-    d_hb_j.alloc(std::max<long>(n_half, 1) * 4);
-    d_hb_t.alloc(std::max<long>(n_half, 1) * 4);
-    d_hb_d.alloc(std::max<long>(n_half, 1) * 24);
-    check_hip(hipMemcpyAsync(d_hb_off.ptr, off.data(), (size_t)(n_atoms + 1) * 4, hipMemcpyHostToDevice, st), "copy");
-    launch_neighbor_half(st, n_atoms, (int*)d_atom_st.ptr, (int*)d_atom_off.ptr, (double*)d_pos.ptr,
-                         (int*)d_tr_off.ptr, (double*)d_trans.ptr, cut2, tol2, tol, nullptr,
-                         (int*)d_hb_off.ptr, (int*)d_hb_j.ptr, (int*)d_hb_t.ptr, (double*)d_hb_d.ptr);
-    check_hip(hipGetLastError(), "half fill launch");
-    check_hip(hipStreamSynchronize(st), "sync");
+    std::vector<int> off;
+    n_half = build_list(true, d_hb_off, d_hb_j, d_hb_t, d_hb_d, off);
     has_half = true;
 }
 
@@ -148,7 +125,7 @@ void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, dou
 void eval_single(ModelHandle& m, const double* axis, const double* pos_c, const int* types, int n,
                  bool fp64, double* energy, double* forces, double* stress) {
     if (!m.built) m.build_device();
-    BatchHandle b(m, 1, axis, pos_c, types, &n);
+    BatchHandle b(&m, m.host.input().cutoff, 1, axis, pos_c, types, &n);
     eval_batch(m, b, fp64, energy, forces, stress);
 }
 

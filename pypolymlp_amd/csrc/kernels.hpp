@@ -19,6 +19,21 @@ void launch_neighbor_half(hipStream_t st, int n, const int* atom_st, const int* 
                           double tol2, double tol, int* count, const int* nb_off, int* nb_j,
                           int* nb_t, double* nb_d);
 
+struct BinGridHost {
+    int nb[3];
+    int mexp[3];
+    double inv[9];
+    const int* tindex;  // device pointer
+};
+void launch_bin_assign(hipStream_t st, int n, const double* pos, const BinGridHost& g, int* bin,
+                       int* wrap, int* bin_count);
+void launch_bin_scatter(hipStream_t st, int n, const int* bin, const int* start, int* cursor,
+                        int* sorted);
+void launch_bin_pairs(hipStream_t st, int n, int half, const double* pos, const int* bin,
+                      const int* wrap, const int* start, const int* sorted, const BinGridHost& g,
+                      const double* trans, double cut2, double tol2, double tol, int* count,
+                      const int* nb_off, int* nb_j, int* nb_t, double* nb_d);
+
 size_t anlm_smem(const ModelDev& md, int elem, int nthreads, int max_noconj);
 void launch_anlm(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, void* a_full,
                  int a_stride, int atom0, int n_atoms, int max_noconj, int negate = 0);

@@ -79,3 +79,28 @@ def test_eval_configs_vs_oracle(cfg, tmp_path):
             f"max dF {np.abs(Fm[k] - fr).max() * 1e3:.3e} meV/A")
         assert abs(Em[k] - er) / n < 1e-4
         assert np.abs(Fm[k] - fr).max() < 1e-3
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+def test_large_cell_binned_lists_and_eval(tmp_path):
+    """Cell-list search (>= 1024 atoms): bit-exact full/half lists and eval vs the oracle."""
+    st = datasets.config4_structure(seed=9, reps=(8, 8, 10))  # 1280 atoms, 26.4 x 26.4 x 33 A
+    ax, pos = st.axis, st.positions_cartesian
+    for half in (False, True):
+        say(f"binned neighbor list n={st.n_total} half={half}")
+        off, nb, tid, d = capi.neighbor(ax, pos, 6.0, half=half)
+        ro, rn, rd = ref.neighbor_list(ax, pos, 6.0, half=half)
+        assert np.array_equal(off, ro)
+        assert np.array_equal(nb, rn)
+        assert np.array_equal(d, rd)
+    spec = datasets.config1_model()
+    m = capi.Model(spec.as_dict())
+    coeffs = np.random.default_rng(3).normal(size=m.n_features) * 1e-3
+    m.set_coeffs(coeffs)
+    say("binned eval")
+    e, f, s = capi.eval_structure(m, ax, pos, st.types)
+    path = write_model_yaml(spec, str(tmp_path / "m.yaml"), coeffs)
+    er, fr, sr = ref.RefModel(path, with_eval=True).eval(ax, pos, st.types)
+    assert e == pytest.approx(er, rel=1e-10)
+    assert np.linalg.norm(f - fr) <= 1e-10 * np.linalg.norm(fr)
+    assert np.linalg.norm(s - sr) <= 1e-10 * np.linalg.norm(sr)
