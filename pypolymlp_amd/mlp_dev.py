@@ -12,8 +12,8 @@ Host-side mirror of pypolymlp's sequential regression:
 The per-row weights and weighted targets are computed here exactly as the
 reference computes them; X, X^T X, X^T y and the energy-row sums come from
 the HIP library (pypolymlp_amd.capi); the ridge solve runs on the GPU
-(plm_ridge) when available and otherwise through LAPACK posv like the
-reference.
+(plm_ridge: FP64 Cholesky with one refinement step). There is no CPU path:
+without the HIP library every entry point raises.
 """
 
 from __future__ import annotations
@@ -155,34 +155,11 @@ def calc_xtx_xty(model: capi.Model, datasets: Sequence[Dataset], precision=capi.
                            min_energy=min_energy, xe_sum=xe_sum, xe_sq_sum=xe_sq)
 
 
-def solve_linear_equation(A, b):
-    """LAPACK posv (upper) as solvers.py:9-45; 1e30 on a failed factorization."""
-    from scipy.linalg.lapack import get_lapack_funcs
-
-    (posv,) = get_lapack_funcs(("posv",), (A, b))
-    _, x, info = posv(A, b, lower=False, overwrite_a=False, overwrite_b=False)
-    if info != 0:
-        x = np.ones(x.shape) * 1e30
-    return x
-
-
-def solver_ridge(xtx, xty, alphas, use_gpu: bool = True):
-    """Coefficients (F x n_alpha) for each alpha (solvers.py:48-84)."""
-    F = xtx.shape[0]
-    if use_gpu:
-        try:
-            return capi.ridge(xtx, xty, alphas).T
-        except RuntimeError as e:
-            if "not available" not in str(e):
-                raise
-    coefs = np.zeros((F, len(alphas)))
-    A = xtx.copy()
-    prev = 0.0
-    for i, a in enumerate(alphas):
-        A.flat[:: F + 1] += a - prev
-        coefs[:, i] = solve_linear_equation(A, xty)
-        prev = a
-    return coefs
+def solver_ridge(xtx, xty, alphas):
+    """Coefficients (F x n_alpha) of (xtx + alpha I) c = xty for each alpha, on
+    the GPU (solvers.py:48-84). A failed factorization gives 1e30 entries, as
+    solvers.py:41-43 does."""
+    return capi.ridge(xtx, xty, alphas).T
 
 
 def compute_rmse(coefs_array, ne: NormalEquations):
@@ -208,10 +185,10 @@ class FitResult:
 
 
 def fit(model: capi.Model, train: List[Dataset], test: List[Dataset], alphas,
-        precision=capi.PLM_FP64, use_gpu_solver: bool = True) -> FitResult:
+        precision=capi.PLM_FP64) -> FitResult:
     """Sequential ridge fit with model selection on the test RMSE (fit.py:12-63)."""
     tr = calc_xtx_xty(model, train, precision=precision)
-    coefs = solver_ridge(tr.xtx, tr.xty, alphas, use_gpu=use_gpu_solver)
+    coefs = solver_ridge(tr.xtx, tr.xty, alphas)
     rmse_train = compute_rmse(coefs, tr)
     te = calc_xtx_xty(model, test, precision=precision, scales=tr.scales.copy(),
                       min_energy=tr.min_energy)

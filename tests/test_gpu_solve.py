@@ -1,0 +1,151 @@
+"""GPU ridge solve and the whole fit, against the oracle.
+
+plm_ridge (blocked FP64 Cholesky on FP64 MFMA plus one refinement step)
+against LAPACK posv as solvers.py:9-84 calls it:
+  - backward error ||(A + a I) c - b|| / (||A + a I|| ||c||) <= 1e-14;
+  - forward difference to posv <= 100 * cond * eps (relative, normwise),
+    where cond is the 2-norm condition number of A + a I;
+  - a matrix that is not positive definite gives 1e30 coefficients like the
+    reference's failed posv.
+
+Whole fit on cfg1 (Morse targets): FP64 GPU fit against the oracle's fit
+(reference X from the compiled sources, numpy X^T X, the same scales, posv):
+predicted test energies within 1e-6 eV/atom and forces within 1e-5 eV/A.
+Mixed precision against FP64: within 0.1 meV/atom and 1 meV/A.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import ref
+from oracle.numerics import solver_ridge_posv
+from pypolymlp_amd import capi, datasets, mlp_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _spd(F, rows, seed, col_scale_decades=3.0):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(rows, F)) * np.logspace(0, -col_scale_decades, F)[None, :]
+    X[:, 1:] += 0.3 * X[:, :-1]
+    A = X.T @ X
+    b = X.T @ rng.normal(size=rows)
+    return A, b
+
+
+@pytest.mark.parametrize("F", [61, 300, 1000])
+def test_ridge_vs_posv(F):
+    A, b = _spd(F, 3 * F, seed=F)
+    alphas = np.array([1e-6, 1e-4, 1e-2]) * np.trace(A) / F
+    say(f"ridge F={F}")
+    C = mlp_dev.solver_ridge(A, b, alphas)
+    Cr = solver_ridge_posv(A, b, alphas)
+    eps = np.finfo(np.float64).eps
+    for k, a in enumerate(alphas):
+        Aa = A + a * np.eye(F)
+        c = C[:, k]
+        bwd = np.linalg.norm(Aa @ c - b) / (np.linalg.norm(Aa, 2) * np.linalg.norm(c))
+        cond = np.linalg.cond(Aa)
+        fwd = np.linalg.norm(c - Cr[:, k]) / np.linalg.norm(Cr[:, k])
+        say(f"  alpha {a:.2e}: backward {bwd:.2e}, cond {cond:.1e}, fwd vs posv {fwd:.2e}")
+        assert bwd <= 1e-14
+        assert fwd <= max(100 * cond * eps, 1e-13)
+
+
+def test_ridge_not_positive_definite():
+    A, b = _spd(130, 400, seed=3)
+    A[70, 70] = -abs(A[70, 70])
+    say("ridge non-PD")
+    C = mlp_dev.solver_ridge(A, b, np.array([1e-8]))
+    assert np.all(C == 1e30)
+    Cr = solver_ridge_posv(A, b, np.array([1e-8]))
+    assert np.all(Cr == 1e30)
+
+
+def _oracle_fit(R, train, test, alphas, min_e, weight_stress=0.1):
+    """Oracle fit: reference X, numpy X^T X, the reference's scale rule, posv."""
+    def normal(ds, scales=None):
+        ds.sort_by_atoms()
+        sts = ds.structures
+        X = R.x_batch([s.axis for s in sts], [s.positions_cartesian for s in sts],
+                      [s.types for s in sts], force=True)
+        w, y = mlp_dev.row_weights(ds, min_e, weight_stress)
+        ne = len(sts)
+        xe_sum = X[:ne].sum(axis=0)
+        xe_sq = (X[:ne] ** 2).sum(axis=0)
+        scales, zero = mlp_dev.compute_scales(scales, xe_sum, xe_sq, ne)
+        Xw = X * w[:, None]
+        xtx = Xw.T @ Xw
+        xty = Xw.T @ y
+        xtx[zero] = 0.0
+        xtx[:, zero] = 0.0
+        xty[zero] = 0.0
+        xtx /= scales[:, None] * scales[None, :]
+        xty /= scales
+        return mlp_dev.NormalEquations(xtx=xtx, xty=xty, y_sq_norm=float(y @ y),
+                                       total_n_data=X.shape[0], scales=scales, min_energy=min_e)
+
+    tr = normal(train)
+    coefs = solver_ridge_posv(tr.xtx, tr.xty, alphas)
+    te = normal(test, scales=tr.scales.copy())
+    rm = mlp_dev.compute_rmse(coefs, te)
+    k = int(np.argmin(rm))
+    return coefs[:, k] / tr.scales, float(alphas[k])
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+def test_fit_cfg1_fp64_vs_oracle_and_mixed(tmp_path):
+    from tests.helpers import write_model_yaml
+
+    spec = datasets.config1_model()
+    full = datasets.config1_dataset(48, seed=11)
+    train = datasets.Dataset(structures=full.structures[:36], energies=full.energies[:36],
+                             forces=full.forces[:36], stresses=full.stresses[:36], name="train")
+    test = datasets.Dataset(structures=full.structures[36:], energies=full.energies[36:],
+                            forces=full.forces[36:], stresses=full.stresses[36:], name="test")
+    alphas = np.array([1e-5, 1e-4, 1e-3, 1e-2])
+    m = capi.Model(spec.as_dict())
+    say("fit cfg1 FP64")
+    r64 = mlp_dev.fit(m, [train], [test], alphas, precision=capi.PLM_FP64)
+    say(f"  alpha {r64.alpha}, rmse train {r64.rmse_train:.4e}, test {r64.rmse_test:.4e}")
+    say("fit cfg1 MIXED")
+    rmx = mlp_dev.fit(m, [train], [test], alphas, precision=capi.PLM_MIXED)
+    say(f"  alpha {rmx.alpha}, rmse train {rmx.rmse_train:.4e}, test {rmx.rmse_test:.4e}")
+    path = write_model_yaml(spec, str(tmp_path / "m.yaml"))
+    R = ref.RefModel(path)
+    say("fit cfg1 oracle")
+    c_ref, a_ref = _oracle_fit(R, train, test, alphas, r64_min_e(train))
+    assert a_ref == r64.alpha
+
+    def predict(coeffs):
+        m.set_coeffs(coeffs)
+        sts = test.structures
+        b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                       [s.types for s in sts])
+        return capi.eval_batch(m, b, capi.PLM_FP64)
+
+    n_at = np.array([s.n_total for s in test.structures])
+    E64, F64, _ = predict(r64.coeffs_unscaled)
+    Er, Fr, _ = predict(c_ref)
+    Em, Fm, _ = predict(rmx.coeffs_unscaled)
+    dref_e = np.max(np.abs(E64 - Er) / n_at)
+    dref_f = max(np.abs(a - b).max() for a, b in zip(F64, Fr))
+    dmix_e = np.max(np.abs(Em - E64) / n_at)
+    dmix_f = max(np.abs(a - b).max() for a, b in zip(Fm, F64))
+    say(f"  FP64 vs oracle fit: dE {dref_e * 1e3:.3e} meV/atom, dF {dref_f * 1e3:.3e} meV/A")
+    say(f"  mixed vs FP64 fit: dE {dmix_e * 1e3:.3e} meV/atom, dF {dmix_f * 1e3:.3e} meV/A")
+    assert dref_e < 1e-6
+    assert dref_f < 1e-5
+    assert dmix_e < 1e-4
+    assert dmix_f < 1e-3
+
+
+def r64_min_e(ds):
+    return mlp_dev.get_min_energy([ds])

@@ -98,3 +98,32 @@ def test_scales_closed_form_and_zero_rule():
     np.testing.assert_allclose(s[0], 1.0)
     assert list(zero) == [False, True, True]
     assert s[1] == 1.0 and s[2] == 1.0
+
+
+def test_oracle_ridge_closed_form():
+    """The oracle's posv ridge (solvers.py:48-84) on a diagonal system and a dense one."""
+    from oracle.numerics import solver_ridge_posv
+
+    d = np.array([1.0, 4.0, 9.0, 16.0])
+    b = np.array([1.0, 2.0, 3.0, 4.0])
+    alphas = np.array([0.0, 0.5, 2.0])
+    C = solver_ridge_posv(np.diag(d), b, alphas)
+    for k, a in enumerate(alphas):
+        np.testing.assert_allclose(C[:, k], b / (d + a), rtol=1e-15)
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(50, 12))
+    A, y = X.T @ X, X.T @ rng.normal(size=50)
+    C = solver_ridge_posv(A, y, np.array([1e-3, 1e-1]))
+    for k, a in enumerate([1e-3, 1e-1]):
+        np.testing.assert_allclose(C[:, k], np.linalg.solve(A + a * np.eye(12), y), rtol=1e-12)
+    A[3, 3] = -1.0
+    assert np.all(solver_ridge_posv(A, y, np.array([0.0])) == 1e30)
+
+
+def test_product_path_fails_loudly_without_library(monkeypatch):
+    monkeypatch.setattr(capi, "_lib", None)
+    monkeypatch.setattr(capi, "LIB_PATH", os.path.join(ROOT, "no_such_library.so"))
+    with pytest.raises(RuntimeError, match="not built"):
+        mlp_dev.solver_ridge(np.eye(3), np.ones(3), np.array([0.1]))
+    with pytest.raises(RuntimeError, match="not built"):
+        capi.Model(datasets.config1_model().as_dict())
