@@ -777,13 +777,12 @@ int plm_normal_result(plm_normal* ap, double* xtx, double* xty, double* xe_sum, 
         if (xtx) {
             std::vector<double> full((size_t)Fp * Fp);
             check_hip(hipMemcpy(full.data(), acc->xtx.ptr, full.size() * 8, hipMemcpyDeviceToHost), "copy");
-            // upper block triangle is valid; mirror by blocks of the Gram tile (128)
-            const int T = 128;
+            // the upper block triangle is accumulated (64-wide tiles in FP64,
+            // 128-wide in mixed precision); its element upper triangle is
+            // valid in both, so mirror element-wise
             for (int i = 0; i < F; ++i)
-                for (int j = 0; j < F; ++j) {
-                    const bool upper = (i / T) <= (j / T);
-                    xtx[(size_t)i * F + j] = upper ? full[(size_t)i * Fp + j] : full[(size_t)j * Fp + i];
-                }
+                for (int j = 0; j < F; ++j)
+                    xtx[(size_t)i * F + j] = i <= j ? full[(size_t)i * Fp + j] : full[(size_t)j * Fp + i];
         }
         std::vector<double> v((size_t)Fp * 3);
         check_hip(hipMemcpy(v.data(), acc->vec.ptr, v.size() * 8, hipMemcpyDeviceToHost), "copy");

@@ -12,9 +12,12 @@ max(column max, 1e-9 * median column max) on the remaining
 cancellation-dominated columns (invariants ~1e-15 in near-symmetric cells,
 which the regression zeroes), which still exposes any indexing or sign error.
 
-Mixed precision (FP32 feature stream): per-column relative error with
-median <= 1e-6 and 99th percentile <= 1e-3; its accuracy claim is made on
-fitted predictions (meV/atom) elsewhere.
+Mixed precision (FP32 feature stream): per-column error relative to
+max(column max, 1e-2 * median column max) with median <= 1e-5, 99th
+percentile <= 1e-3 and maximum <= 1e-2 (FP32 rounding of a cancelling
+invariant is relative to the size of its terms, i.e. of the typical feature,
+not of its cancelled value); its accuracy claim is made on fitted
+predictions (meV/atom, meV/A) in test_gpu_solve.py.
 
 X^T X / X^T y: normwise 1e-10 (FP64 MFMA) and 1e-4 (split bf16 x3) against
 numpy on the FP64 X.
@@ -48,10 +51,26 @@ def colwise_rel(X, Xr):
 
 
 def check_mixed(Xm, Xr):
-    e = colwise_rel(Xm, Xr)
-    say(f"  mixed per-column rel err: median {np.median(e):.2e} p99 {np.quantile(e, 0.99):.2e}")
-    assert np.median(e) < 1e-6
+    colmax = np.abs(Xr).max(axis=0)
+    floor = np.maximum(colmax, 1e-2 * np.median(colmax[colmax > 0]))
+    floor[floor == 0] = 1.0
+    e = np.abs(Xm - Xr).max(axis=0) / floor
+    say(f"  mixed per-column err: median {np.median(e):.2e} p99 {np.quantile(e, 0.99):.2e} "
+        f"max {e.max():.2e}")
+    assert np.median(e) < 1e-5
     assert np.quantile(e, 0.99) < 1e-3
+    assert e.max() < 1e-2
+
+
+def report_worst(X, Xr, noise, k=5):
+    colmax = np.abs(Xr).max(axis=0)
+    colmax[colmax == 0] = 1.0
+    rel = np.abs(X - Xr).max(axis=0) / colmax
+    spread = noise.max(axis=0) / colmax
+    for c in np.argsort(rel)[::-1][:k]:
+        r = int(np.argmax(np.abs(X[:, c] - Xr[:, c])))
+        say(f"    col {c}: rel err {rel[c]:.2e}, oracle spread {spread[c]:.2e}, colmax {colmax[c]:.3e}, "
+            f"worst row {r}")
 
 
 def _structs(sts):
@@ -116,6 +135,8 @@ def test_x_rows_configs(cfg, n_st, tmp_path):
     Xr, noise = oracle_x_and_noise(ref.RefModel(path), axes, pos, types)
     strict, loose, n_loose = x_parity(X, Xr, noise)
     say(f"  fp64 column-relative err {strict:.2e} (resolved), {loose:.2e} over {n_loose} cancelling columns")
+    report_worst(X, Xr, noise)
+    assert np.linalg.norm(X - Xr) <= 1e-10 * np.linalg.norm(Xr)
     assert strict <= 1e-10
     assert loose <= 1e-6
     check_mixed(capi.features_x(m, b, capi.PLM_MIXED), Xr)
