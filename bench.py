@@ -118,7 +118,7 @@ def main():
             acc.accumulate(batch, w, y, prec)
         t1 = time.perf_counter()
         capi.profile_enable(False)
-        kern = {k: capi.profile_get(k) for k in ("anlm", "xblock", "gram", "xty")}
+        kern = {k: capi.profile_get(k) for k in ("anlm", "pairdata", "xrows", "gram", "xty")}
         return (t1 - t0) / steps, kern
 
     log("timing mixed ...")
@@ -140,13 +140,13 @@ def main():
                 "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
                 "kernel": "gram_bf16x3_kernel"}
     else:
-        ms_x, n_x = k_mixed["xblock"]
+        ms_x, n_x = k_mixed["xrows"]
         # algorithmic bytes of one launch: X written once in FP32 plus the
         # order parameters and pair records read
         per_launch = (float(R) * F * 4.0) / n_x
         achieved = per_launch / (ms_x / n_x * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "xblock_kernel"}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "xrows_kernel"}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -39,6 +39,7 @@ SOURCES = [
     ("eval.cpp", []),
     ("kernels_eval.hip", []),
     ("kernels_x.hip", []),
+    ("kernels_xrows.hip", []),
     ("kernels_nb.hip", []),
     ("kernels_gram.hip", []),
 ]

@@ -14,6 +14,7 @@
 
 #include "../../include/polymlp_amd.h"
 #include "capi_internal.hpp"
+#include "pair_data.hpp"
 #include "cell.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
@@ -213,6 +214,7 @@ void ModelHandle::build_device() {
         D.wt_conj = up(*this, wtj);
         D.lin_global = up(*this, T.linear_global);
         host_wz_off.push_back(wzo);
+        host_wz_ylm.push_back(wzy);
     }
     built = true;
 }
@@ -284,8 +286,22 @@ const ColumnTables& ModelHandle::columns(int CB) {
         need_off[(size_t)t * (nblk + 1) + nblk] = (int)need.size();
         bw_off[(size_t)t * (nblk + 1) + nblk] = (int)bw_nz.size();
     }
+    // block kind (all columns linear features) and Y keys its nonzeros use
+    std::vector<unsigned char> kind(nblk);
+    std::vector<int> nkey(nblk, 0);
+    for (int b = 0; b < nblk; ++b) {
+        bool lin = true;
+        for (int c = c0[b]; c < c0[b + 1]; ++c) lin = lin && nfac[c] == 1;
+        kind[b] = lin ? 1 : 0;
+        for (int t = 0; t < nt; ++t)
+            for (int z = bw_off[(size_t)t * (nblk + 1) + b]; z < bw_off[(size_t)t * (nblk + 1) + b + 1]; ++z)
+                nkey[b] = std::max(nkey[b], host_wz_ylm[t][bw_nz[z]] + 1);
+        ct->nkey_max = std::max(ct->nkey_max, nkey[b]);
+    }
     ColumnsDev& cd = ct->cd;
     cd.n_cols = F;
+    cd.blk_kind = ct->store.upload(kind.data(), kind.size());
+    cd.blk_nkey = ct->store.upload(nkey.data(), nkey.size() * 4);
     cd.nfac = ct->store.upload(nfac.data(), nfac.size());
     std::vector<int> bl((size_t)nt * F * 3, -1);
     for (int t = 0; t < nt; ++t)
@@ -474,6 +490,7 @@ BatchDev BatchHandle::view() const {
     b.st_atom_off = (const int*)d_atom_off.ptr;
     b.st_row_off = (const int*)d_row_off.ptr;
     b.types = (const int*)d_types.ptr;
+    b.atom_st = (const int*)d_atom_st.ptr;
     b.pos = (const double*)d_pos.ptr;
     b.nb_off = (const int*)d_nb_off.ptr;
     b.nb_j = (const int*)d_nb_j.ptr;
@@ -488,19 +505,19 @@ static constexpr size_t kLdsBytes = 160 * 1024;
 
 XPlan plan_x(ModelHandle& m, int rows_max, bool fp64) {
     const int elem = fp64 ? 8 : 4;
-    XPlan p{};
-    const int lmax = m.md.max_l;
-    for (int CB : {64, 48, 32, 24, 16, 8}) {
-        for (int NP : {16, 8, 4}) {
-            const auto& ct = m.columns(CB);
-            XBlockParams P{};
-            P.CB = CB, P.NP = NP, P.need_max = ct.need_max, P.wz_max = ct.wz_max;
-            P.rows_max = rows_max, P.lmax = lmax;
-            const size_t sm = xblock_smem(m.md, P, elem);
+    for (int CB : {64, 32, 16}) {
+        const auto& ct = m.columns(CB);
+        const PairRec R(m.md.n_fn, m.md.n_lmh);
+        const int ppw = 64 / CB;
+        const int slot = ppw * R.prefix(ct.nkey_max);
+        if (slot > xrows_stage_limit()) continue;
+        for (int NW : {8, 4}) {
+            XRowsParams P{};
+            P.CB = CB, P.NW = NW, P.need_max = ct.need_max, P.wz_max = ct.wz_max;
+            P.rows_max = rows_max, P.ntp = m.md.n_tp, P.slot = slot;
+            const size_t sm = xrows_smem(P, elem);
             if (sm <= kLdsBytes - 1024) {
-                p.P = P;
-                p.ct = &ct;
-                p.smem = sm;
+                XPlan p{P, &ct, sm};
                 return p;
             }
         }
@@ -509,7 +526,8 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64) {
 }
 
 void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
-                  const double* roww, double* xe_sum, double* xe_sq, DevBuf& a_buf) {
+                  const double* roww, double* xe_sum, double* xe_sq, DevBuf& a_buf,
+                  DevBuf& pd_buf) {
     Device& dev = device();
     hipStream_t st = dev.stream;
     const int a0 = b.atom_off[s0], a1 = b.atom_off[s1];
@@ -523,18 +541,33 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
         launch_anlm(st, fp64, m.md, bd, a_buf.ptr, a_stride, a0, na, m.max_noconj);
     }
     check_hip(hipGetLastError(), "anlm launch");
+    const long pb = b.nb_off[a0], pe = b.nb_off[a1];
+    const PairRec R(m.md.n_fn, m.md.n_lmh);
+    pd_buf.alloc((size_t)std::max<long>(pe - pb, 1) * R.stride * elem);
+    {
+        ProfScope ps("pairdata", st);
+        launch_pairdata(st, fp64, m.md, bd, a0, na, pb, pd_buf.ptr, R, m.md.max_l);
+    }
+    check_hip(hipGetLastError(), "pairdata launch");
     int rows_max = 0;
     for (int s = s0; s < s1; ++s) rows_max = std::max(rows_max, 7 + 3 * (b.atom_off[s + 1] - b.atom_off[s]));
     XPlan pl = plan_x(m, rows_max, fp64);
     pl.P.ld = ld;
     pl.P.st0 = s0;
+    pl.P.st_end = s1;
     pl.P.row_base = b.row_off[s0];
+    pl.P.xcd_map = (s1 - s0) >= 16 ? 1 : 0;
+    pl.P.pair_base = pb;
     {
-        ProfScope ps("xblock", st);
-        launch_xblock(st, fp64, m.md, pl.ct->cd, bd, pl.ct->bw, pl.P, s1 - s0, a_buf.ptr, a_stride,
-                      a0, X, roww, xe_sum, xe_sq);
+        ProfScope ps("xrows", st);
+        launch_xrows(st, fp64, m.md, pl.ct->cd, bd, pl.ct->bw, pl.P, a_buf.ptr, a_stride, a0,
+                     pd_buf.ptr, R, X, roww, xe_sum, xe_sq);
     }
-    check_hip(hipGetLastError(), "xblock launch");
+    check_hip(hipGetLastError(), "xrows launch");
+}
+
+size_t pair_record_bytes(const ModelHandle& m, bool fp64) {
+    return (size_t)PairRec(m.md.n_fn, m.md.n_lmh).stride * (fp64 ? 8 : 4);
 }
 
 }  // namespace plm
@@ -704,7 +737,7 @@ int plm_features_x(plm_model* mp, plm_batch* bp, int precision, double* x_out) {
         const int elem = fp64 ? 8 : 4;
         Device& dev = device();
         hipStream_t st = dev.stream;
-        DevBuf xb, ab;
+        DevBuf xb, ab, pdb;
         std::vector<double> hx((size_t)F);
         std::vector<long> order(R);
         plm_batch_row_order(bp, order.data());
@@ -719,7 +752,7 @@ int plm_features_x(plm_model* mp, plm_batch* bp, int precision, double* x_out) {
             while (s1 < b->n_st && (size_t)(b->row_off[s1 + 1] - b->row_off[s0]) * row_bytes < (1ull << 30)) ++s1;
             const long r0 = b->row_off[s0], r1 = b->row_off[s1];
             xb.alloc((size_t)(r1 - r0) * row_bytes);
-            build_x_rows(*m, *b, s0, s1, fp64, xb.ptr, F, nullptr, nullptr, nullptr, ab);
+            build_x_rows(*m, *b, s0, s1, fp64, xb.ptr, F, nullptr, nullptr, nullptr, ab, pdb);
             hbuf.resize((size_t)(r1 - r0) * row_bytes);
             check_hip(hipMemcpyAsync(hbuf.data(), xb.ptr, hbuf.size(), hipMemcpyDeviceToHost, st), "copy");
             check_hip(hipStreamSynchronize(st), "sync");

@@ -59,7 +59,7 @@ struct ColumnTables {
     Store store;
     ColumnsDev cd{};
     BlockWz bw{};
-    int need_max = 0, wz_max = 0, CB = 0;
+    int need_max = 0, wz_max = 0, CB = 0, nkey_max = 0;
 };
 
 struct ModelHandle {
@@ -69,6 +69,7 @@ struct ModelHandle {
     bool built = false;
     int max_nlm = 0, max_noconj = 0;
     std::vector<std::vector<int>> host_wz_off;
+    std::vector<std::vector<int>> host_wz_ylm;  // Y key of each derivative-weight nonzero
     std::map<int, std::unique_ptr<ColumnTables>> col_cache;
     // evaluation tables
     Store pot_store;
@@ -107,14 +108,19 @@ struct BatchHandle {
 };
 
 struct XPlan {
-    XBlockParams P;
+    XRowsParams P;
     const ColumnTables* ct;
     size_t smem;
 };
 XPlan plan_x(ModelHandle& m, int rows_max, bool fp64);
 
+// X rows of structures [s0, s1) into X (row-major, leading dimension ld),
+// weighted by roww when given; a_buf and pd_buf are the order-parameter and
+// pair-record work buffers (grown as needed).
 void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
-                  const double* roww, double* xe_sum, double* xe_sq, DevBuf& a_buf);
+                  const double* roww, double* xe_sum, double* xe_sq, DevBuf& a_buf,
+                  DevBuf& pd_buf);
+size_t pair_record_bytes(const ModelHandle& m, bool fp64);
 
 struct NormalHandle {
     ModelHandle* model = nullptr;
@@ -122,7 +128,7 @@ struct NormalHandle {
     long n_rows = 0;
     DevBuf xtx;   // F_pad x F_pad, upper block triangle accumulated
     DevBuf vec;   // [xty | xe_sum | xe_sq_sum], each F_pad
-    DevBuf xbuf, abuf, wbuf, ybuf;
+    DevBuf xbuf, abuf, pdbuf, wbuf, ybuf;
 };
 
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,

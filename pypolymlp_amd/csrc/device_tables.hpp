@@ -74,6 +74,8 @@ struct ColumnsDev {
     const int* need_off;              // [n_type][n_blocks+1]
     const int* need_feat;             // local feature ids needed by the block
     const int* col_need;              // [n_type][F][3] index into the block's need list
+    const unsigned char* blk_kind;    // [n_blocks] 1: every column is a linear feature
+    const int* blk_nkey;              // [n_blocks] Y keys the block's nonzeros use
 };
 
 // A batch of structures resident in device memory.
@@ -83,6 +85,7 @@ struct BatchDev {
     const int* st_atom_off;   // [n_st+1]
     const int* st_row_off;    // [n_st+1] row offsets of X (7 + 3N per structure)
     const int* types;         // [A]
+    const int* atom_st;       // [A] structure of each atom
     const double* pos;        // [A*3] (possibly rewrapped) Cartesian positions
     const int* nb_off;        // [A+1] full neighbour list
     const int* nb_j;          // [M] global atom index

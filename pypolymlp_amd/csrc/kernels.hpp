@@ -7,7 +7,7 @@
 
 namespace plm {
 
-struct XBlockParams;
+struct XRowsParams;
 struct BlockWz;
 
 void launch_neighbor_full(hipStream_t st, int n, const int* atom_st, const int* st_atom_off,
@@ -38,16 +38,19 @@ size_t anlm_smem(const ModelDev& md, int elem, int nthreads, int max_noconj);
 void launch_anlm(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, void* a_full,
                  int a_stride, int atom0, int n_atoms, int max_noconj, int negate = 0);
 
-struct XBlockParams {
-    int CB;
-    int NP;
-    int need_max;
-    int wz_max;
-    int rows_max;
-    int lmax;
-    int ld;
-    int st0;
-    int row_base;
+struct XRowsParams {
+    int CB;        // columns per block (lanes per pair): 64, 32 or 16
+    int NW;        // waves per workgroup
+    int need_max;  // needed features per block (max over types and blocks)
+    int wz_max;    // derivative-weight nonzeros per block (max)
+    int rows_max;  // 7 + 3N of the largest structure
+    int ntp;       // type pairs
+    int slot;      // elements of a wave's record slot (>= 64/CB * record prefix)
+    int ld;        // leading dimension of X
+    int st0, st_end;
+    int row_base;  // first X row of the structure range
+    int xcd_map;   // 1: all blocks of a structure on one XCD
+    long pair_base;
 };
 
 struct BlockWz {
@@ -56,11 +59,16 @@ struct BlockWz {
     const int* need_wz_off;
 };
 
-size_t xblock_smem(const ModelDev& md, const XBlockParams& P, int elem);
-void launch_xblock(hipStream_t st, bool fp64, const ModelDev& md, const ColumnsDev& cd,
-                   const BatchDev& bd, const BlockWz& bw, const XBlockParams& P, int n_st,
-                   const void* a_full, int a_stride, int atom_base, void* X, const double* roww,
-                   double* xe_sum, double* xe_sq);
+struct PairRec;
+size_t pairdata_smem(const ModelDev& md, int elem);
+void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, int atom0,
+                     int n_atoms, long pair_base, void* pd, const PairRec& R, int lmax);
+size_t xrows_smem(const XRowsParams& P, int elem);
+int xrows_stage_limit();
+void launch_xrows(hipStream_t st, bool fp64, const ModelDev& md, const ColumnsDev& cd,
+                  const BatchDev& bd, const BlockWz& bw, const XRowsParams& P, const void* a_full,
+                  int a_stride, int atom_base, const void* pd, const PairRec& R, void* X,
+                  const double* roww, double* xe_sum, double* xe_sq);
 
 struct EvalDev;
 void launch_prodsum(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,

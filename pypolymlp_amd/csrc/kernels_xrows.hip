@@ -1,0 +1,439 @@
+// Energy, stress and force rows of X for gfx950.
+//
+// pairdata_kernel evaluates, once per batch, the radial functions and Y_lm
+// with their derivatives for every neighbour pair (PairRec records).
+//
+// xrows_kernel builds the rows of one column block of one structure with all
+// of the block's rows resident in LDS. For each atom i of the structure the
+// workgroup first evaluates the invariants and derivative weights the block
+// needs:
+//   x_f(i)        feature values (polynomial columns only)
+//   W'_i[f, h]    = d x_f / d a_h   (h = non-conjugate order parameter)
+// Then each wave takes its own pairs (i -> j) of atom i, stages their records
+// in a private LDS slot, and every lane, which owns one column, adds
+//   q = dx_c/dd = Re sum_h coef * W'_i[f, h] (f'_n Y_lm d/r + f_n dY_lm)
+// to the rows of atom j (LDS atomics; consecutive lanes hit consecutive
+// words), to a register for atom i and to six register stress sums. The pair
+// loop itself has no workgroup barrier.
+//
+// Reference loop nests:
+//   Local::compute_anlmtp_d        src/cxx/src/compute/local.cpp:116-217
+//   Features::compute_features_deriv  src/cxx/src/polymlp/polymlp_features.cpp:208-248
+//   Model::model_polynomial, model_order{1,2,3}  src/cxx/src/compute/model.cpp:159-267
+//   PyModel row layout (energy, 6 stress, 3N forces)  src/cxx/src/compute/py_model.cpp:58-106
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "device_tables.hpp"
+#include "kernels.hpp"
+#include "pair_data.hpp"
+
+namespace plm {
+
+constexpr int kPdPairs = 16;   // pairs per LDS chunk in pairdata_kernel
+constexpr int kStageRegs = 24; // staged record elements per lane (64 lanes)
+
+// ---------------------------------------------------------------------------
+// Pair records (X path direction d = r_j - r_i + T, local.cpp:149).
+// One workgroup per atom.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) pairdata_kernel(ModelDev md, BatchDev bd, int atom0,
+                                                       int n_atoms, long pair_base,
+                                                       T* __restrict__ pd, PairRec R, int lmax) {
+    extern __shared__ unsigned char smem_raw[];
+    const int ia = blockIdx.x;
+    if (ia >= n_atoms) return;
+    const int i = atom0 + ia;
+    const int t1 = bd.types[i];
+    const int a0 = bd.st_atom_off[bd.atom_st[i]];
+    const PairLayout<T> L(md.n_fn, md.n_lmh);
+    T* pairs = reinterpret_cast<T*>(smem_raw);
+    const int p0 = bd.nb_off[i], p1 = bd.nb_off[i + 1];
+    const int nfn = md.n_fn;
+    const int pre = R.prefix(R.nkey);
+    for (int c0 = p0; c0 < p1; c0 += kPdPairs) {
+        const int np = min(kPdPairs, p1 - c0);
+        __syncthreads();
+        pairs_coop<T>(md, L, pairs, np, c0, bd, t1, a0, true, 1.0, lmax);
+        for (int e = threadIdx.x; e < np * pre; e += blockDim.x) {
+            const int p = e / pre, o = e % pre;
+            T* slot = pairs + (size_t)p * L.stride;
+            T v = T(0);
+            if (o < 7) {
+                v = L.misc(slot)[o];
+            } else if (o >= 8 && o < 8 + nfn) {
+                v = L.fn(slot)[o - 8];
+            } else if (o >= 8 + nfn && o < 8 + 2 * nfn) {
+                v = L.fd(slot)[o - 8 - nfn];
+            } else if (o >= 8 + 2 * nfn) {
+                const int q = o - 8 - 2 * nfn;
+                const int key = q >> 3, c = q & 7;
+                if (c < 2) {
+                    const Cpx<T> y = L.Y(slot)[key];
+                    v = c == 0 ? y.re : y.im;
+                } else {
+                    const Cpx<T> y = L.dY(slot)[3 * key + ((c - 2) >> 1)];
+                    v = (c & 1) ? y.im : y.re;
+                }
+            }
+            // pairs beyond the cutoff keep only misc (valid = 0)
+            if (o >= 8 && L.misc(slot)[4] == T(0)) v = T(0);
+            pd[(size_t)(c0 + p - pair_base) * R.stride + o] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS carve-up of xrows_kernel (identical on host and device)
+// ---------------------------------------------------------------------------
+struct XRowsLds {
+    size_t tile, wv, xneed, nk, zr, waves, wave_stride, qoff, total;
+    __host__ __device__ XRowsLds(const XRowsParams& P, int elem) {
+        auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+        size_t o = 0;
+        tile = o;
+        o = al(o + (size_t)P.rows_max * P.CB * elem);
+        wv = o;
+        o = al(o + (size_t)P.wz_max * 2 * elem);
+        xneed = o;
+        o = al(o + (size_t)P.need_max * elem);
+        nk = o;
+        o = al(o + (size_t)P.wz_max * 4);
+        zr = o;
+        o = al(o + (size_t)P.need_max * (P.ntp + 1) * 4);
+        waves = o;
+        qoff = al((size_t)P.slot * elem);
+        wave_stride = al(qoff + (size_t)(64 / P.CB) * P.need_max * 3 * elem);
+        total = o + (size_t)P.NW * wave_stride;
+    }
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T>
+__device__ __forceinline__ Cpx<T> prod_terms(const Cpx<T>* a, const int* keys, int nk) {
+    Cpx<T> v = {T(1), T(0)};
+    for (int k = 0; k < nk; ++k) {
+        const int id = keys[k];
+        if (id < 0) break;
+        v = cmul(v, a[id]);
+    }
+    return v;
+}
+
+// d x_f / d d for one pair record: W' nonzeros z in [za, zb) of feature f
+// restricted to the pair's type pair. f_n < 1e-20 is skipped (local.cpp:164).
+template <class T>
+__device__ __forceinline__ void q_feature(const T* rec, const PairRec& R,
+                                          const Cpx<T>* __restrict__ Wv,
+                                          const int* __restrict__ nk, int za, int zb, T& q0,
+                                          T& q1, T& q2) {
+    const T* fn = rec + R.fn();
+    const T* fd = rec + R.fd();
+    const T ir = T(1) / rec[3];
+    const T ux = rec[0] * ir, uy = rec[1] * ir, uz = rec[2] * ir;
+    q0 = q1 = q2 = T(0);
+    for (int z = za; z < zb; ++z) {
+        const int pk = nk[z];
+        const int nid = pk & 255, key = (pk >> 8) - 1;
+        const T fv = fn[nid];
+        if (fv < T(1e-20)) continue;
+        const T fdv = fd[nid];
+        const Cpx<T> W = Wv[z];
+        if (key < 0) {
+            const T r0 = fdv * W.re;
+            q0 += r0 * ux, q1 += r0 * uy, q2 += r0 * uz;
+        } else {
+            const T* y = rec + R.ylm(key);
+            const T ry = (W.re * y[0] - W.im * y[1]) * fdv;
+            q0 += ry * ux + fv * (W.re * y[2] - W.im * y[3]);
+            q1 += ry * uy + fv * (W.re * y[4] - W.im * y[5]);
+            q2 += ry * uz + fv * (W.re * y[6] - W.im * y[7]);
+        }
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(512) xrows_kernel(ModelDev md, ColumnsDev cd, BatchDev bd,
+                                                    BlockWz bw, XRowsParams P,
+                                                    const Cpx<T>* __restrict__ a_full,
+                                                    int a_stride, int atom_base,
+                                                    const T* __restrict__ pd, PairRec R,
+                                                    T* __restrict__ X,
+                                                    const double* __restrict__ roww,
+                                                    double* __restrict__ xe_sum,
+                                                    double* __restrict__ xe_sq) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    int blk, s;
+    if (P.xcd_map) {
+        // the blocks of one structure run on one XCD (workgroups are dealt to
+        // the 8 XCDs round robin), so its pair records are shared in one L2
+        const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+        s = P.st0 + (k / cd.n_blocks) * 8 + xcd;
+        blk = k % cd.n_blocks;
+        if (s >= P.st_end) return;
+    } else {
+        blk = blockIdx.x % cd.n_blocks;
+        s = P.st0 + blockIdx.x / cd.n_blocks;
+    }
+    const XRowsLds Lo(P, sizeof(T));
+    T* tile = reinterpret_cast<T*>(smem_raw + Lo.tile);
+    Cpx<T>* Wv = reinterpret_cast<Cpx<T>*>(smem_raw + Lo.wv);
+    T* xneed = reinterpret_cast<T*>(smem_raw + Lo.xneed);
+    int* nk = reinterpret_cast<int*>(smem_raw + Lo.nk);
+    int* zr = reinterpret_cast<int*>(smem_raw + Lo.zr);
+
+    const int c0 = cd.blk_c0[blk];
+    const int ncol = cd.blk_c0[blk + 1] - c0;
+    const int a0 = bd.st_atom_off[s], nat = bd.st_atom_off[s + 1] - a0;
+    const int rows = 7 + 3 * nat;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int NW = blockDim.x >> 6;
+    const int ntp = md.n_tp;
+    const int CB = P.CB, PPW = 64 / CB;
+    const int col = lane % CB, sub = lane / CB;
+    const bool linear = cd.blk_kind[blk] == 1;
+    const int pre = R.prefix(cd.blk_nkey[blk]);
+    T* slot = reinterpret_cast<T*>(smem_raw + Lo.waves + (size_t)wave * Lo.wave_stride);
+    T* qb = reinterpret_cast<T*>(smem_raw + Lo.waves + (size_t)wave * Lo.wave_stride + Lo.qoff);
+    const int nb1 = cd.n_blocks + 1;
+    const int c = c0 + col;
+    const bool col_ok = col < ncol;
+    const int nfac = col_ok ? cd.nfac[c] : 0;
+
+    for (int e = tid; e < rows * CB; e += blockDim.x) tile[e] = T(0);
+
+    // this lane's first element of a staged group of records: record p0s, offset o0s
+    const int p0s = lane / pre, o0s = lane % pre;
+
+    T sacc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    T eacc = T(0);
+
+    for (int ia = 0; ia < nat; ++ia) {
+        const int i = a0 + ia;
+        const int t1 = bd.types[i];
+        const TypeDev& TT = md.t[t1];
+        const Cpx<T>* a = a_full + (size_t)(i - atom_base) * a_stride;
+        const int nf0 = cd.need_off[t1 * nb1 + blk];
+        const int nneed = cd.need_off[t1 * nb1 + blk + 1] - nf0;
+        const int* need = cd.need_feat + nf0;
+        const int z0 = bw.off[t1 * nb1 + blk];
+        const int nz = bw.off[t1 * nb1 + blk + 1] - z0;
+        const int* needwz = bw.need_wz_off + nf0;
+        __syncthreads();
+        for (int k = tid; k < nneed; k += blockDim.x) {
+            const int f = need[k];
+            T v = T(0);
+            for (int e = TT.fv_off[f]; e < TT.fv_off[f + 1]; ++e)
+                v += T(TT.fv_coeff[e]) * prod_terms<T>(a, TT.fv_keys + 4 * (size_t)e, 4).re;
+            xneed[k] = v;
+        }
+        for (int z = tid; z < nz; z += blockDim.x) {
+            const int g = bw.nz[z0 + z];
+            Cpx<T> w = {T(0), T(0)};
+            for (int e = TT.wz_term_off[g]; e < TT.wz_term_off[g + 1]; ++e) {
+                Cpx<T> pr = prod_terms<T>(a, TT.wt_keys + 3 * (size_t)e, 3);
+                if (TT.wt_conj[e]) pr.im = -pr.im;
+                const T cf = T(TT.wt_coeff[e]);
+                w.re += cf * pr.re;
+                w.im += cf * pr.im;
+            }
+            Wv[z] = w;
+            nk[z] = ((TT.wz_ylm[g] + 1) << 8) | TT.wz_nid[g];
+        }
+        for (int e = tid; e < nneed * (ntp + 1); e += blockDim.x) {
+            const int k = e / (ntp + 1), tp = e % (ntp + 1);
+            const int f = need[k];
+            zr[e] = needwz[k] + TT.wz_off[f * ntp + tp] - TT.wz_off[f * ntp];
+        }
+        __syncthreads();
+
+        // this lane's column: need indices of its factors and coefficients
+        int cn0 = -1, cn1 = -1, cn2 = -1;
+        T k0 = T(0), k1 = T(0), k2 = T(0);
+        if (col_ok) {
+            const int* cn = cd.col_need + ((size_t)t1 * cd.n_cols + c) * 3;
+            cn0 = cn[0];
+            if (nfac > 1) cn1 = cn[1];
+            if (nfac > 2) cn2 = cn[2];
+        }
+        const bool active = cn0 >= 0;
+        if (active) {
+            T e = T(0);
+            if (nfac == 1) {
+                e = xneed[cn0], k0 = T(1);
+            } else if (nfac == 2) {
+                const T x0 = xneed[cn0], x1 = xneed[cn1];
+                e = x0 * x1, k0 = x1, k1 = x0;
+            } else {
+                const T x0 = xneed[cn0], x1 = xneed[cn1], x2 = xneed[cn2];
+                e = x0 * x1 * x2, k0 = x1 * x2, k1 = x0 * x2, k2 = x0 * x1;
+            }
+            if (wave == 0 && sub == 0) eacc += e;
+        }
+        T self0 = T(0), self1 = T(0), self2 = T(0);
+        const int p0 = bd.nb_off[i], p1 = bd.nb_off[i + 1];
+
+        // records of a group of PPW consecutive pairs are staged through
+        // registers: the next group is loaded while the current one is used
+        T stage[kStageRegs];
+        auto load_group = [&](int pc) {
+            const int nel = min(PPW, p1 - pc) * pre;
+            const T* src = pd + (size_t)(pc - P.pair_base) * R.stride;
+            int p = p0s, o = o0s;
+#pragma unroll
+            for (int r = 0; r < kStageRegs; ++r) {
+                if (lane + 64 * r < nel) stage[r] = src[(size_t)p * R.stride + o];
+                o += 64;
+                while (o >= pre) o -= pre, ++p;
+            }
+        };
+        int pc = p0 + wave * PPW;
+        if (pc < p1) load_group(pc);
+        for (; pc < p1; pc += NW * PPW) {
+            const int np = min(PPW, p1 - pc);
+#pragma unroll
+            for (int r = 0; r < kStageRegs; ++r)
+                if (lane + 64 * r < np * pre) slot[lane + 64 * r] = stage[r];
+            const int nxt = pc + NW * PPW;
+            if (nxt < p1) load_group(nxt);
+            wave_sync();
+            const T* rec = slot + (size_t)sub * pre;
+            const bool pv = sub < np && rec[4] != T(0);
+            if (linear) {
+                if (pv && active) {
+                    const int tp = int(rec[5]);
+                    T v0, v1, v2;
+                    q_feature<T>(rec, R, Wv, nk, zr[cn0 * (ntp + 1) + tp],
+                                 zr[cn0 * (ntp + 1) + tp + 1], v0, v1, v2);
+                    self0 += v0, self1 += v1, self2 += v2;
+                    T* rj = tile + (size_t)(7 + 3 * int(rec[6])) * CB + col;
+                    atomicAdd(rj, -v0);
+                    atomicAdd(rj + CB, -v1);
+                    atomicAdd(rj + 2 * CB, -v2);
+                    const T dx = rec[0], dy = rec[1], dz = rec[2];
+                    sacc[0] -= v0 * dx, sacc[1] -= v1 * dy, sacc[2] -= v2 * dz;
+                    sacc[3] -= v0 * dy, sacc[4] -= v1 * dz, sacc[5] -= v2 * dx;
+                }
+            } else {
+                if (pv) {
+                    const int tp = int(rec[5]);
+                    for (int k = col; k < nneed; k += CB) {
+                        T q0, q1, q2;
+                        q_feature<T>(rec, R, Wv, nk, zr[k * (ntp + 1) + tp],
+                                     zr[k * (ntp + 1) + tp + 1], q0, q1, q2);
+                        T* qq = qb + ((size_t)sub * P.need_max + k) * 3;
+                        qq[0] = q0, qq[1] = q1, qq[2] = q2;
+                    }
+                }
+                wave_sync();
+                if (pv && active) {
+                    const T* q = qb + (size_t)sub * P.need_max * 3;
+                    T v0 = k0 * q[3 * cn0], v1 = k0 * q[3 * cn0 + 1], v2 = k0 * q[3 * cn0 + 2];
+                    if (nfac > 1) {
+                        v0 += k1 * q[3 * cn1], v1 += k1 * q[3 * cn1 + 1], v2 += k1 * q[3 * cn1 + 2];
+                    }
+                    if (nfac > 2) {
+                        v0 += k2 * q[3 * cn2], v1 += k2 * q[3 * cn2 + 1], v2 += k2 * q[3 * cn2 + 2];
+                    }
+                    self0 += v0, self1 += v1, self2 += v2;
+                    T* rj = tile + (size_t)(7 + 3 * int(rec[6])) * CB + col;
+                    atomicAdd(rj, -v0);
+                    atomicAdd(rj + CB, -v1);
+                    atomicAdd(rj + 2 * CB, -v2);
+                    const T dx = rec[0], dy = rec[1], dz = rec[2];
+                    sacc[0] -= v0 * dx, sacc[1] -= v1 * dy, sacc[2] -= v2 * dz;
+                    sacc[3] -= v0 * dy, sacc[4] -= v1 * dz, sacc[5] -= v2 * dx;
+                }
+            }
+            wave_sync();
+        }
+        if (active) {
+            T* ri = tile + (size_t)(7 + 3 * ia) * CB + col;
+            atomicAdd(ri, self0);
+            atomicAdd(ri + CB, self1);
+            atomicAdd(ri + 2 * CB, self2);
+        }
+    }
+    if (col_ok) {
+        for (int k = 0; k < 6; ++k) atomicAdd(tile + (size_t)(1 + k) * CB + col, sacc[k]);
+        if (wave == 0 && sub == 0) atomicAdd(tile + col, eacc);
+    }
+    __syncthreads();
+    // write out (weights applied per row); the raw energy row feeds the scale sums
+    const int rbase = bd.st_row_off[s] - P.row_base;
+    for (int e = tid; e < rows * CB; e += blockDim.x) {
+        const int r = e / CB, cc = e % CB;
+        if (cc >= ncol) continue;
+        const T v = tile[e];
+        const double w = roww ? roww[rbase + r] : 1.0;
+        X[(size_t)(rbase + r) * P.ld + c0 + cc] = T(w * double(v));
+        if (r == 0 && xe_sum) {
+            atomicAdd(xe_sum + c0 + cc, double(v));
+            atomicAdd(xe_sq + c0 + cc, double(v) * double(v));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+size_t pairdata_smem(const ModelDev& md, int elem) {
+    const PairLayout<double> L(md.n_fn, md.n_lmh);
+    return (size_t)kPdPairs * L.stride * elem;
+}
+
+void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, int atom0,
+                     int n_atoms, long pair_base, void* pd, const PairRec& R, int lmax) {
+    const size_t sm = pairdata_smem(md, fp64 ? 8 : 4);
+    if (fp64) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pairdata_kernel<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        pairdata_kernel<double><<<n_atoms, 64, sm, st>>>(md, bd, atom0, n_atoms, pair_base,
+                                                         reinterpret_cast<double*>(pd), R, lmax);
+    } else {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pairdata_kernel<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        pairdata_kernel<float><<<n_atoms, 64, sm, st>>>(md, bd, atom0, n_atoms, pair_base,
+                                                        reinterpret_cast<float*>(pd), R, lmax);
+    }
+}
+
+size_t xrows_smem(const XRowsParams& P, int elem) { return XRowsLds(P, elem).total; }
+
+int xrows_stage_limit() { return 64 * kStageRegs; }
+
+template <class T>
+static void launch_xrows_t(hipStream_t st, const ModelDev& md, const ColumnsDev& cd,
+                           const BatchDev& bd, const BlockWz& bw, const XRowsParams& P,
+                           const void* a_full, int a_stride, int atom_base, const void* pd,
+                           const PairRec& R, void* X, const double* roww, double* xe_sum,
+                           double* xe_sq) {
+    const size_t sm = xrows_smem(P, sizeof(T));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&xrows_kernel<T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    const int n_st = P.st_end - P.st0;
+    const long grid = P.xcd_map ? (long)((n_st + 7) / 8) * 8 * cd.n_blocks : (long)n_st * cd.n_blocks;
+    xrows_kernel<T><<<grid, 64 * P.NW, sm, st>>>(
+        md, cd, bd, bw, P, reinterpret_cast<const Cpx<T>*>(a_full), a_stride, atom_base,
+        reinterpret_cast<const T*>(pd), R, reinterpret_cast<T*>(X), roww, xe_sum, xe_sq);
+}
+
+void launch_xrows(hipStream_t st, bool fp64, const ModelDev& md, const ColumnsDev& cd,
+                  const BatchDev& bd, const BlockWz& bw, const XRowsParams& P, const void* a_full,
+                  int a_stride, int atom_base, const void* pd, const PairRec& R, void* X,
+                  const double* roww, double* xe_sum, double* xe_sq) {
+    if (fp64)
+        launch_xrows_t<double>(st, md, cd, bd, bw, P, a_full, a_stride, atom_base, pd, R, X, roww,
+                               xe_sum, xe_sq);
+    else
+        launch_xrows_t<float>(st, md, cd, bd, bw, P, a_full, a_stride, atom_base, pd, R, X, roww,
+                              xe_sum, xe_sq);
+}
+
+}  // namespace plm

@@ -15,6 +15,7 @@ namespace plm {
 
 static constexpr size_t kXBudget = 3ull << 30;   // bytes of X per sub-batch
 static constexpr size_t kABudget = 2ull << 30;   // bytes of order parameters per sub-batch
+static constexpr size_t kPBudget = 6ull << 30;   // bytes of pair records per sub-batch
 static constexpr int kChunkRows = 4096;          // FP32 accumulation span of the bf16 Gram
 
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
@@ -26,6 +27,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
     const int F = acc.F, ld = acc.F_pad;
     const int kstep = 32;
     const size_t row_bytes = (size_t)ld * elem;
+    const size_t rec_bytes = pair_record_bytes(m, fp64);
     double* xty = reinterpret_cast<double*>(acc.vec.ptr);
     double* xe_sum = xty + acc.F_pad;
     double* xe_sq = xe_sum + acc.F_pad;
@@ -35,8 +37,9 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
         auto fits = [&](int s_end) {
             const size_t rows = (size_t)(b.row_off[s_end] - b.row_off[s0]);
             const size_t atoms = (size_t)(b.atom_off[s_end] - b.atom_off[s0]);
+            const size_t pairs = (size_t)(b.nb_off[b.atom_off[s_end]] - b.nb_off[b.atom_off[s0]]);
             return (rows + kstep) * row_bytes <= kXBudget &&
-                   atoms * m.max_nlm * 2 * elem <= kABudget;
+                   atoms * m.max_nlm * 2 * elem <= kABudget && pairs * rec_bytes <= kPBudget;
         };
         while (s1 < b.n_st && fits(s1 + 1)) ++s1;
         const long r0 = b.row_off[s0], r1 = b.row_off[s1];
@@ -49,7 +52,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
         check_hip(hipMemcpyAsync(acc.wbuf.ptr, row_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
         check_hip(hipMemcpyAsync(acc.ybuf.ptr, y_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
         build_x_rows(m, b, s0, s1, fp64, acc.xbuf.ptr, ld, (const double*)acc.wbuf.ptr, xe_sum, xe_sq,
-                     acc.abuf);
+                     acc.abuf, acc.pdbuf);
         {
             ProfScope ps("gram", st);
             launch_gram(st, fp64, acc.xbuf.ptr, ld, (int)rows_pad, acc.F_pad, (double*)acc.xtx.ptr,
