@@ -11,8 +11,9 @@ A step is one pass of the hot path over the whole dataset with the
 structures, translations and neighbour lists already resident in HBM: order
 parameters, the X rows (energy, stress, force) in column blocks, and their
 accumulation into X^T X and X^T y. The headline value is the mixed form
-(FP32 features, split-bf16 x3 MFMA Gram, FP64 block sums) in rows/s; the
-plain FP64 form is timed in the same run. Per-kernel times come from HIP
+(FP32 features; the Gram of X quantised per 4096-row chunk and column to
+three 7-bit integer slices, exact on int8 MFMA with FP64 sums) in rows/s;
+the plain FP64 form is timed in the same run. Per-kernel times come from HIP
 events on the library stream (plm_profile_*).
 
 The CPU baseline times the reference's own C++ (compiled from its sources into
@@ -34,7 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BF16_PEAK_TFLOPS = 2500.0
+INT8_PEAK_TOPS = 5000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -118,7 +119,7 @@ def main():
             acc.accumulate(batch, w, y, prec)
         t1 = time.perf_counter()
         capi.profile_enable(False)
-        kern = {k: capi.profile_get(k) for k in ("anlm", "pairdata", "xrows", "gram", "xty")}
+        kern = {k: capi.profile_get(k) for k in ("anlm", "pairdata", "xrows", "quantize", "gram")}
         return (t1 - t0) / steps, kern
 
     log("timing mixed ...")
@@ -128,25 +129,36 @@ def main():
     dt_fp64, k_fp64 = run(capi.PLM_FP64, args.fp64_steps, 0)
     log(f"fp64: {dt_fp64 * 1e3:.1f} ms/step, {R / dt_fp64:.4g} rows/s, kernels {k_fp64}")
 
-    # dominant kernel of the mixed step
+    # dominant kernel of the mixed step and its roofline. Algorithmic work of
+    # the Gram: the exact 3-slice int8 Gram of [X | y] needs 9 slice products
+    # per (i <= j) pair per row, 2 ops each; the FP64 Gram 2 flops per pair per row.
     ms = {k: v[0] for k, v in k_mixed.items()}
     dom = max(ms, key=ms.get)
+    Fy = F + 1
+    pairs_ij = Fy * (Fy + 1) / 2.0
     ms_g, n_g = k_mixed["gram"]
-    algo_flops = float(R) * F * (F + 1)  # X^T X upper triangle, multiply-add = 2 flops
-    if dom == "gram":
-        per_launch = 3.0 * algo_flops / n_g  # three bf16 products per element
-        achieved = per_launch / (ms_g / n_g * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
-                "kernel": "gram_bf16x3_kernel"}
-    else:
-        ms_x, n_x = k_mixed["xrows"]
-        # algorithmic bytes of one launch: X written once in FP32 plus the
-        # order parameters and pair records read
-        per_launch = (float(R) * F * 4.0) / n_x
-        achieved = per_launch / (ms_x / n_x * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "xrows_kernel"}
+    gram_i8 = {"bound": "mfma", "kernel": "gram_i8_kernel", "unit": "TOP/s (int8)",
+               "peak": INT8_PEAK_TOPS,
+               "achieved": 9 * 2 * pairs_ij * R / (ms_g * 1e-3) / 1e12}
+    gram_i8["frac"] = gram_i8["achieved"] / INT8_PEAK_TOPS
+    ms_g64, _ = k_fp64["gram"]
+    gram_f64 = {"bound": "mfma", "kernel": "gram_f64_kernel", "unit": "TFLOP/s (fp64)",
+                "peak": FP64_PEAK_TFLOPS,
+                "achieved": 2 * pairs_ij * R / (ms_g64 * 1e-3) / 1e12}
+    gram_f64["frac"] = gram_f64["achieved"] / FP64_PEAK_TFLOPS
+    ms_x, n_x = k_mixed["xrows"]
+    # X rows: FP32 X written once, plus the pair records and order parameters
+    # read once (the algorithmic minimum; the kernel itself is VALU/LDS bound)
+    n_fn = len(spec.as_dict()["model"]["pair_params"])
+    n_lmh = (spec.max_l + 1) * (spec.max_l + 2) // 2
+    rec_bytes = 4.0 * ((8 + 2 * n_fn + 8 * n_lmh + 3) // 4 * 4)
+    x_bytes = float(R) * F * 4.0 + batch.n_pairs * rec_bytes
+    xrows = {"bound": "hbm", "kernel": "xrows_kernel", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+             "achieved": x_bytes / (ms_x * 1e-3) / 1e9}
+    xrows["frac"] = xrows["achieved"] / HBM_PEAK_GBS
+    by_name = {"gram": gram_i8, "xrows": xrows}
+    roof = dict(by_name.get(dom, gram_i8))
+    roof["traffic"] = None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -166,7 +178,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-features/bf16x3-gram/fp64-accum",
+        "dtype": "fp32 features; int8x3-slice exact Gram, fp64 sums",
         "data": "synthetic (seeded binary fcc alloy structures)",
         "config": {"workload": "binary alloy fit: 1000 x 128-atom fcc 4x4x2, rc 7, 14 Gaussians, "
                                "gtinv order 3 max_l (8,4), max_p 3, model_type 3",
@@ -177,6 +189,7 @@ def main():
         "kernel_ms_per_step_mixed": {k: v[0] / args.steps for k, v in k_mixed.items()},
         "kernel_ms_per_step_fp64": {k: v[0] / args.fp64_steps for k, v in k_fp64.items()},
         "roofline": roof,
+        "roofline_kernels": {"gram_i8": gram_i8, "gram_f64": gram_f64, "xrows": xrows},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -94,6 +94,9 @@ int plm_batch_create(const plm_model* m, int n_st, const double* axes, const dou
 void plm_batch_destroy(plm_batch* b);
 /* rows of X for the batch with forces and stresses: sum(7 + 3 n_s) */
 int plm_batch_n_rows(const plm_batch* b, long* rows);
+/* Neighbour pairs (full list) of the batch; no reference counterpart (batch
+ * statistics for throughput accounting). */
+int plm_batch_n_pairs(const plm_batch* b, long* pairs);
 
 /* PotentialModel.get_x (py_model.cpp:10-54): X (rows x F, row-major) with the
  * reference's row layout [n_st energy rows | 6 stress rows per structure |
@@ -110,9 +113,14 @@ int plm_normal_create(const plm_model* m, plm_normal** out);
 void plm_normal_destroy(plm_normal* acc);
 int plm_normal_accumulate(plm_normal* acc, plm_batch* b, const double* row_w, const double* y_w,
                           int precision);
-/* xtx: F x F full symmetric (row-major); xty, xe_sum, xe_sq_sum: F. */
+/* xtx: F x F full symmetric (row-major); xty, xe_sum, xe_sq_sum: F;
+ * y_sq: sum of squared weighted targets. X^T y and y^T y are accumulated by
+ * the same Gram pass as X^T X (targets as column F of X), so all three come
+ * from the same X; in PLM_MIXED that X is the FP32 X quantised to three
+ * 7-bit slices per (4096-row chunk, column) exponent, whose Gram is exact.
+ * Any output may be NULL. */
 int plm_normal_result(plm_normal* acc, double* xtx, double* xty, double* xe_sum,
-                      double* xe_sq_sum, long* n_rows);
+                      double* xe_sq_sum, double* y_sq, long* n_rows);
 /* Row permutation: for each row of the batch in the reference layout, the
  * batch-internal row index (batch rows are [e, 6 s, 3N f] per structure). */
 int plm_batch_row_order(const plm_batch* b, long* order);

@@ -24,7 +24,7 @@ EXPORTS = [
     "plm_last_error", "plm_device_count", "plm_model_create", "plm_model_destroy",
     "plm_model_n_features", "plm_model_type_sizes", "plm_model_set_coeffs",
     "plm_cell_translations", "plm_neighbor", "plm_batch_create", "plm_batch_destroy",
-    "plm_batch_n_rows", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
+    "plm_batch_n_rows", "plm_batch_n_pairs", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
     "plm_eval", "plm_eval_batch", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
 ]
@@ -72,13 +72,14 @@ def lib():
         L.plm_batch_destroy.argtypes = [vp]
         L.plm_batch_destroy.restype = None
         L.plm_batch_n_rows.argtypes = [vp, _lp]
+        L.plm_batch_n_pairs.argtypes = [vp, _lp]
         L.plm_batch_row_order.argtypes = [vp, _lp]
         L.plm_features_x.argtypes = [vp, vp, ctypes.c_int, _dp]
         L.plm_normal_create.argtypes = [vp, ctypes.POINTER(vp)]
         L.plm_normal_destroy.argtypes = [vp]
         L.plm_normal_destroy.restype = None
         L.plm_normal_accumulate.argtypes = [vp, vp, _dp, _dp, ctypes.c_int]
-        L.plm_normal_result.argtypes = [vp, _dp, _dp, _dp, _dp, _lp]
+        L.plm_normal_result.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _lp]
         L.plm_ridge.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _ip]
         L.plm_eval.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.plm_eval_batch.argtypes = [vp, vp, ctypes.c_int, _dp, _dp, _dp]
@@ -215,6 +216,12 @@ class Batch:
         _check(lib().plm_batch_n_rows(self._h, ctypes.byref(r)))
         return r.value
 
+    @property
+    def n_pairs(self) -> int:
+        r = ctypes.c_long()
+        _check(lib().plm_batch_n_pairs(self._h, ctypes.byref(r)))
+        return r.value
+
     def row_order(self) -> np.ndarray:
         """Batch-internal row index of each row in the reference layout."""
         o = np.zeros(self.n_rows, dtype=np.int64)
@@ -253,15 +260,18 @@ class Normal:
         _check(lib().plm_normal_accumulate(self._h, batch.handle, _pd(w), _pd(y), precision))
 
     def result(self):
+        """(xtx, xty, xe_sum, xe_sq_sum, y_sq, n_rows); xty and y_sq come from
+        the same Gram pass as xtx."""
         F = self.model.n_features
         xtx = np.zeros((F, F))
         xty = np.zeros(F)
         s = np.zeros(F)
         s2 = np.zeros(F)
+        ysq = ctypes.c_double()
         n = ctypes.c_long()
         _check(lib().plm_normal_result(self._h, _pd(xtx), _pd(xty), _pd(s), _pd(s2),
-                                       ctypes.byref(n)))
-        return xtx, xty, s, s2, n.value
+                                       ctypes.byref(ysq), ctypes.byref(n)))
+        return xtx, xty, s, s2, ysq.value, n.value
 
 
 def cell_translations(axis, positions_c, cutoff: float):

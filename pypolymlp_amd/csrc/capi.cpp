@@ -712,6 +712,10 @@ int plm_batch_n_rows(const plm_batch* b, long* rows) {
     PLM_TRY({ *rows = reinterpret_cast<const BatchHandle*>(b)->row_off.back(); });
 }
 
+int plm_batch_n_pairs(const plm_batch* b, long* pairs) {
+    PLM_TRY({ *pairs = reinterpret_cast<const BatchHandle*>(b)->n_pairs; });
+}
+
 int plm_batch_row_order(const plm_batch* bp, long* order) {
     PLM_TRY({
         const auto* b = reinterpret_cast<const BatchHandle*>(bp);
@@ -777,9 +781,9 @@ int plm_normal_create(const plm_model* mp, plm_normal** out) {
         acc->model = m;
         const int F = m->host.n_features();
         acc->F = F;
-        acc->F_pad = (F + 127) / 128 * 128;
+        acc->F_pad = (F + 1 + 127) / 128 * 128;  // X columns, then the target column F
         acc->xtx.alloc((size_t)acc->F_pad * acc->F_pad * 8);
-        acc->vec.alloc((size_t)acc->F_pad * 3 * 8);
+        acc->vec.alloc((size_t)acc->F_pad * 2 * 8);
         Device& dev = device();
         check_hip(hipMemsetAsync(acc->xtx.ptr, 0, acc->xtx.bytes, dev.stream), "memset");
         check_hip(hipMemsetAsync(acc->vec.ptr, 0, acc->vec.bytes, dev.stream), "memset");
@@ -800,28 +804,29 @@ int plm_normal_accumulate(plm_normal* ap, plm_batch* bp, const double* row_w, co
 }
 
 int plm_normal_result(plm_normal* ap, double* xtx, double* xty, double* xe_sum, double* xe_sq_sum,
-                      long* n_rows) {
+                      double* y_sq, long* n_rows) {
     PLM_TRY({
         auto* acc = reinterpret_cast<NormalHandle*>(ap);
         Device& dev = device();
         hipStream_t st = dev.stream;
         const int F = acc->F, Fp = acc->F_pad;
         check_hip(hipStreamSynchronize(st), "sync");
-        if (xtx) {
-            std::vector<double> full((size_t)Fp * Fp);
-            check_hip(hipMemcpy(full.data(), acc->xtx.ptr, full.size() * 8, hipMemcpyDeviceToHost), "copy");
-            // the upper block triangle is accumulated (64-wide tiles in FP64,
-            // 128-wide in mixed precision); its element upper triangle is
-            // valid in both, so mirror element-wise
+        // Gram of [X | y]: the upper element triangle is valid (the tiles cover
+        // the upper block triangle), so mirror element-wise; column F holds
+        // X^T y and element (F, F) y^T y
+        std::vector<double> full((size_t)Fp * Fp);
+        check_hip(hipMemcpy(full.data(), acc->xtx.ptr, full.size() * 8, hipMemcpyDeviceToHost), "copy");
+        if (xtx)
             for (int i = 0; i < F; ++i)
                 for (int j = 0; j < F; ++j)
                     xtx[(size_t)i * F + j] = i <= j ? full[(size_t)i * Fp + j] : full[(size_t)j * Fp + i];
-        }
-        std::vector<double> v((size_t)Fp * 3);
+        if (xty)
+            for (int i = 0; i < F; ++i) xty[i] = full[(size_t)i * Fp + F];
+        if (y_sq) *y_sq = full[(size_t)F * Fp + F];
+        std::vector<double> v((size_t)Fp * 2);
         check_hip(hipMemcpy(v.data(), acc->vec.ptr, v.size() * 8, hipMemcpyDeviceToHost), "copy");
-        if (xty) std::memcpy(xty, v.data(), (size_t)F * 8);
-        if (xe_sum) std::memcpy(xe_sum, v.data() + Fp, (size_t)F * 8);
-        if (xe_sq_sum) std::memcpy(xe_sq_sum, v.data() + 2 * (size_t)Fp, (size_t)F * 8);
+        if (xe_sum) std::memcpy(xe_sum, v.data(), (size_t)F * 8);
+        if (xe_sq_sum) std::memcpy(xe_sq_sum, v.data() + Fp, (size_t)F * 8);
         if (n_rows) *n_rows = acc->n_rows;
     });
 }

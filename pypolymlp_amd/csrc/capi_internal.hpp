@@ -126,9 +126,9 @@ struct NormalHandle {
     ModelHandle* model = nullptr;
     int F = 0, F_pad = 0;
     long n_rows = 0;
-    DevBuf xtx;   // F_pad x F_pad, upper block triangle accumulated
-    DevBuf vec;   // [xty | xe_sum | xe_sq_sum], each F_pad
-    DevBuf xbuf, abuf, pdbuf, wbuf, ybuf;
+    DevBuf xtx;   // F_pad x F_pad Gram of [X | y], upper triangle accumulated
+    DevBuf vec;   // [xe_sum | xe_sq_sum], each F_pad
+    DevBuf xbuf, abuf, pdbuf, wbuf, ybuf, qbuf;
 };
 
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,

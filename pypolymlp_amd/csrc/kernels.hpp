@@ -79,11 +79,17 @@ void launch_pair(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& e
                  const double* hb_d, const void* ps_e, const void* ps_f, int ps_stride,
                  double* energy, double* forces, double* stress, int pair_model);
 
-int gram_tile(bool fp64);
-int gram_kstep(bool fp64);
-void launch_gram(hipStream_t st, bool fp64, const void* X, int ld, int rows, int F_pad, double* C,
-                 int ldc, int chunk_rows);
-void launch_xty(hipStream_t st, bool fp64, const void* X, int ld, int rows, int F, const double* y,
-                double* xty);
+// X^T X with the weighted targets in column F of X (ld >= F + 1, multiple of
+// 128; rows_pad multiple of 32). Mixed: quantise X into the int8 slice buffer
+// (gram_quant_bytes) then accumulate the exact Gram of the quantised X.
+void launch_set_column(hipStream_t st, bool fp64, void* X, int ld, int col, const double* y,
+                       int rows);
+void launch_gram_f64(hipStream_t st, const double* X, int ld, int rows_pad, int F_pad, double* C,
+                     int ldc);
+size_t gram_quant_bytes(int rows_pad, int ld);
+void launch_gram_quantize(hipStream_t st, const float* X, int ld, int rows_pad, int F_pad,
+                          void* quant);
+void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int F_pad, const void* quant, double* C,
+                    int ldc);
 
 }  // namespace plm

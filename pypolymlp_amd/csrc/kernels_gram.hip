@@ -1,23 +1,31 @@
-// Gram accumulation C += X^T X (upper block triangle) and X^T y on MFMA.
+// Gram accumulation C += X^T X (upper block triangle) on MFMA. The weighted
+// targets sit in column F of X, so the same accumulation gives X^T y
+// (C[:F, F]) and y^T y (C[F, F]) from exactly the X the Gram sees.
 //
 //   plain FP64 : v_mfma_f64_16x16x4_f64, FP64 accumulation (the reference's
-//                numpy x.T @ x, src/mlp_dev/core/utils_sequential.py:94-109).
-//   mixed      : X in FP32 split as x = hi + lo (two bf16), three products
-//                hi*hi + hi*lo + lo*hi per element on v_mfma_f32_32x32x16_bf16
-//                with FP32 accumulation over a row chunk, then added to the
-//                FP64 result (block sums in FP64).
+//                numpy x.T @ x, src/pypolymlp/mlp_dev/core/utils_sequential.py:94-109).
+//   mixed      : X in FP32, quantised per (4096-row chunk, column) to a shared
+//                exponent E and three signed 7-bit integer slices,
+//                x~ = 2^(E-7) (d0 + d1 2^-7 + d2 2^-14).
+//                All nine slice products run on v_mfma_i32_32x32x32_i8 with
+//                exact int32 accumulation, one accumulator per level s+t, and
+//                are combined in FP64 once per chunk. The result is the exact
+//                Gram of x~ (positive semidefinite, consistent with X^T y),
+//                which the ill-conditioned ridge problem needs: a Gram that
+//                drops products or rounds partial sums is indefinite at the
+//                scale of the ridge parameter.
 //
-// X is row-major with leading dimension ld (>= F, multiple of the tile) and
-// rows padded with zeros to a multiple of the K step.
+// X is row-major with leading dimension ld (>= F + 1, multiple of 128) and
+// rows padded with zeros to a multiple of 32.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
 
 namespace plm {
 
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 typedef __attribute__((__vector_size__(4 * sizeof(double)))) double doublex4;
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 // Map a linear tile id to (bi, bj) with bi <= bj over nb blocks.
 __device__ __forceinline__ void tri_index(int t, int nb, int& bi, int& bj) {
@@ -77,141 +85,218 @@ __global__ void __launch_bounds__(256) gram_f64_kernel(const double* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Mixed: 128x128 output tile per 256-thread workgroup (2x2 waves of 64x64 =
-// 2x2 MFMA 32x32 blocks each), K step 32 rows; A/B panels staged in LDS as
-// bf16 hi/lo, k contiguous per column.
+// Mixed: quantisation
 // ---------------------------------------------------------------------------
-constexpr int GB_T = 128;
-constexpr int GB_K = 32;
-constexpr int GB_KP = GB_K + 8;  // padded k stride (bf16 elements)
+constexpr int kQChunk = 4096;  // rows sharing one exponent per column
+constexpr int kQK = 32;        // rows per K step (one i8 MFMA)
 
-__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
-    hi = (__bf16)x;
-    lo = (__bf16)(x - (float)hi);
-}
-
-__global__ void __launch_bounds__(256) gram_bf16x3_kernel(const float* __restrict__ X, int ld,
-                                                          int rows, int nb, int chunk_rows,
-                                                          double* __restrict__ C, int ldc) {
-    __shared__ __attribute__((aligned(16))) __bf16 Ah[GB_T * GB_KP];
-    __shared__ __attribute__((aligned(16))) __bf16 Al[GB_T * GB_KP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bh[GB_T * GB_KP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bl[GB_T * GB_KP];
-    int bi, bj;
-    tri_index(blockIdx.x, nb, bi, bj);
-    const int I0 = bi * GB_T, J0 = bj * GB_T;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
-    floatx16 acc[2][2];
-    auto zero = [&]() {
-        for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b)
-                for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
-    };
-    auto flush = [&]() {
-        for (int m = 0; m < 2; ++m)
-            for (int n = 0; n < 2; ++n)
-                for (int v = 0; v < 16; ++v) {
-                    const int i = I0 + wi + 32 * m + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
-                    const int j = J0 + wj + 32 * n + (lane & 31);
-                    C[(size_t)i * ldc + j] += (double)acc[m][n][v];
-                }
-    };
-    zero();
-    int since = 0;
-    for (int k0 = 0; k0 < rows; k0 += GB_K) {
-        __syncthreads();
-        // 32 rows x 128 cols per panel: each thread 16 consecutive columns of one row
-        {
-            const int kk = tid >> 3, cb = (tid & 7) * 16;
-            const float* xr = X + (size_t)(k0 + kk) * ld;
-            for (int c = 0; c < 16; c += 4) {
-                const float4 va = *reinterpret_cast<const float4*>(xr + I0 + cb + c);
-                const float4 vb = *reinterpret_cast<const float4*>(xr + J0 + cb + c);
-                const float a4[4] = {va.x, va.y, va.z, va.w};
-                const float b4[4] = {vb.x, vb.y, vb.z, vb.w};
-                for (int u = 0; u < 4; ++u) {
-                    __bf16 h, l;
-                    split_bf16(a4[u], h, l);
-                    Ah[(cb + c + u) * GB_KP + kk] = h;
-                    Al[(cb + c + u) * GB_KP + kk] = l;
-                    split_bf16(b4[u], h, l);
-                    Bh[(cb + c + u) * GB_KP + kk] = h;
-                    Bl[(cb + c + u) * GB_KP + kk] = l;
-                }
-            }
-        }
-        __syncthreads();
-        for (int ks = 0; ks < GB_K; ks += 16) {
-            const int kof = ks + 8 * (lane >> 5);
-            bf16x8 ah[2], al[2], bh[2], bl[2];
-            for (int m = 0; m < 2; ++m) {
-                const int row = wi + 32 * m + (lane & 31);
-                ah[m] = *reinterpret_cast<const bf16x8*>(&Ah[row * GB_KP + kof]);
-                al[m] = *reinterpret_cast<const bf16x8*>(&Al[row * GB_KP + kof]);
-            }
-            for (int n = 0; n < 2; ++n) {
-                const int col = wj + 32 * n + (lane & 31);
-                bh[n] = *reinterpret_cast<const bf16x8*>(&Bh[col * GB_KP + kof]);
-                bl[n] = *reinterpret_cast<const bf16x8*>(&Bl[col * GB_KP + kof]);
-            }
-            for (int m = 0; m < 2; ++m)
-                for (int n = 0; n < 2; ++n) {
-                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bh[n], acc[m][n], 0, 0, 0);
-                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[m], bl[n], acc[m][n], 0, 0, 0);
-                    acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[m], bh[n], acc[m][n], 0, 0, 0);
-                }
-        }
-        since += GB_K;
-        if (since >= chunk_rows) {
-            flush();
-            zero();
-            since = 0;
-        }
-    }
-    if (since > 0) flush();
-}
-
-// X^T y: xty[c] += sum_r X[r][c] * y[r]; one thread per column, rows split
-// across blockIdx.y with FP64 atomics.
-template <class T>
-__global__ void xty_kernel(const T* __restrict__ X, int ld, int rows, int F,
-                           const double* __restrict__ y, double* __restrict__ xty, int rows_per) {
+// E[chunk][c]: smallest E with |x| < 2^E over the chunk's rows (0 for a zero column).
+__global__ void colexp_kernel(const float* __restrict__ X, int ld, int rows, int ncols,
+                              int* __restrict__ E) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= F) return;
-    const int r0 = blockIdx.y * rows_per, r1 = min(rows, r0 + rows_per);
-    double s = 0.0;
-    for (int r = r0; r < r1; ++r) s += double(X[(size_t)r * ld + c]) * y[r];
-    atomicAdd(xty + c, s);
+    const int chunk = blockIdx.y;
+    if (c >= ncols) return;
+    const int r0 = chunk * kQChunk, r1 = min(rows, r0 + kQChunk);
+    float m = 0.f;
+    for (int r = r0; r < r1; ++r) m = fmaxf(m, fabsf(X[(size_t)r * ld + c]));
+    int e = 0;
+    if (m > 0.f) (void)frexpf(m, &e);
+    E[(size_t)chunk * ld + c] = e;
 }
 
-int gram_tile(bool fp64) { return fp64 ? G64_T : GB_T; }
-int gram_kstep(bool fp64) { return fp64 ? G64_K : GB_K; }
-
-void launch_gram(hipStream_t st, bool fp64, const void* X, int ld, int rows, int F_pad, double* C,
-                 int ldc, int chunk_rows) {
-    const int T = gram_tile(fp64);
-    const int nb = F_pad / T;
-    const int ntiles = nb * (nb + 1) / 2;
-    if (fp64)
-        gram_f64_kernel<<<ntiles, 256, 0, st>>>(reinterpret_cast<const double*>(X), ld, rows, nb, C,
-                                                ldc);
-    else
-        gram_bf16x3_kernel<<<ntiles, 256, 0, st>>>(reinterpret_cast<const float*>(X), ld, rows, nb,
-                                                   chunk_rows, C, ldc);
+// Q[s][rb][c][k] (int8): slice s of x[32 rb + k][c], k-contiguous per column so
+// that 16 bytes are one lane's MFMA operand.
+__global__ void quantize_kernel(const float* __restrict__ X, int ld, int rows_pad, int ncols,
+                                const int* __restrict__ E, signed char* __restrict__ Q) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int rb = blockIdx.y;
+    if (c >= ncols) return;
+    const int nrb = rows_pad / kQK;
+    const int e = E[(size_t)((rb * kQK) / kQChunk) * ld + c];
+    const size_t plane = (size_t)nrb * ld * kQK;
+    signed char* q = Q + ((size_t)rb * ld + c) * kQK;
+    for (int k0 = 0; k0 < kQK; k0 += 16) {
+        union {
+            signed char b[16];
+            v4i v;
+        } d0, d1, d2;
+        for (int k = 0; k < 16; ++k) {
+            const float x = X[(size_t)(rb * kQK + k0 + k) * ld + c];
+            const float m = ldexpf(x, 7 - e);  // |m| < 128
+            const float t0 = truncf(m);
+            const float r1 = (m - t0) * 128.f;  // exact
+            const float t1 = truncf(r1);
+            const float r2 = (r1 - t1) * 128.f;  // exact
+            const float t2 = fminf(fmaxf(rintf(r2), -127.f), 127.f);
+            d0.b[k] = (signed char)t0, d1.b[k] = (signed char)t1, d2.b[k] = (signed char)t2;
+        }
+        *reinterpret_cast<v4i*>(q + k0) = d0.v;
+        *reinterpret_cast<v4i*>(q + plane + k0) = d1.v;
+        *reinterpret_cast<v4i*>(q + 2 * plane + k0) = d2.v;
+    }
 }
 
-void launch_xty(hipStream_t st, bool fp64, const void* X, int ld, int rows, int F, const double* y,
-                double* xty) {
+// ---------------------------------------------------------------------------
+// Mixed: 128 (i) x 64 (j) output tile per 256-thread workgroup, each wave a
+// 32x64 strip as two 32x32 blocks with five int32 accumulators each (levels
+// s + t = 0..4). Operand panels (3 slices x columns x 32 rows) are staged in
+// LDS, double-buffered, with a 48-byte column stride (b128 reads without bank
+// conflicts). Tiles (bi, bj) with bj >= 2 bi cover the upper triangle.
+// ---------------------------------------------------------------------------
+constexpr int GI_TI = 128;
+constexpr int GI_TJ = 64;
+constexpr int GI_CS = 48;  // LDS bytes per column (32 + pad)
+
+__device__ __forceinline__ void tile_index_i8(int t, int nbi, int& bi, int& bj) {
+    const int nbj = 2 * nbi;
+    int r = 0, rem = t;
+    while (rem >= nbj - 2 * r) rem -= nbj - 2 * r, ++r;
+    bi = r, bj = 2 * r + rem;
+}
+
+__global__ void __launch_bounds__(256) gram_i8_kernel(const signed char* __restrict__ Q, int ld,
+                                                      int rows_pad, int nbi,
+                                                      const int* __restrict__ E,
+                                                      double* __restrict__ C, int ldc) {
+    __shared__ __attribute__((aligned(16))) signed char smA[2][3 * GI_TI * GI_CS];
+    __shared__ __attribute__((aligned(16))) signed char smB[2][3 * GI_TJ * GI_CS];
+    int bi, bj;
+    tile_index_i8(blockIdx.x, nbi, bi, bj);
+    const int I0 = bi * GI_TI, J0 = bj * GI_TJ;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave * 32;
+    const int nrb = rows_pad / kQK;
+    const size_t plane = (size_t)nrb * ld * kQK;
+    v16i acc[2][5];
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[n][k] = v16i{};
+    // global -> registers for one K step (16 B pieces): A 3 x 128 columns x 2,
+    // B 3 x 64 columns x 2 (threads < 128)
+    v4i ld_a[3], ld_b[3];
+    const int col = tid >> 1, half = tid & 1;
+    const bool has_b = tid < 2 * GI_TJ;
+    auto load = [&](int rb) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const signed char* base = Q + s * plane + (size_t)rb * ld * kQK;
+            ld_a[s] = *reinterpret_cast<const v4i*>(base + (size_t)(I0 + col) * kQK + 16 * half);
+            if (has_b)
+                ld_b[s] = *reinterpret_cast<const v4i*>(base + (size_t)(J0 + col) * kQK + 16 * half);
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            *reinterpret_cast<v4i*>(&smA[buf][(s * GI_TI + col) * GI_CS + 16 * half]) = ld_a[s];
+            if (has_b)
+                *reinterpret_cast<v4i*>(&smB[buf][(s * GI_TJ + col) * GI_CS + 16 * half]) = ld_b[s];
+        }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    const int steps_per_chunk = kQChunk / kQK;
+    for (int rb = 0; rb < nrb; ++rb) {
+        const int buf = rb & 1;
+        if (rb + 1 < nrb) load(rb + 1);
+        // operands: lane -> column (lane % 32) of its block, k half (lane / 32)
+        v4i a[3], b[2][3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            a[s] = *reinterpret_cast<const v4i*>(
+                &smA[buf][(s * GI_TI + wi + (lane & 31)) * GI_CS + 16 * (lane >> 5)]);
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
+                b[n][s] = *reinterpret_cast<const v4i*>(
+                    &smB[buf][(s * GI_TJ + 32 * n + (lane & 31)) * GI_CS + 16 * (lane >> 5)]);
+        }
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+                    acc[n][s + t] =
+                        __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b[n][t], acc[n][s + t], 0, 0, 0);
+        if ((rb + 1) % steps_per_chunk == 0 || rb + 1 == nrb) {
+            // x~_i x~_j = 2^(E_i + E_j - 14) sum_k acc_k 2^(-7k): exact int32
+            // sums combined in FP64; D[(v%4) + 8(v/4) + 4(l/32)][l%32]
+            const int* Ec = E + (size_t)(rb / steps_per_chunk) * ld;
+#pragma unroll
+            for (int n = 0; n < 2; ++n) {
+                const int j = J0 + 32 * n + (lane & 31);
+                const int ej = Ec[j];
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int i = I0 + wi + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
+                    double t = double(acc[n][4][v]);
+                    t = t * 0.0078125 + double(acc[n][3][v]);
+                    t = t * 0.0078125 + double(acc[n][2][v]);
+                    t = t * 0.0078125 + double(acc[n][1][v]);
+                    t = t * 0.0078125 + double(acc[n][0][v]);
+                    C[(size_t)i * ldc + j] += ldexp(t, Ec[i] + ej - 14);
+                }
+#pragma unroll
+                for (int k = 0; k < 5; ++k) acc[n][k] = v16i{};
+            }
+        }
+        if (rb + 1 < nrb) store(buf ^ 1);
+        __syncthreads();
+    }
+}
+
+// y (FP64, one value per row) into column col of X (FP32 or FP64)
+template <class T>
+__global__ void set_column_kernel(T* __restrict__ X, int ld, int col, const double* __restrict__ y,
+                                  int rows) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows) X[(size_t)r * ld + col] = T(y[r]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_set_column(hipStream_t st, bool fp64, void* X, int ld, int col, const double* y,
+                       int rows) {
     const int B = 256;
-    const int rows_per = 2048;
-    dim3 grid((F + B - 1) / B, (rows + rows_per - 1) / rows_per);
     if (fp64)
-        xty_kernel<double><<<grid, B, 0, st>>>(reinterpret_cast<const double*>(X), ld, rows, F, y,
-                                               xty, rows_per);
+        set_column_kernel<double><<<(rows + B - 1) / B, B, 0, st>>>(reinterpret_cast<double*>(X),
+                                                                    ld, col, y, rows);
     else
-        xty_kernel<float><<<grid, B, 0, st>>>(reinterpret_cast<const float*>(X), ld, rows, F, y,
-                                              xty, rows_per);
+        set_column_kernel<float><<<(rows + B - 1) / B, B, 0, st>>>(reinterpret_cast<float*>(X), ld,
+                                                                   col, y, rows);
+}
+
+size_t gram_quant_bytes(int rows_pad, int ld) {
+    const size_t n_chunks = ((size_t)rows_pad + kQChunk - 1) / kQChunk;
+    return 3 * (size_t)rows_pad * ld + n_chunks * ld * sizeof(int);
+}
+
+void launch_gram_f64(hipStream_t st, const double* X, int ld, int rows_pad, int F_pad, double* C,
+                     int ldc) {
+    const int nb = F_pad / G64_T;
+    gram_f64_kernel<<<nb * (nb + 1) / 2, 256, 0, st>>>(X, ld, rows_pad, nb, C, ldc);
+}
+
+void launch_gram_quantize(hipStream_t st, const float* X, int ld, int rows_pad, int F_pad,
+                          void* quant) {
+    const int n_chunks = (rows_pad + kQChunk - 1) / kQChunk;
+    signed char* Q = reinterpret_cast<signed char*>(quant);
+    int* E = reinterpret_cast<int*>(Q + 3 * (size_t)rows_pad * ld);
+    const int B = 256;
+    colexp_kernel<<<dim3((F_pad + B - 1) / B, n_chunks), B, 0, st>>>(X, ld, rows_pad, F_pad, E);
+    quantize_kernel<<<dim3((F_pad + B - 1) / B, rows_pad / kQK), B, 0, st>>>(X, ld, rows_pad,
+                                                                              F_pad, E, Q);
+}
+
+void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int F_pad, const void* quant, double* C,
+                    int ldc) {
+    const signed char* Q = reinterpret_cast<const signed char*>(quant);
+    const int* E = reinterpret_cast<const int*>(Q + 3 * (size_t)rows_pad * ld);
+    const int nbi = F_pad / GI_TI;
+    gram_i8_kernel<<<nbi * (nbi + 1), 256, 0, st>>>(Q, ld, rows_pad, nbi, E, C, ldc);
 }
 
 }  // namespace plm

@@ -2,9 +2,12 @@
 //
 // Structures are processed in sub-batches whose X rows (bounded to a few GiB)
 // are built by the column-block kernel straight into a row-major buffer with
-// the row weights applied, then folded into the FP64 Gram matrix and X^T y.
-// X for the whole dataset is never materialised. Mirrors the per-batch work of
-// _compute_products_single_batch (src/mlp_dev/core/data_sequential.py:97-155).
+// the row weights applied. The weighted targets are placed in column F of the
+// same buffer, and the Gram kernel folds the whole [X | y] block into the
+// FP64 accumulator, so X^T y and y^T y come from exactly the X that X^T X
+// sees. X for the whole dataset is never materialised. Mirrors the per-batch
+// work of _compute_products_single_batch
+// (src/pypolymlp/mlp_dev/core/data_sequential.py:97-155).
 #include <algorithm>
 #include <cstring>
 
@@ -16,7 +19,6 @@ namespace plm {
 static constexpr size_t kXBudget = 3ull << 30;   // bytes of X per sub-batch
 static constexpr size_t kABudget = 2ull << 30;   // bytes of order parameters per sub-batch
 static constexpr size_t kPBudget = 6ull << 30;   // bytes of pair records per sub-batch
-static constexpr int kChunkRows = 4096;          // FP32 accumulation span of the bf16 Gram
 
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
                        bool fp64) {
@@ -28,8 +30,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
     const int kstep = 32;
     const size_t row_bytes = (size_t)ld * elem;
     const size_t rec_bytes = pair_record_bytes(m, fp64);
-    double* xty = reinterpret_cast<double*>(acc.vec.ptr);
-    double* xe_sum = xty + acc.F_pad;
+    double* xe_sum = reinterpret_cast<double*>(acc.vec.ptr);
     double* xe_sq = xe_sum + acc.F_pad;
     int s0 = 0;
     while (s0 < b.n_st) {
@@ -53,17 +54,25 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
         check_hip(hipMemcpyAsync(acc.ybuf.ptr, y_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
         build_x_rows(m, b, s0, s1, fp64, acc.xbuf.ptr, ld, (const double*)acc.wbuf.ptr, xe_sum, xe_sq,
                      acc.abuf, acc.pdbuf);
-        {
+        launch_set_column(st, fp64, acc.xbuf.ptr, ld, F, (const double*)acc.ybuf.ptr, (int)rows);
+        check_hip(hipGetLastError(), "target column");
+        if (fp64) {
             ProfScope ps("gram", st);
-            launch_gram(st, fp64, acc.xbuf.ptr, ld, (int)rows_pad, acc.F_pad, (double*)acc.xtx.ptr,
-                        acc.F_pad, kChunkRows);
+            launch_gram_f64(st, reinterpret_cast<const double*>(acc.xbuf.ptr), ld, (int)rows_pad,
+                            acc.F_pad, (double*)acc.xtx.ptr, acc.F_pad);
+        } else {
+            acc.qbuf.alloc(gram_quant_bytes((int)rows_pad, ld));
+            {
+                ProfScope ps("quantize", st);
+                launch_gram_quantize(st, reinterpret_cast<const float*>(acc.xbuf.ptr), ld,
+                                     (int)rows_pad, acc.F_pad, acc.qbuf.ptr);
+            }
+            check_hip(hipGetLastError(), "quantize launch");
+            ProfScope ps("gram", st);
+            launch_gram_i8(st, ld, (int)rows_pad, acc.F_pad, acc.qbuf.ptr, (double*)acc.xtx.ptr,
+                           acc.F_pad);
         }
         check_hip(hipGetLastError(), "gram launch");
-        {
-            ProfScope ps("xty", st);
-            launch_xty(st, fp64, acc.xbuf.ptr, ld, (int)rows, F, (const double*)acc.ybuf.ptr, xty);
-        }
-        check_hip(hipGetLastError(), "xty launch");
         acc.n_rows += rows;
         s0 = s1;
     }

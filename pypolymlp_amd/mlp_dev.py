@@ -124,7 +124,6 @@ def calc_xtx_xty(model: capi.Model, datasets: Sequence[Dataset], precision=capi.
     if min_energy is None:
         min_energy = get_min_energy(datasets)
     acc = capi.Normal(model)
-    y_sq = 0.0
     for ds in datasets:
         ds.sort_by_atoms()
         n = len(ds.structures)
@@ -139,8 +138,7 @@ def calc_xtx_xty(model: capi.Model, datasets: Sequence[Dataset], precision=capi.
             batch = capi.Batch(model, [s.axis for s in sts], [s.positions_cartesian for s in sts],
                                [s.types for s in sts])
             acc.accumulate(batch, w, yw, precision)
-            y_sq += float(yw @ yw)
-    xtx, xty, xe_sum, xe_sq, n_rows = acc.result()
+    xtx, xty, xe_sum, xe_sq, y_sq, n_rows = acc.result()
     n_data = sum(len(d.energies) for d in datasets)
     scales, zero = compute_scales(scales, xe_sum, xe_sq, n_data,
                                   include_force=datasets[0].include_force,

@@ -19,8 +19,9 @@ invariant is relative to the size of its terms, i.e. of the typical feature,
 not of its cancelled value); its accuracy claim is made on fitted
 predictions (meV/atom, meV/A) in test_gpu_solve.py.
 
-X^T X / X^T y: normwise 1e-10 (FP64 MFMA) and 1e-4 (split bf16 x3) against
-numpy on the FP64 X.
+X^T X / X^T y / y^T y: normwise 1e-10 (FP64 MFMA) against numpy on the FP64
+X; the mixed form (exact Gram of the FP32 X quantised to three 7-bit slices)
+within 1e-5 and positive semidefinite to FP64 rounding.
 """
 
 import sys
@@ -155,7 +156,7 @@ def test_normal_equations_cfg1():
     y = rng.normal(size=Xr.shape[0])
     acc = capi.Normal(m)
     acc.accumulate(b, w, w * y, capi.PLM_FP64)
-    xtx, xty, s, s2, n = acc.result()
+    xtx, xty, s, s2, ysq, n = acc.result()
     Xw = Xr * w[:, None]
     ref_xtx = Xw.T @ Xw
     assert n == Xr.shape[0]
@@ -167,5 +168,19 @@ def test_normal_equations_cfg1():
     say("normal equations cfg1 MIXED")
     accm = capi.Normal(m)
     accm.accumulate(b, w, w * y, capi.PLM_MIXED)
-    xtxm, xtym, _, _, _ = accm.result()
-    assert np.linalg.norm(xtxm - ref_xtx) / np.linalg.norm(ref_xtx) < 1e-4
+    xtxm, xtym, _, _, ysqm, _ = accm.result()
+    rel = np.linalg.norm(xtxm - ref_xtx) / np.linalg.norm(ref_xtx)
+    say(f"  mixed X^T X normwise rel err {rel:.2e}")
+    assert rel < 1e-5
+    ref_xty = Xw.T @ (w * y)
+    assert np.linalg.norm(xtym - ref_xty) / np.linalg.norm(ref_xty) < 1e-5
+    assert ysqm == pytest.approx(float((w * y) @ (w * y)), rel=1e-5)
+    assert ysq == pytest.approx(float((w * y) @ (w * y)), rel=1e-12)
+    # the mixed Gram of [X | y] is the exact Gram of one quantised matrix:
+    # positive semidefinite to FP64 rounding (a Gram with dropped products or
+    # rounded FP32 partial sums is indefinite far above this level)
+    G = np.block([[xtxm, xtym[:, None]], [xtym[None, :], np.array([[ysqm]])]])
+    d = 1.0 / np.sqrt(np.maximum(np.diag(G), 1e-300))
+    ev = np.linalg.eigvalsh(G * d[:, None] * d[None, :])
+    say(f"  mixed [X|y] Gram, unit diagonal: eigenvalue min {ev.min():.2e} max {ev.max():.2e}")
+    assert ev.min() > -1e-10 * ev.max()
