@@ -76,6 +76,48 @@ def cpu_baseline(spec, structures, n_sample):
     }
 
 
+def eval_leg(steps: int):
+    """PolymlpEval on the 32,000-atom bcc cell (20 x 20 x 40, a = 3.30 A,
+    sigma = 0.1 A) with the coefficients of a GPU FP64 ridge fit (alpha = 1e-3)
+    of the 200-structure bcc dataset. One step = neighbour list (cell list),
+    order parameters, per-atom term contraction and the half-pair loop for
+    energy, forces and stress, with the positions already resident."""
+    from pypolymlp_amd import capi, datasets, mlp_dev
+
+    spec = datasets.config1_model()
+    ds = datasets.config1_dataset(200)
+    model = capi.Model(spec.as_dict())
+    ne = mlp_dev.calc_xtx_xty(model, [ds], precision=capi.PLM_FP64)
+    coef = mlp_dev.solver_ridge(ne.xtx, ne.xty, np.array([1e-3]))[:, 0]
+    model.set_coeffs(coef / ne.scales)
+    st = datasets.config4_structure()
+    ax, pos, types = st.axis, st.positions_cartesian, st.types
+    out = {}
+    res = {}
+    for name, prec in (("fp64", capi.PLM_FP64), ("mixed", capi.PLM_MIXED)):
+        res[name] = capi.eval_structure(model, ax, pos, types, prec)  # warm-up
+        capi.profile_reset()
+        capi.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            capi.eval_structure(model, ax, pos, types, prec)
+        dt = (time.perf_counter() - t0) / steps
+        capi.profile_enable(False)
+        out[f"eval_atoms_per_s_{name}"] = st.n_total / dt
+        out[f"eval_ms_per_step_{name}"] = dt * 1e3
+        log(f"eval {name}: {dt * 1e3:.2f} ms/step, {st.n_total / dt:.4g} atoms/s")
+    e64, f64, s64 = res["fp64"]
+    em, fm, sm = res["mixed"]
+    out["eval_mixed_vs_fp64"] = {
+        "energy_meV_per_atom": abs(em - e64) / st.n_total * 1e3,
+        "force_max_meV_per_A": float(np.abs(fm - f64).max() * 1e3),
+        "stress_max_rel": float(np.abs(sm - s64).max() / np.abs(s64).max()),
+    }
+    out["eval_workload"] = (f"{st.n_total}-atom bcc 20x20x40, a=3.30, sigma=0.1; cfg1 model "
+                            "(rc 6, 11 radial, gtinv order 2, max_l 4, max_p 2) fitted on the GPU")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +127,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--eval-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path here runs on a single MI355X; "
@@ -125,9 +168,11 @@ def main():
     log("timing mixed ...")
     dt_mixed, k_mixed = run(capi.PLM_MIXED, args.steps, args.warmup)
     log(f"mixed: {dt_mixed * 1e3:.1f} ms/step, {R / dt_mixed:.4g} rows/s, kernels {k_mixed}")
-    log("timing fp64 ...")
-    dt_fp64, k_fp64 = run(capi.PLM_FP64, args.fp64_steps, 0)
-    log(f"fp64: {dt_fp64 * 1e3:.1f} ms/step, {R / dt_fp64:.4g} rows/s, kernels {k_fp64}")
+    dt_fp64, k_fp64 = None, {}
+    if args.fp64_steps > 0:
+        log("timing fp64 ...")
+        dt_fp64, k_fp64 = run(capi.PLM_FP64, args.fp64_steps, 0)
+        log(f"fp64: {dt_fp64 * 1e3:.1f} ms/step, {R / dt_fp64:.4g} rows/s, kernels {k_fp64}")
 
     # dominant kernel of the mixed step and its roofline. Algorithmic work of
     # the Gram: the exact 3-slice int8 Gram of [X | y] needs 9 slice products
@@ -136,16 +181,19 @@ def main():
     dom = max(ms, key=ms.get)
     Fy = F + 1
     pairs_ij = Fy * (Fy + 1) / 2.0
+    # profile totals cover all timed steps: work per step x steps / total time
     ms_g, n_g = k_mixed["gram"]
     gram_i8 = {"bound": "mfma", "kernel": "gram_i8_kernel", "unit": "TOP/s (int8)",
-               "peak": INT8_PEAK_TOPS,
-               "achieved": 9 * 2 * pairs_ij * R / (ms_g * 1e-3) / 1e12}
+               "peak": INT8_PEAK_TOPS, "launches_per_step": n_g / args.steps,
+               "achieved": 9 * 2 * pairs_ij * R * args.steps / (ms_g * 1e-3) / 1e12}
     gram_i8["frac"] = gram_i8["achieved"] / INT8_PEAK_TOPS
-    ms_g64, _ = k_fp64["gram"]
-    gram_f64 = {"bound": "mfma", "kernel": "gram_f64_kernel", "unit": "TFLOP/s (fp64)",
-                "peak": FP64_PEAK_TFLOPS,
-                "achieved": 2 * pairs_ij * R / (ms_g64 * 1e-3) / 1e12}
-    gram_f64["frac"] = gram_f64["achieved"] / FP64_PEAK_TFLOPS
+    gram_f64 = None
+    if args.fp64_steps > 0:
+        ms_g64, n_g64 = k_fp64["gram"]
+        gram_f64 = {"bound": "mfma", "kernel": "gram_f64_kernel", "unit": "TFLOP/s (fp64)",
+                    "peak": FP64_PEAK_TFLOPS, "launches_per_step": n_g64 / args.fp64_steps,
+                    "achieved": 2 * pairs_ij * R * args.fp64_steps / (ms_g64 * 1e-3) / 1e12}
+        gram_f64["frac"] = gram_f64["achieved"] / FP64_PEAK_TFLOPS
     ms_x, n_x = k_mixed["xrows"]
     # X rows: FP32 X written once, plus the pair records and order parameters
     # read once (the algorithmic minimum; the kernel itself is VALU/LDS bound)
@@ -154,11 +202,14 @@ def main():
     rec_bytes = 4.0 * ((8 + 2 * n_fn + 8 * n_lmh + 3) // 4 * 4)
     x_bytes = float(R) * F * 4.0 + batch.n_pairs * rec_bytes
     xrows = {"bound": "hbm", "kernel": "xrows_kernel", "unit": "GB/s", "peak": HBM_PEAK_GBS,
-             "achieved": x_bytes / (ms_x * 1e-3) / 1e9}
+             "launches_per_step": n_x / args.steps,
+             "achieved": x_bytes * args.steps / (ms_x * 1e-3) / 1e9}
     xrows["frac"] = xrows["achieved"] / HBM_PEAK_GBS
     by_name = {"gram": gram_i8, "xrows": xrows}
     roof = dict(by_name.get(dom, gram_i8))
     roof["traffic"] = None
+
+    ev = eval_leg(args.eval_steps) if args.eval_steps > 0 else {}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -183,15 +234,16 @@ def main():
         "config": {"workload": "binary alloy fit: 1000 x 128-atom fcc 4x4x2, rc 7, 14 Gaussians, "
                                "gtinv order 3 max_l (8,4), max_p 3, model_type 3",
                    "n_structures": len(sts), "rows": R, "features": F},
-        "fp64_rows_per_s": R / dt_fp64,
-        "fp64_ms_per_step": dt_fp64 * 1e3,
-        "speedup_mixed_vs_fp64": dt_fp64 / dt_mixed,
+        "fp64_rows_per_s": R / dt_fp64 if dt_fp64 else None,
+        "fp64_ms_per_step": dt_fp64 * 1e3 if dt_fp64 else None,
+        "speedup_mixed_vs_fp64": dt_fp64 / dt_mixed if dt_fp64 else None,
         "kernel_ms_per_step_mixed": {k: v[0] / args.steps for k, v in k_mixed.items()},
         "kernel_ms_per_step_fp64": {k: v[0] / args.fp64_steps for k, v in k_fp64.items()},
         "roofline": roof,
         "roofline_kernels": {"gram_i8": gram_i8, "gram_f64": gram_f64, "xrows": xrows},
         "cpu_baseline": cpu,
     }
+    out.update(ev)
     print(json.dumps(out))
 
 

@@ -154,6 +154,10 @@ void ModelHandle::build_device() {
             }
             fvo[f + 1] = (int)fvc.size();
         }
+        // products evaluated one by one and dropped when |Re| <= 1e-20 when the
+        // type has more than 1000 of them (Features::compute_features,
+        // polymlp_features.cpp:184-193); the pair form has no such rule
+        D.fv_skip = (gt && T.prod.size() > 1000) ? 1 : 0;
         D.fv_off = up(*this, fvo);
         D.fv_coeff = up(*this, fvc);
         D.fv_keys = up(*this, fvk);
@@ -215,6 +219,9 @@ void ModelHandle::build_device() {
         D.lin_global = up(*this, T.linear_global);
         host_wz_off.push_back(wzo);
         host_wz_ylm.push_back(wzy);
+        host_wz_nid.push_back(wzn);
+        wz_max_type = std::max(wz_max_type, (int)wzn.size());
+        lin_max_type = std::max(lin_max_type, L);
     }
     built = true;
 }
@@ -314,9 +321,28 @@ const ColumnTables& ModelHandle::columns(int CB) {
     cd.need_off = ct->store.upload(need_off.data(), need_off.size() * 4);
     cd.need_feat = ct->store.upload(need.data(), need.size() * 4);
     cd.col_need = ct->store.upload(col_need.data(), col_need.size() * 4);
+    // packed (key + 1) << 8 | n_id per nonzero, and per needed feature the
+    // block-local nonzero offset of each type pair
+    std::vector<int> bw_nk(bw_nz.size()), bw_zr(need.size() * (size_t)(ntp + 1));
+    for (int t = 0; t < nt; ++t) {
+        const auto& wzo = host_wz_off[t];
+        for (int b = 0; b < nblk; ++b) {
+            for (int z = bw_off[(size_t)t * (nblk + 1) + b]; z < bw_off[(size_t)t * (nblk + 1) + b + 1]; ++z) {
+                const int g = bw_nz[z];
+                if (host_wz_nid[t][g] > 255) throw std::invalid_argument("more than 256 radial functions");
+                bw_nk[z] = ((host_wz_ylm[t][g] + 1) << 8) | host_wz_nid[t][g];
+            }
+            for (int k = need_off[(size_t)t * (nblk + 1) + b]; k < need_off[(size_t)t * (nblk + 1) + b + 1]; ++k) {
+                const int f = need[k];
+                for (int tp = 0; tp <= ntp; ++tp)
+                    bw_zr[(size_t)k * (ntp + 1) + tp] = need_wz[k] + wzo[(size_t)f * ntp + tp] - wzo[(size_t)f * ntp];
+            }
+        }
+    }
     ct->bw.off = ct->store.upload(bw_off.data(), bw_off.size() * 4);
     ct->bw.nz = ct->store.upload(bw_nz.data(), bw_nz.size() * 4);
-    ct->bw.need_wz_off = ct->store.upload(need_wz.data(), need_wz.size() * 4);
+    ct->bw.nk = ct->store.upload(bw_nk.data(), bw_nk.size() * 4);
+    ct->bw.zr = ct->store.upload(bw_zr.data(), bw_zr.size() * 4);
     ct->need_max = need_max;
     ct->wz_max = wz_max;
     ct->CB = CB;
@@ -526,27 +552,35 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64) {
 }
 
 void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
-                  const double* roww, double* xe_sum, double* xe_sq, DevBuf& a_buf,
-                  DevBuf& pd_buf) {
+                  const double* roww, double* xe_sum, double* xe_sq, XWork& work) {
     Device& dev = device();
     hipStream_t st = dev.stream;
     const int a0 = b.atom_off[s0], a1 = b.atom_off[s1];
     const int na = a1 - a0;
     const int elem = fp64 ? 8 : 4;
     const int a_stride = m.max_nlm;
-    a_buf.alloc((size_t)na * a_stride * 2 * elem);
+    work.a.alloc((size_t)na * a_stride * 2 * elem);
     BatchDev bd = b.view();
     {
         ProfScope ps("anlm", st);
-        launch_anlm(st, fp64, m.md, bd, a_buf.ptr, a_stride, a0, na, m.max_noconj);
+        launch_anlm(st, fp64, m.md, bd, work.a.ptr, a_stride, a0, na, m.max_noconj);
     }
     check_hip(hipGetLastError(), "anlm launch");
+    const int w_stride = std::max(m.wz_max_type, 1), x_stride = std::max(m.lin_max_type, 1);
+    work.w.alloc((size_t)na * w_stride * 2 * elem);
+    work.x.alloc((size_t)na * x_stride * elem);
+    {
+        ProfScope ps("wprime", st);
+        launch_wprime(st, fp64, m.md, bd, work.a.ptr, a_stride, a0, na, work.w.ptr, w_stride,
+                      work.x.ptr, x_stride, m.max_nlm);
+    }
+    check_hip(hipGetLastError(), "wprime launch");
     const long pb = b.nb_off[a0], pe = b.nb_off[a1];
     const PairRec R(m.md.n_fn, m.md.n_lmh);
-    pd_buf.alloc((size_t)std::max<long>(pe - pb, 1) * R.stride * elem);
+    work.pd.alloc((size_t)std::max<long>(pe - pb, 1) * R.stride * elem);
     {
         ProfScope ps("pairdata", st);
-        launch_pairdata(st, fp64, m.md, bd, a0, na, pb, pd_buf.ptr, R, m.md.max_l);
+        launch_pairdata(st, fp64, m.md, bd, a0, na, pb, work.pd.ptr, R, m.md.max_l);
     }
     check_hip(hipGetLastError(), "pairdata launch");
     int rows_max = 0;
@@ -560,10 +594,16 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     pl.P.pair_base = pb;
     {
         ProfScope ps("xrows", st);
-        launch_xrows(st, fp64, m.md, pl.ct->cd, bd, pl.ct->bw, pl.P, a_buf.ptr, a_stride, a0,
-                     pd_buf.ptr, R, X, roww, xe_sum, xe_sq);
+        launch_xrows(st, fp64, m.md, pl.ct->cd, bd, pl.ct->bw, pl.P, work.w.ptr, w_stride,
+                     work.x.ptr, x_stride, a0, work.pd.ptr, R, X, roww, xe_sum, xe_sq);
     }
     check_hip(hipGetLastError(), "xrows launch");
+}
+
+size_t atom_work_bytes(const ModelHandle& m, bool fp64) {
+    const size_t elem = fp64 ? 8 : 4;
+    return elem * (2 * (size_t)m.max_nlm + 2 * (size_t)std::max(m.wz_max_type, 1) +
+                   (size_t)std::max(m.lin_max_type, 1));
 }
 
 size_t pair_record_bytes(const ModelHandle& m, bool fp64) {
@@ -741,7 +781,8 @@ int plm_features_x(plm_model* mp, plm_batch* bp, int precision, double* x_out) {
         const int elem = fp64 ? 8 : 4;
         Device& dev = device();
         hipStream_t st = dev.stream;
-        DevBuf xb, ab, pdb;
+        DevBuf xb;
+        XWork work;
         std::vector<double> hx((size_t)F);
         std::vector<long> order(R);
         plm_batch_row_order(bp, order.data());
@@ -756,7 +797,7 @@ int plm_features_x(plm_model* mp, plm_batch* bp, int precision, double* x_out) {
             while (s1 < b->n_st && (size_t)(b->row_off[s1 + 1] - b->row_off[s0]) * row_bytes < (1ull << 30)) ++s1;
             const long r0 = b->row_off[s0], r1 = b->row_off[s1];
             xb.alloc((size_t)(r1 - r0) * row_bytes);
-            build_x_rows(*m, *b, s0, s1, fp64, xb.ptr, F, nullptr, nullptr, nullptr, ab, pdb);
+            build_x_rows(*m, *b, s0, s1, fp64, xb.ptr, F, nullptr, nullptr, nullptr, work);
             hbuf.resize((size_t)(r1 - r0) * row_bytes);
             check_hip(hipMemcpyAsync(hbuf.data(), xb.ptr, hbuf.size(), hipMemcpyDeviceToHost, st), "copy");
             check_hip(hipStreamSynchronize(st), "sync");

@@ -70,6 +70,8 @@ struct ModelHandle {
     int max_nlm = 0, max_noconj = 0;
     std::vector<std::vector<int>> host_wz_off;
     std::vector<std::vector<int>> host_wz_ylm;  // Y key of each derivative-weight nonzero
+    std::vector<std::vector<int>> host_wz_nid;  // radial index of each nonzero
+    int wz_max_type = 0, lin_max_type = 0;      // nonzeros / linear features, max over types
     std::map<int, std::unique_ptr<ColumnTables>> col_cache;
     // evaluation tables
     Store pot_store;
@@ -114,12 +116,18 @@ struct XPlan {
 };
 XPlan plan_x(ModelHandle& m, int rows_max, bool fp64);
 
+// Work buffers of the X build (grown as needed): order parameters, W' and
+// feature values per atom, pair records.
+struct XWork {
+    DevBuf a, w, x, pd;
+};
+
 // X rows of structures [s0, s1) into X (row-major, leading dimension ld),
-// weighted by roww when given; a_buf and pd_buf are the order-parameter and
-// pair-record work buffers (grown as needed).
+// weighted by roww when given.
 void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
-                  const double* roww, double* xe_sum, double* xe_sq, DevBuf& a_buf,
-                  DevBuf& pd_buf);
+                  const double* roww, double* xe_sum, double* xe_sq, XWork& work);
+// device bytes per atom of the X build's per-atom work buffers
+size_t atom_work_bytes(const ModelHandle& m, bool fp64);
 size_t pair_record_bytes(const ModelHandle& m, bool fp64);
 
 struct NormalHandle {
@@ -128,7 +136,8 @@ struct NormalHandle {
     long n_rows = 0;
     DevBuf xtx;   // F_pad x F_pad Gram of [X | y], upper triangle accumulated
     DevBuf vec;   // [xe_sum | xe_sq_sum], each F_pad
-    DevBuf xbuf, abuf, pdbuf, wbuf, ybuf, qbuf;
+    DevBuf xbuf, wbuf, ybuf, qbuf;
+    XWork work;
 };
 
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,

@@ -28,6 +28,8 @@ struct TypeDev {
     const int* fv_off;         // [n_lin+1]
     const double* fv_coeff;
     const int* fv_keys;        // [nterm*4], -1 padded
+    int fv_skip;               // 1: terms with |Re prod| <= 1e-20 are dropped (more than 1000
+                               // value products, polymlp_features.cpp:184-193)
     // derivative weights W'[f, h]: nonzeros sorted by (f, tp)
     const int* wz_off;         // [n_lin*n_tp + 1]
     const int* wz_nid;         // radial n_id of the head

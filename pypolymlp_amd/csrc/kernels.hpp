@@ -53,22 +53,32 @@ struct XRowsParams {
     long pair_base;
 };
 
+// Per (type, column block) derivative-weight tables: the block's nonzeros
+// (indices into the type's W' list), their packed (Y key + 1) << 8 | n_id,
+// and per needed feature the nonzero range of each type pair.
 struct BlockWz {
-    const int* off;
-    const int* nz;
-    const int* need_wz_off;
+    const int* off;          // [n_type][n_blocks+1] into nz / nk
+    const int* nz;           // type-level W' index of each nonzero
+    const int* nk;           // packed key / radial index of each nonzero
+    const int* zr;           // [need entries][n_tp+1] block-local nonzero offsets
 };
 
 struct PairRec;
 size_t pairdata_smem(const ModelDev& md, int elem);
 void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, int atom0,
                      int n_atoms, long pair_base, void* pd, const PairRec& R, int lmax);
+// W'_i[g] for every nonzero g of the atom's type and the feature values
+// x_f(i), for atoms [atom0, atom0 + n_atoms): w[(i - atom0) * w_stride + g],
+// x[(i - atom0) * x_stride + f].
+void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,
+                   const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
+                   void* x, int x_stride, int max_nlm);
 size_t xrows_smem(const XRowsParams& P, int elem);
 int xrows_stage_limit();
 void launch_xrows(hipStream_t st, bool fp64, const ModelDev& md, const ColumnsDev& cd,
-                  const BatchDev& bd, const BlockWz& bw, const XRowsParams& P, const void* a_full,
-                  int a_stride, int atom_base, const void* pd, const PairRec& R, void* X,
-                  const double* roww, double* xe_sum, double* xe_sq);
+                  const BatchDev& bd, const BlockWz& bw, const XRowsParams& P, const void* w,
+                  int w_stride, const void* x, int x_stride, int atom_base, const void* pd,
+                  const PairRec& R, void* X, const double* roww, double* xe_sum, double* xe_sq);
 
 struct EvalDev;
 void launch_prodsum(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,

@@ -52,6 +52,7 @@ __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, Ba
             Cpx<T> pr = {T(1), T(0)};
             const int* keys = TT.fv_keys + 4 * (size_t)e;
             for (int k = 0; k < 4 && keys[k] >= 0; ++k) pr = cmul(pr, a[keys[k]]);
+            if (TT.fv_skip && fabs(pr.re) <= T(1e-20)) continue;
             v += T(TT.fv_coeff[e]) * pr.re;
         }
         xs[f] = v;

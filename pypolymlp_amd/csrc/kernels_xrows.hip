@@ -85,6 +85,66 @@ __global__ void __launch_bounds__(256) pairdata_kernel(ModelDev md, BatchDev bd,
 }
 
 // ---------------------------------------------------------------------------
+// Derivative weights W'_i[g] of every nonzero g of the atom's type and the
+// feature values x_f(i), once per atom (one workgroup per atom, the order
+// parameters staged in LDS).
+//   W'[f, h] = sum over invariant terms c * prod(other factors), conjugate
+//   heads folded in (polymlp_features.cpp:208-248, prod_anlmtp_deriv);
+//   x_f = sum c * Re prod, dropping |Re prod| <= 1e-20 where the reference
+//   does (polymlp_features.cpp:184-193).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ Cpx<T> prod_terms(const Cpx<T>* a, const int* keys, int nk) {
+    Cpx<T> v = {T(1), T(0)};
+    for (int k = 0; k < nk; ++k) {
+        const int id = keys[k];
+        if (id < 0) break;
+        v = cmul(v, a[id]);
+    }
+    return v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
+                                                     const Cpx<T>* __restrict__ a_full,
+                                                     int a_stride, int atom0, int n_atoms,
+                                                     Cpx<T>* __restrict__ w, int w_stride,
+                                                     T* __restrict__ x, int x_stride) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    Cpx<T>* a = reinterpret_cast<Cpx<T>*>(smem_raw);
+    const int ia = blockIdx.x;
+    if (ia >= n_atoms) return;
+    const int i = atom0 + ia;
+    const TypeDev& TT = md.t[bd.types[i]];
+    const Cpx<T>* ag = a_full + (size_t)ia * a_stride;
+    for (int k = threadIdx.x; k < TT.n_nlm; k += blockDim.x) a[k] = ag[k];
+    __syncthreads();
+    const int nzt = TT.wz_off[TT.n_lin * md.n_tp];
+    Cpx<T>* wo = w + (size_t)ia * w_stride;
+    for (int g = threadIdx.x; g < nzt; g += blockDim.x) {
+        Cpx<T> v = {T(0), T(0)};
+        for (int e = TT.wz_term_off[g]; e < TT.wz_term_off[g + 1]; ++e) {
+            Cpx<T> pr = prod_terms<T>(a, TT.wt_keys + 3 * (size_t)e, 3);
+            if (TT.wt_conj[e]) pr.im = -pr.im;
+            const T cf = T(TT.wt_coeff[e]);
+            v.re += cf * pr.re;
+            v.im += cf * pr.im;
+        }
+        wo[g] = v;
+    }
+    T* xo = x + (size_t)ia * x_stride;
+    for (int f = threadIdx.x; f < TT.n_lin; f += blockDim.x) {
+        T v = T(0);
+        for (int e = TT.fv_off[f]; e < TT.fv_off[f + 1]; ++e) {
+            const T pr = prod_terms<T>(a, TT.fv_keys + 4 * (size_t)e, 4).re;
+            if (TT.fv_skip && fabs(pr) <= T(1e-20)) continue;
+            v += T(TT.fv_coeff[e]) * pr;
+        }
+        xo[f] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // LDS carve-up of xrows_kernel (identical on host and device)
 // ---------------------------------------------------------------------------
 struct XRowsLds {
@@ -113,17 +173,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <class T>
-__device__ __forceinline__ Cpx<T> prod_terms(const Cpx<T>* a, const int* keys, int nk) {
-    Cpx<T> v = {T(1), T(0)};
-    for (int k = 0; k < nk; ++k) {
-        const int id = keys[k];
-        if (id < 0) break;
-        v = cmul(v, a[id]);
-    }
-    return v;
 }
 
 // d x_f / d d for one pair record: W' nonzeros z in [za, zb) of feature f
@@ -161,8 +210,9 @@ __device__ __forceinline__ void q_feature(const T* rec, const PairRec& R,
 template <class T>
 __global__ void __launch_bounds__(512) xrows_kernel(ModelDev md, ColumnsDev cd, BatchDev bd,
                                                     BlockWz bw, XRowsParams P,
-                                                    const Cpx<T>* __restrict__ a_full,
-                                                    int a_stride, int atom_base,
+                                                    const Cpx<T>* __restrict__ wall, int w_stride,
+                                                    const T* __restrict__ xall, int x_stride,
+                                                    int atom_base,
                                                     const T* __restrict__ pd, PairRec R,
                                                     T* __restrict__ X,
                                                     const double* __restrict__ roww,
@@ -217,40 +267,21 @@ __global__ void __launch_bounds__(512) xrows_kernel(ModelDev md, ColumnsDev cd, 
     for (int ia = 0; ia < nat; ++ia) {
         const int i = a0 + ia;
         const int t1 = bd.types[i];
-        const TypeDev& TT = md.t[t1];
-        const Cpx<T>* a = a_full + (size_t)(i - atom_base) * a_stride;
+        const Cpx<T>* wi = wall + (size_t)(i - atom_base) * w_stride;
+        const T* xi = xall + (size_t)(i - atom_base) * x_stride;
         const int nf0 = cd.need_off[t1 * nb1 + blk];
         const int nneed = cd.need_off[t1 * nb1 + blk + 1] - nf0;
         const int* need = cd.need_feat + nf0;
         const int z0 = bw.off[t1 * nb1 + blk];
         const int nz = bw.off[t1 * nb1 + blk + 1] - z0;
-        const int* needwz = bw.need_wz_off + nf0;
         __syncthreads();
-        for (int k = tid; k < nneed; k += blockDim.x) {
-            const int f = need[k];
-            T v = T(0);
-            for (int e = TT.fv_off[f]; e < TT.fv_off[f + 1]; ++e)
-                v += T(TT.fv_coeff[e]) * prod_terms<T>(a, TT.fv_keys + 4 * (size_t)e, 4).re;
-            xneed[k] = v;
-        }
+        for (int k = tid; k < nneed; k += blockDim.x) xneed[k] = xi[need[k]];
         for (int z = tid; z < nz; z += blockDim.x) {
-            const int g = bw.nz[z0 + z];
-            Cpx<T> w = {T(0), T(0)};
-            for (int e = TT.wz_term_off[g]; e < TT.wz_term_off[g + 1]; ++e) {
-                Cpx<T> pr = prod_terms<T>(a, TT.wt_keys + 3 * (size_t)e, 3);
-                if (TT.wt_conj[e]) pr.im = -pr.im;
-                const T cf = T(TT.wt_coeff[e]);
-                w.re += cf * pr.re;
-                w.im += cf * pr.im;
-            }
-            Wv[z] = w;
-            nk[z] = ((TT.wz_ylm[g] + 1) << 8) | TT.wz_nid[g];
+            Wv[z] = wi[bw.nz[z0 + z]];
+            nk[z] = bw.nk[z0 + z];
         }
-        for (int e = tid; e < nneed * (ntp + 1); e += blockDim.x) {
-            const int k = e / (ntp + 1), tp = e % (ntp + 1);
-            const int f = need[k];
-            zr[e] = needwz[k] + TT.wz_off[f * ntp + tp] - TT.wz_off[f * ntp];
-        }
+        const int* zsrc = bw.zr + (size_t)nf0 * (ntp + 1);
+        for (int e = tid; e < nneed * (ntp + 1); e += blockDim.x) zr[e] = zsrc[e];
         __syncthreads();
 
         // this lane's column: need indices of its factors and coefficients
@@ -408,32 +439,52 @@ size_t xrows_smem(const XRowsParams& P, int elem) { return XRowsLds(P, elem).tot
 
 int xrows_stage_limit() { return 64 * kStageRegs; }
 
+void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,
+                   const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
+                   void* x, int x_stride, int max_nlm) {
+    const size_t sm = (size_t)max_nlm * 2 * (fp64 ? 8 : 4);
+    if (fp64) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&wprime_kernel<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        wprime_kernel<double><<<n_atoms, 256, sm, st>>>(
+            md, bd, reinterpret_cast<const Cpx<double>*>(a_full), a_stride, atom0, n_atoms,
+            reinterpret_cast<Cpx<double>*>(w), w_stride, reinterpret_cast<double*>(x), x_stride);
+    } else {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&wprime_kernel<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        wprime_kernel<float><<<n_atoms, 256, sm, st>>>(
+            md, bd, reinterpret_cast<const Cpx<float>*>(a_full), a_stride, atom0, n_atoms,
+            reinterpret_cast<Cpx<float>*>(w), w_stride, reinterpret_cast<float*>(x), x_stride);
+    }
+}
+
 template <class T>
 static void launch_xrows_t(hipStream_t st, const ModelDev& md, const ColumnsDev& cd,
                            const BatchDev& bd, const BlockWz& bw, const XRowsParams& P,
-                           const void* a_full, int a_stride, int atom_base, const void* pd,
-                           const PairRec& R, void* X, const double* roww, double* xe_sum,
-                           double* xe_sq) {
+                           const void* w, int w_stride, const void* x, int x_stride,
+                           int atom_base, const void* pd, const PairRec& R, void* X,
+                           const double* roww, double* xe_sum, double* xe_sq) {
     const size_t sm = xrows_smem(P, sizeof(T));
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&xrows_kernel<T>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     const int n_st = P.st_end - P.st0;
     const long grid = P.xcd_map ? (long)((n_st + 7) / 8) * 8 * cd.n_blocks : (long)n_st * cd.n_blocks;
     xrows_kernel<T><<<grid, 64 * P.NW, sm, st>>>(
-        md, cd, bd, bw, P, reinterpret_cast<const Cpx<T>*>(a_full), a_stride, atom_base,
-        reinterpret_cast<const T*>(pd), R, reinterpret_cast<T*>(X), roww, xe_sum, xe_sq);
+        md, cd, bd, bw, P, reinterpret_cast<const Cpx<T>*>(w), w_stride,
+        reinterpret_cast<const T*>(x), x_stride, atom_base, reinterpret_cast<const T*>(pd), R,
+        reinterpret_cast<T*>(X), roww, xe_sum, xe_sq);
 }
 
 void launch_xrows(hipStream_t st, bool fp64, const ModelDev& md, const ColumnsDev& cd,
-                  const BatchDev& bd, const BlockWz& bw, const XRowsParams& P, const void* a_full,
-                  int a_stride, int atom_base, const void* pd, const PairRec& R, void* X,
-                  const double* roww, double* xe_sum, double* xe_sq) {
+                  const BatchDev& bd, const BlockWz& bw, const XRowsParams& P, const void* w,
+                  int w_stride, const void* x, int x_stride, int atom_base, const void* pd,
+                  const PairRec& R, void* X, const double* roww, double* xe_sum, double* xe_sq) {
     if (fp64)
-        launch_xrows_t<double>(st, md, cd, bd, bw, P, a_full, a_stride, atom_base, pd, R, X, roww,
-                               xe_sum, xe_sq);
+        launch_xrows_t<double>(st, md, cd, bd, bw, P, w, w_stride, x, x_stride, atom_base, pd, R,
+                               X, roww, xe_sum, xe_sq);
     else
-        launch_xrows_t<float>(st, md, cd, bd, bw, P, a_full, a_stride, atom_base, pd, R, X, roww,
-                              xe_sum, xe_sq);
+        launch_xrows_t<float>(st, md, cd, bd, bw, P, w, w_stride, x, x_stride, atom_base, pd, R,
+                              X, roww, xe_sum, xe_sq);
 }
 
 }  // namespace plm

@@ -17,7 +17,7 @@
 namespace plm {
 
 static constexpr size_t kXBudget = 3ull << 30;   // bytes of X per sub-batch
-static constexpr size_t kABudget = 2ull << 30;   // bytes of order parameters per sub-batch
+static constexpr size_t kABudget = 8ull << 30;   // bytes of per-atom work (a, W', x) per sub-batch
 static constexpr size_t kPBudget = 6ull << 30;   // bytes of pair records per sub-batch
 
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
@@ -30,6 +30,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
     const int kstep = 32;
     const size_t row_bytes = (size_t)ld * elem;
     const size_t rec_bytes = pair_record_bytes(m, fp64);
+    const size_t atom_bytes = atom_work_bytes(m, fp64);
     double* xe_sum = reinterpret_cast<double*>(acc.vec.ptr);
     double* xe_sq = xe_sum + acc.F_pad;
     int s0 = 0;
@@ -40,7 +41,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
             const size_t atoms = (size_t)(b.atom_off[s_end] - b.atom_off[s0]);
             const size_t pairs = (size_t)(b.nb_off[b.atom_off[s_end]] - b.nb_off[b.atom_off[s0]]);
             return (rows + kstep) * row_bytes <= kXBudget &&
-                   atoms * m.max_nlm * 2 * elem <= kABudget && pairs * rec_bytes <= kPBudget;
+                   atoms * atom_bytes <= kABudget && pairs * rec_bytes <= kPBudget;
         };
         while (s1 < b.n_st && fits(s1 + 1)) ++s1;
         const long r0 = b.row_off[s0], r1 = b.row_off[s1];
@@ -53,7 +54,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
         check_hip(hipMemcpyAsync(acc.wbuf.ptr, row_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
         check_hip(hipMemcpyAsync(acc.ybuf.ptr, y_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
         build_x_rows(m, b, s0, s1, fp64, acc.xbuf.ptr, ld, (const double*)acc.wbuf.ptr, xe_sum, xe_sq,
-                     acc.abuf, acc.pdbuf);
+                     acc.work);
         launch_set_column(st, fp64, acc.xbuf.ptr, ld, F, (const double*)acc.ybuf.ptr, (int)rows);
         check_hip(hipGetLastError(), "target column");
         if (fp64) {

@@ -12,12 +12,8 @@ max(column max, 1e-9 * median column max) on the remaining
 cancellation-dominated columns (invariants ~1e-15 in near-symmetric cells,
 which the regression zeroes), which still exposes any indexing or sign error.
 
-Mixed precision (FP32 feature stream): per-column error relative to
-max(column max, 1e-2 * median column max) with median <= 1e-5, 99th
-percentile <= 1e-3 and maximum <= 1e-2 (FP32 rounding of a cancelling
-invariant is relative to the size of its terms, i.e. of the typical feature,
-not of its cancelled value); its accuracy claim is made on fitted
-predictions (meV/atom, meV/A) in test_gpu_solve.py.
+Mixed precision (FP32 feature stream): see check_mixed; its accuracy claim
+is made on fitted predictions (meV/atom, meV/A) in test_gpu_solve.py.
 
 X^T X / X^T y / y^T y: normwise 1e-10 (FP64 MFMA) against numpy on the FP64
 X; the mixed form (exact Gram of the FP32 X quantised to three 7-bit slices)
@@ -51,16 +47,32 @@ def colwise_rel(X, Xr):
     return np.max(np.abs(X - Xr) / scale[None, :], axis=0)
 
 
-def check_mixed(Xm, Xr):
+def check_mixed(Xm, Xr, noise):
+    """FP32 feature stream against the FP64 oracle.
+
+    Columns the oracle resolves (rotation spread <= 1e-12 of the column max):
+    column-relative error median <= 1e-5, 99th percentile <= 1e-3, max <= 1e-2.
+    Cancellation-dominated columns: error within 1e3 of the FP32 analogue of
+    the oracle's own FP64 spread (spread * eps32 / eps64), floored at FP32
+    rounding of the column max.
+    """
     colmax = np.abs(Xr).max(axis=0)
-    floor = np.maximum(colmax, 1e-2 * np.median(colmax[colmax > 0]))
-    floor[floor == 0] = 1.0
-    e = np.abs(Xm - Xr).max(axis=0) / floor
-    say(f"  mixed per-column err: median {np.median(e):.2e} p99 {np.quantile(e, 0.99):.2e} "
+    colmax[colmax == 0] = 1.0
+    err = np.abs(Xm - Xr).max(axis=0)
+    spread = noise.max(axis=0)
+    ok = spread <= 1e-12 * colmax
+    e = err[ok] / colmax[ok]
+    say(f"  mixed, resolved columns: median {np.median(e):.2e} p99 {np.quantile(e, 0.99):.2e} "
         f"max {e.max():.2e}")
     assert np.median(e) < 1e-5
     assert np.quantile(e, 0.99) < 1e-3
     assert e.max() < 1e-2
+    if (~ok).any():
+        eps_ratio = np.finfo(np.float32).eps / np.finfo(np.float64).eps
+        bound = 1e3 * np.maximum(spread * eps_ratio, np.finfo(np.float32).eps * colmax)
+        r = (err / bound)[~ok]
+        say(f"  mixed, {int((~ok).sum())} cancelling columns: max err / bound {r.max():.2e}")
+        assert r.max() <= 1.0
 
 
 def report_worst(X, Xr, noise, k=5):
@@ -114,7 +126,7 @@ def test_x_rows_mgo(model):
     assert strict <= 1e-10
     assert loose <= 1e-6
     say(f"x rows model={model} MIXED")
-    check_mixed(capi.features_x(m, b, capi.PLM_MIXED), Xr)
+    check_mixed(capi.features_x(m, b, capi.PLM_MIXED), Xr, noise)
 
 
 @pytest.mark.parametrize("cfg,n_st", [("cfg1", 3), ("cfg2", 2), ("cfg3", 4)])
@@ -140,7 +152,7 @@ def test_x_rows_configs(cfg, n_st, tmp_path):
     assert np.linalg.norm(X - Xr) <= 1e-10 * np.linalg.norm(Xr)
     assert strict <= 1e-10
     assert loose <= 1e-6
-    check_mixed(capi.features_x(m, b, capi.PLM_MIXED), Xr)
+    check_mixed(capi.features_x(m, b, capi.PLM_MIXED), Xr, noise)
 
 
 def test_normal_equations_cfg1():
