@@ -125,6 +125,20 @@ int plm_normal_result(plm_normal* acc, double* xtx, double* xty, double* xe_sum,
  * batch-internal row index (batch rows are [e, 6 s, 3N f] per structure). */
 int plm_batch_row_order(const plm_batch* b, long* order);
 
+/* Surrogate network of the hyperparameter search (no reference counterpart:
+ * the reference fits every grid candidate; utils/grid_search is the
+ * exhaustive fallback this network prunes). FP32 MLP 5-64-64-2 with SiLU,
+ * parameters [W1 (64x5) | b1 | W2 (64x64) | b2 | W3 (2x64) | b3], loss =
+ * mean (y - t)^2. Training runs `steps` full-batch Adam steps (beta 0.9 /
+ * 0.999, eps 1e-8; step count continues from t0) in one launch; params and
+ * the Adam moments are updated in place. n <= 128 for train / grad. */
+int plm_surrogate_n_params(int* n);
+int plm_surrogate_train(float* params, float* adam_m, float* adam_v, int t0, const float* x,
+                        const float* y, int n, int steps, float lr, float* loss);
+int plm_surrogate_grad(const float* params, const float* x, const float* y, int n, float* grad,
+                       float* loss);
+int plm_surrogate_predict(const float* params, const float* x, int n, float* y);
+
 /* Ridge regression on the normal equations (solvers.py:9-84): for each alpha
  * (ascending), Cholesky of (xtx + alpha I) in FP64 on the GPU, solve, and one
  * step of iterative refinement. xtx: F x F symmetric; coeffs: n_alpha x F.

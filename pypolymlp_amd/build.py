@@ -42,6 +42,8 @@ SOURCES = [
     ("kernels_xrows.hip", []),
     ("kernels_nb.hip", []),
     ("kernels_gram.hip", []),
+    ("kernels_surrogate.hip", []),
+    ("surrogate.cpp", []),
 ]
 
 

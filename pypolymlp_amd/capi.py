@@ -27,11 +27,14 @@ EXPORTS = [
     "plm_batch_n_rows", "plm_batch_n_pairs", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
     "plm_eval", "plm_eval_batch", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
+    "plm_surrogate_n_params", "plm_surrogate_train", "plm_surrogate_grad",
+    "plm_surrogate_predict",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _lp = ctypes.POINTER(ctypes.c_long)
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class ModelDesc(ctypes.Structure):
@@ -84,6 +87,11 @@ def lib():
         L.plm_eval.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.plm_eval_batch.argtypes = [vp, vp, ctypes.c_int, _dp, _dp, _dp]
         L.plm_profile_enable.argtypes = [ctypes.c_int]
+        L.plm_surrogate_n_params.argtypes = [_ip]
+        L.plm_surrogate_train.argtypes = [_fp, _fp, _fp, ctypes.c_int, _fp, _fp, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_float, _fp]
+        L.plm_surrogate_grad.argtypes = [_fp, _fp, _fp, ctypes.c_int, _fp, _fp]
+        L.plm_surrogate_predict.argtypes = [_fp, _fp, ctypes.c_int, _fp]
         L.plm_profile_get.argtypes = [ctypes.c_char_p, _dp, _lp]
         _lib = L
     return _lib
@@ -355,3 +363,49 @@ def profile_get(name: str):
     n = ctypes.c_long()
     _check(lib().plm_profile_get(name.encode(), ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+# --- surrogate network (FP32 MLP 5-64-64-2, SiLU) ---------------------------
+
+SURR_IN, SURR_HIDDEN, SURR_OUT = 5, 64, 2
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pf(a):
+    return a.ctypes.data_as(_fp)
+
+
+def surrogate_n_params() -> int:
+    n = ctypes.c_int()
+    _check(lib().plm_surrogate_n_params(ctypes.byref(n)))
+    return n.value
+
+
+def surrogate_train(params, m, v, t0: int, x, y, steps: int, lr: float) -> float:
+    """Full-batch Adam steps on the GPU; params, m, v (float32) updated in place."""
+    assert params.dtype == np.float32 and m.dtype == np.float32 and v.dtype == np.float32
+    x, y = _f(x), _f(y)
+    loss = ctypes.c_float()
+    _check(lib().plm_surrogate_train(_pf(params), _pf(m), _pf(v), int(t0), _pf(x), _pf(y),
+                                     len(x), int(steps), float(lr), ctypes.byref(loss)))
+    return loss.value
+
+
+def surrogate_grad(params, x, y):
+    """(gradient of the mean squared loss, loss) at params, on the GPU."""
+    params, x, y = _f(params), _f(x), _f(y)
+    g = np.zeros_like(params)
+    loss = ctypes.c_float()
+    _check(lib().plm_surrogate_grad(_pf(params), _pf(x), _pf(y), len(x), _pf(g),
+                                    ctypes.byref(loss)))
+    return g, loss.value
+
+
+def surrogate_predict(params, x):
+    params, x = _f(params), _f(x)
+    y = np.zeros((len(x), SURR_OUT), dtype=np.float32)
+    _check(lib().plm_surrogate_predict(_pf(params), _pf(x), len(x), _pf(y)))
+    return y

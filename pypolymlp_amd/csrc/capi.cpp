@@ -903,4 +903,31 @@ int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas,
     PLM_TRY({ ridge_solve(F, xtx, xty, alphas, n_alpha, coeffs, info); });
 }
 
+int plm_surrogate_n_params(int* n) { PLM_TRY({ *n = kSurrParams; }); }
+
+int plm_surrogate_train(float* params, float* adam_m, float* adam_v, int t0, const float* x,
+                        const float* y, int n, int steps, float lr, float* loss) {
+    PLM_TRY({
+        if (n < 1 || n > kSurrMaxBatch) throw std::invalid_argument("training batch size out of range");
+        if (steps < 1) throw std::invalid_argument("steps must be positive");
+        surrogate_run(params, adam_m, adam_v, t0, x, y, n, steps, lr, 0, nullptr, loss);
+    });
+}
+
+int plm_surrogate_grad(const float* params, const float* x, const float* y, int n, float* grad,
+                       float* loss) {
+    PLM_TRY({
+        if (n < 1 || n > kSurrMaxBatch) throw std::invalid_argument("batch size out of range");
+        surrogate_run(const_cast<float*>(params), nullptr, nullptr, 0, x, y, n, 1, 0.f, 1, grad,
+                      loss);
+    });
+}
+
+int plm_surrogate_predict(const float* params, const float* x, int n, float* y) {
+    PLM_TRY({
+        if (n < 0) throw std::invalid_argument("negative batch size");
+        surrogate_predict(params, x, n, y);
+    });
+}
+
 }  // extern "C"

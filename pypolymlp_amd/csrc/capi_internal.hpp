@@ -151,4 +151,8 @@ void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, dou
 void ridge_solve(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
                  double* coeffs, int* info);
 
+void surrogate_run(float* params, float* adam_m, float* adam_v, int t0, const float* x,
+                   const float* y, int n, int steps, float lr, int mode, float* grad, float* loss);
+void surrogate_predict(const float* params, const float* x, int n, float* y);
+
 }  // namespace plm

@@ -102,4 +102,15 @@ void launch_gram_quantize(hipStream_t st, const float* X, int ld, int rows_pad, 
 void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int F_pad, const void* quant, double* C,
                     int ldc);
 
+// Surrogate network of the hyperparameter search (FP32 MLP 5-64-64-2, SiLU).
+constexpr int kSurrIn = 5, kSurrHidden = 64, kSurrOut = 2;
+constexpr int kSurrParams = kSurrHidden * kSurrIn + kSurrHidden + kSurrHidden * kSurrHidden +
+                            kSurrHidden + kSurrOut * kSurrHidden + kSurrOut;
+constexpr int kSurrMaxBatch = 128;
+size_t surrogate_smem(int n);
+void launch_surr_train(hipStream_t st, float* P, float* M, float* V, const float* X,
+                       const float* T, int n, int steps, int t0, float lr, float beta1,
+                       float beta2, float eps, int mode, float* grad, float* loss);
+void launch_surr_predict(hipStream_t st, const float* P, const float* X, int n, float* Y);
+
 }  // namespace plm

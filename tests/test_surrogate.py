@@ -1,0 +1,82 @@
+"""Surrogate network (FP32 MLP 5-64-64-2, SiLU) of the hyperparameter search.
+
+CPU: the FP64 oracle's gradient against central finite differences, so the
+oracle's backward pass is pinned independently of the kernels.
+GPU: the HIP forward, gradient and Adam steps against the oracle:
+  forward  normwise 1e-5 relative;
+  gradient normwise 1e-4 relative (FP32 sums over 64-wide dot products);
+  50 Adam steps: parameters within 1e-3 (normwise) and loss within 1e-3
+  relative of the FP64 run from the same start.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import surrogate as oracle_net
+from pypolymlp_amd import capi
+
+
+def _init(seed=0):
+    rng = np.random.default_rng(seed)
+    parts = []
+    for shp in oracle_net.SIZES:
+        fan_in = shp[1] if len(shp) == 2 else 1
+        parts.append(rng.normal(0.0, 1.0 / np.sqrt(fan_in), size=shp).ravel() if len(shp) == 2
+                     else 0.1 * rng.normal(size=shp))
+    return np.concatenate(parts)
+
+
+def _data(n=32, seed=1):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1.5, 1.5, size=(n, 5))
+    y = np.stack([np.sin(x[:, 0]) + 0.3 * x[:, 1] * x[:, 2], np.cos(x[:, 3]) - 0.2 * x[:, 4]], 1)
+    return x, y
+
+
+def test_oracle_gradient_finite_differences():
+    p = _init()
+    x, y = _data(16)
+    _, g = oracle_net.loss_grad(p, x, y)
+    rng = np.random.default_rng(2)
+    idx = rng.choice(len(p), 40, replace=False)
+    h = 1e-6
+    for k in idx:
+        e = np.zeros_like(p)
+        e[k] = h
+        fd = (oracle_net.loss_grad(p + e, x, y)[0] - oracle_net.loss_grad(p - e, x, y)[0]) / (2 * h)
+        assert fd == pytest.approx(g[k], rel=1e-5, abs=1e-9)
+    assert oracle_net.n_params() == len(p) == 4674
+
+
+@pytest.mark.gpu
+def test_surrogate_forward_and_gradient_vs_oracle():
+    assert capi.surrogate_n_params() == oracle_net.n_params()
+    p = _init()
+    x, y = _data(32)
+    yp = capi.surrogate_predict(p, x)
+    yr, _ = oracle_net.forward(p, x)
+    assert np.linalg.norm(yp - yr) <= 1e-5 * np.linalg.norm(yr)
+    g, loss = capi.surrogate_grad(p, x, y)
+    lr, gr = oracle_net.loss_grad(p, x, y)
+    assert loss == pytest.approx(lr, rel=1e-5)
+    assert np.linalg.norm(g - gr) <= 1e-4 * np.linalg.norm(gr)
+    # predictions in chunks (more than one launch)
+    xb = np.random.default_rng(3).uniform(-1, 1, size=(300, 5))
+    assert np.linalg.norm(capi.surrogate_predict(p, xb) - oracle_net.forward(p, xb)[0]) <= \
+        1e-5 * np.linalg.norm(oracle_net.forward(p, xb)[0])
+
+
+@pytest.mark.gpu
+def test_surrogate_adam_matches_oracle():
+    p0 = _init(5)
+    x, y = _data(32, seed=6)
+    p = p0.astype(np.float32)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    loss = capi.surrogate_train(p, m, v, 0, x, y, steps=50, lr=3e-3)
+    pr, mr, vr, lr_ = oracle_net.adam(p0, np.zeros_like(p0), np.zeros_like(p0), 0, x, y, 50, 3e-3)
+    assert np.linalg.norm(p - pr) <= 1e-3 * np.linalg.norm(pr)
+    assert loss == pytest.approx(lr_, rel=1e-3)
+    # continuing from t0 = 50 reduces the loss further
+    loss2 = capi.surrogate_train(p, m, v, 50, x, y, steps=500, lr=3e-3)
+    assert loss2 < 0.5 * loss
