@@ -128,7 +128,20 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eval-steps", type=int, default=5)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run three plain 1 GiB device copies (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        x = torch.ones(256 * 1024 * 1024, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied 3 x 1 GiB")
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path here runs on a single MI355X; "
               "multi-GPU scaling is not part of this build.", file=sys.stderr)

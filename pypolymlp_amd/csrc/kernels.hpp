@@ -90,7 +90,7 @@ void launch_pair(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& e
                  double* energy, double* forces, double* stress, int pair_model);
 
 // X^T X with the weighted targets in column F of X (ld >= F + 1, multiple of
-// 128; rows_pad multiple of 32). Mixed: quantise X into the int8 slice buffer
+// 128; rows_pad multiple of 64). Mixed: quantise X into the int8 slice buffer
 // (gram_quant_bytes) then accumulate the exact Gram of the quantised X.
 void launch_set_column(hipStream_t st, bool fp64, void* X, int ld, int col, const double* y,
                        int rows);

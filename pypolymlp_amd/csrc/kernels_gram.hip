@@ -16,7 +16,7 @@
 //                scale of the ridge parameter.
 //
 // X is row-major with leading dimension ld (>= F + 1, multiple of 128) and
-// rows padded with zeros to a multiple of 32.
+// rows padded with zeros to a multiple of 64.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -139,13 +139,19 @@ __global__ void quantize_kernel(const float* __restrict__ X, int ld, int rows_pa
 // ---------------------------------------------------------------------------
 // Mixed: 128 (i) x 64 (j) output tile per 256-thread workgroup, each wave a
 // 32x64 strip as two 32x32 blocks with five int32 accumulators each (levels
-// s + t = 0..4). Operand panels (3 slices x columns x 32 rows) are staged in
-// LDS, double-buffered, with a 48-byte column stride (b128 reads without bank
-// conflicts). Tiles (bi, bj) with bj >= 2 bi cover the upper triangle.
+// s + t = 0..4). Stages of 64 rows: each thread's share of a stage is loaded
+// into registers one stage ahead and stored into one of two LDS buffers
+// (48-byte column stride, b128 reads without bank conflicts). Tiles (bi, bj) with bj >= 2 bi cover
+// the upper triangle.
 // ---------------------------------------------------------------------------
 constexpr int GI_TI = 128;
 constexpr int GI_TJ = 64;
-constexpr int GI_CS = 48;  // LDS bytes per column (32 + pad)
+constexpr int GI_CS = 48;                        // LDS bytes per column and 32-row block
+constexpr int GI_RB = 2;                         // 32-row blocks per stage
+constexpr int GI_A = 3 * GI_RB * GI_TI * GI_CS;  // A panel bytes per stage
+constexpr int GI_B = 3 * GI_RB * GI_TJ * GI_CS;  // B panel bytes per stage
+constexpr int GI_LA = 3 * GI_RB * GI_TI * 2 / 256;  // 16-byte A pieces per thread per stage
+constexpr int GI_LB = 3 * GI_RB * GI_TJ * 2 / 256;  // 16-byte B pieces per thread per stage
 
 __device__ __forceinline__ void tile_index_i8(int t, int nbi, int& bi, int& bj) {
     const int nbj = 2 * nbi;
@@ -154,95 +160,119 @@ __device__ __forceinline__ void tile_index_i8(int t, int nbi, int& bi, int& bj) 
     bi = r, bj = 2 * r + rem;
 }
 
+size_t gram_i8_smem() { return 2 * (size_t)(GI_A + GI_B); }
+
 __global__ void __launch_bounds__(256) gram_i8_kernel(const signed char* __restrict__ Q, int ld,
                                                       int rows_pad, int nbi,
                                                       const int* __restrict__ E,
                                                       double* __restrict__ C, int ldc) {
-    __shared__ __attribute__((aligned(16))) signed char smA[2][3 * GI_TI * GI_CS];
-    __shared__ __attribute__((aligned(16))) signed char smB[2][3 * GI_TJ * GI_CS];
+    extern __shared__ __attribute__((aligned(16))) signed char smem_i8[];
     int bi, bj;
     tile_index_i8(blockIdx.x, nbi, bi, bj);
     const int I0 = bi * GI_TI, J0 = bj * GI_TJ;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave * 32;
     const int nrb = rows_pad / kQK;
+    const int nst = nrb / GI_RB;
     const size_t plane = (size_t)nrb * ld * kQK;
     v16i acc[2][5];
 #pragma unroll
     for (int n = 0; n < 2; ++n)
 #pragma unroll
         for (int k = 0; k < 5; ++k) acc[n][k] = v16i{};
-    // global -> registers for one K step (16 B pieces): A 3 x 128 columns x 2,
-    // B 3 x 64 columns x 2 (threads < 128)
-    v4i ld_a[3], ld_b[3];
-    const int col = tid >> 1, half = tid & 1;
-    const bool has_b = tid < 2 * GI_TJ;
-    auto load = [&](int rb) {
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            const signed char* base = Q + s * plane + (size_t)rb * ld * kQK;
-            ld_a[s] = *reinterpret_cast<const v4i*>(base + (size_t)(I0 + col) * kQK + 16 * half);
-            if (has_b)
-                ld_b[s] = *reinterpret_cast<const v4i*>(base + (size_t)(J0 + col) * kQK + 16 * half);
-        }
+    // piece p = tid + 256 r -> ((s * GI_RB + rbo) * cols + col) * 2 + half
+    auto g_off = [&](int p, int cols, int col0, int st) -> size_t {
+        const int half = p & 1, col = (p >> 1) % cols, sr = (p >> 1) / cols;
+        const int s = sr / GI_RB, rbo = sr % GI_RB;
+        return s * plane + ((size_t)(st * GI_RB + rbo) * ld + col0 + col) * kQK + 16 * half;
     };
-    auto store = [&](int buf) {
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            *reinterpret_cast<v4i*>(&smA[buf][(s * GI_TI + col) * GI_CS + 16 * half]) = ld_a[s];
-            if (has_b)
-                *reinterpret_cast<v4i*>(&smB[buf][(s * GI_TJ + col) * GI_CS + 16 * half]) = ld_b[s];
-        }
+    auto l_off = [&](int p, int cols) -> int {
+        const int half = p & 1, col = (p >> 1) % cols, sr = (p >> 1) / cols;
+        return (sr * cols + col) * GI_CS + 16 * half;
     };
-    load(0);
-    store(0);
-    __syncthreads();
-    const int steps_per_chunk = kQChunk / kQK;
-    for (int rb = 0; rb < nrb; ++rb) {
-        const int buf = rb & 1;
-        if (rb + 1 < nrb) load(rb + 1);
-        // operands: lane -> column (lane % 32) of its block, k half (lane / 32)
-        v4i a[3], b[2][3];
+    v4i ra[GI_LA], rb[GI_LB];
+    auto load = [&](int st, v4i* ra, v4i* rb) {
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            a[s] = *reinterpret_cast<const v4i*>(
-                &smA[buf][(s * GI_TI + wi + (lane & 31)) * GI_CS + 16 * (lane >> 5)]);
+        for (int r = 0; r < GI_LA; ++r)
+            ra[r] = *reinterpret_cast<const v4i*>(Q + g_off(tid + 256 * r, GI_TI, I0, st));
+#pragma unroll
+        for (int r = 0; r < GI_LB; ++r)
+            rb[r] = *reinterpret_cast<const v4i*>(Q + g_off(tid + 256 * r, GI_TJ, J0, st));
+    };
+    auto store = [&](int buf, const v4i* ra, const v4i* rb) {
+        signed char* A = smem_i8 + (size_t)buf * (GI_A + GI_B);
+        signed char* B = A + GI_A;
+#pragma unroll
+        for (int r = 0; r < GI_LA; ++r)
+            *reinterpret_cast<v4i*>(A + l_off(tid + 256 * r, GI_TI)) = ra[r];
+#pragma unroll
+        for (int r = 0; r < GI_LB; ++r)
+            *reinterpret_cast<v4i*>(B + l_off(tid + 256 * r, GI_TJ)) = rb[r];
+    };
+    auto compute = [&](int buf) {
+        const signed char* A = smem_i8 + (size_t)buf * (GI_A + GI_B);
+        const signed char* B = A + GI_A;
+#pragma unroll 1
+        for (int rbo = 0; rbo < GI_RB; ++rbo) {
+            // lane -> column (lane % 32) of its block, k half (lane / 32)
+            v4i a[3], b[2][3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                a[s] = *reinterpret_cast<const v4i*>(
+                    A + ((s * GI_RB + rbo) * GI_TI + wi + (lane & 31)) * GI_CS + 16 * (lane >> 5));
+#pragma unroll
+                for (int n = 0; n < 2; ++n)
+                    b[n][s] = *reinterpret_cast<const v4i*>(
+                        B + ((s * GI_RB + rbo) * GI_TJ + 32 * n + (lane & 31)) * GI_CS +
+                        16 * (lane >> 5));
+            }
 #pragma unroll
             for (int n = 0; n < 2; ++n)
-                b[n][s] = *reinterpret_cast<const v4i*>(
-                    &smB[buf][(s * GI_TJ + 32 * n + (lane & 31)) * GI_CS + 16 * (lane >> 5)]);
+#pragma unroll
+                for (int s = 0; s < 3; ++s)
+#pragma unroll
+                    for (int t = 0; t < 3; ++t)
+                        acc[n][s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b[n][t],
+                                                                              acc[n][s + t], 0, 0, 0);
         }
+    };
+    // x~_i x~_j = 2^(E_i + E_j - 14) sum_k acc_k 2^(-7k): exact int32 sums
+    // combined in FP64; D[(v%4) + 8(v/4) + 4(l/32)][l%32]
+    auto flush = [&](int chunk) {
+        const int* Ec = E + (size_t)chunk * ld;
 #pragma unroll
-        for (int n = 0; n < 2; ++n)
+        for (int n = 0; n < 2; ++n) {
+            const int j = J0 + 32 * n + (lane & 31);
+            const int ej = Ec[j];
 #pragma unroll
-            for (int s = 0; s < 3; ++s)
-#pragma unroll
-                for (int t = 0; t < 3; ++t)
-                    acc[n][s + t] =
-                        __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b[n][t], acc[n][s + t], 0, 0, 0);
-        if ((rb + 1) % steps_per_chunk == 0 || rb + 1 == nrb) {
-            // x~_i x~_j = 2^(E_i + E_j - 14) sum_k acc_k 2^(-7k): exact int32
-            // sums combined in FP64; D[(v%4) + 8(v/4) + 4(l/32)][l%32]
-            const int* Ec = E + (size_t)(rb / steps_per_chunk) * ld;
-#pragma unroll
-            for (int n = 0; n < 2; ++n) {
-                const int j = J0 + 32 * n + (lane & 31);
-                const int ej = Ec[j];
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int i = I0 + wi + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
-                    double t = double(acc[n][4][v]);
-                    t = t * 0.0078125 + double(acc[n][3][v]);
-                    t = t * 0.0078125 + double(acc[n][2][v]);
-                    t = t * 0.0078125 + double(acc[n][1][v]);
-                    t = t * 0.0078125 + double(acc[n][0][v]);
-                    C[(size_t)i * ldc + j] += ldexp(t, Ec[i] + ej - 14);
-                }
-#pragma unroll
-                for (int k = 0; k < 5; ++k) acc[n][k] = v16i{};
+            for (int v = 0; v < 16; ++v) {
+                const int i = I0 + wi + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
+                double t = double(acc[n][4][v]);
+                t = t * 0.0078125 + double(acc[n][3][v]);
+                t = t * 0.0078125 + double(acc[n][2][v]);
+                t = t * 0.0078125 + double(acc[n][1][v]);
+                t = t * 0.0078125 + double(acc[n][0][v]);
+                C[(size_t)i * ldc + j] += ldexp(t, Ec[i] + ej - 14);
             }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc[n][k] = v16i{};
         }
-        if (rb + 1 < nrb) store(buf ^ 1);
+    };
+    const int st_per_chunk = kQChunk / (kQK * GI_RB);
+    // stage st + 1 is stored to LDS at the start of iteration st (its loads
+    // were issued one iteration earlier), then stage st + 2 is loaded into the
+    // freed registers while stage st is multiplied
+    load(0, ra, rb);
+    store(0, ra, rb);
+    if (nst > 1) load(1, ra, rb);
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        if (st + 1 < nst) {
+            store((st + 1) & 1, ra, rb);
+            if (st + 2 < nst) load(st + 2, ra, rb);
+        }
+        compute(st & 1);
+        if ((st + 1) % st_per_chunk == 0 || st + 1 == nst) flush(st / st_per_chunk);
         __syncthreads();
     }
 }
@@ -296,7 +326,10 @@ void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int F_pad, const void*
     const signed char* Q = reinterpret_cast<const signed char*>(quant);
     const int* E = reinterpret_cast<const int*>(Q + 3 * (size_t)rows_pad * ld);
     const int nbi = F_pad / GI_TI;
-    gram_i8_kernel<<<nbi * (nbi + 1), 256, 0, st>>>(Q, ld, rows_pad, nbi, E, C, ldc);
+    const size_t sm = gram_i8_smem();
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&gram_i8_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    gram_i8_kernel<<<nbi * (nbi + 1), 256, sm, st>>>(Q, ld, rows_pad, nbi, E, C, ldc);
 }
 
 }  // namespace plm

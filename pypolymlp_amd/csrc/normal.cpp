@@ -27,7 +27,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
     hipStream_t st = dev.stream;
     const int elem = fp64 ? 8 : 4;
     const int F = acc.F, ld = acc.F_pad;
-    const int kstep = 32;
+    const int kstep = 64;  // rows_pad multiple of the Gram's 64-row stage
     const size_t row_bytes = (size_t)ld * elem;
     const size_t rec_bytes = pair_record_bytes(m, fp64);
     const size_t atom_bytes = atom_work_bytes(m, fp64);
