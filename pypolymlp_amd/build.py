@@ -43,6 +43,8 @@ SOURCES = [
     ("kernels_nb.hip", []),
     ("kernels_gram.hip", []),
     ("kernels_surrogate.hip", []),
+    ("kernels_poly.hip", []),
+    ("poly.cpp", []),
     ("surrogate.cpp", []),
 ]
 
@@ -61,7 +63,7 @@ def build_library(jobs: int = 8) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda a: _compile(*a), SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stderr}")

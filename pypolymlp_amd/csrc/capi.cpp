@@ -227,8 +227,9 @@ void ModelHandle::build_device() {
 }
 
 // Column recipes and per-(type, block) tables for a block width CB.
-const ColumnTables& ModelHandle::columns(int CB) {
-    auto it = col_cache.find(CB);
+const ColumnTables& ModelHandle::columns(int CB, int cover) {
+    const int key = CB * 2 + (cover < host.n_features() ? 1 : 0);
+    auto it = col_cache.find(key);
     if (it != col_cache.end()) return *it->second;
     const Model& M = host;
     const int nt = M.n_type(), F = M.n_features(), nlin = M.n_linear(), ntp = M.n_tp();
@@ -251,9 +252,11 @@ const ColumnTables& ModelHandle::columns(int CB) {
         const auto& lg = M.type(t).linear_global;
         for (size_t k = 0; k < lg.size(); ++k) g2l[t][lg[k]] = (int)k;
     }
-    const int nblk = (F + CB - 1) / CB;
+    // blocks over columns [0, cover): all columns, or only the linear ones when
+    // the polynomial columns are built by the factor-derivative GEMM
+    const int nblk = (cover + CB - 1) / CB;
     std::vector<int> c0(nblk + 1);
-    for (int b = 0; b <= nblk; ++b) c0[b] = std::min(F, b * CB);
+    for (int b = 0; b <= nblk; ++b) c0[b] = std::min(cover, b * CB);
     std::vector<int> need_off((size_t)nt * (nblk + 1), 0), need, col_need((size_t)nt * F * 3, -1);
     std::vector<int> bw_off((size_t)nt * (nblk + 1), 0), bw_nz, need_wz;
     int need_max = 1, wz_max = 1;
@@ -347,7 +350,7 @@ const ColumnTables& ModelHandle::columns(int CB) {
     ct->wz_max = wz_max;
     ct->CB = CB;
     auto& ref = *ct;
-    col_cache[CB] = std::move(ct);
+    col_cache[key] = std::move(ct);
     return ref;
 }
 
@@ -529,10 +532,10 @@ BatchDev BatchHandle::view() const {
 // ---------------------------------------------------------------------------
 static constexpr size_t kLdsBytes = 160 * 1024;
 
-XPlan plan_x(ModelHandle& m, int rows_max, bool fp64) {
+XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover) {
     const int elem = fp64 ? 8 : 4;
     for (int CB : {64, 32, 16}) {
-        const auto& ct = m.columns(CB);
+        const auto& ct = m.columns(CB, cover);
         const PairRec R(m.md.n_fn, m.md.n_lmh);
         const int ppw = 64 / CB;
         const int slot = ppw * R.prefix(ct.nkey_max);
@@ -585,7 +588,8 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     check_hip(hipGetLastError(), "pairdata launch");
     int rows_max = 0;
     for (int s = s0; s < s1; ++s) rows_max = std::max(rows_max, 7 + 3 * (b.atom_off[s + 1] - b.atom_off[s]));
-    XPlan pl = plan_x(m, rows_max, fp64);
+    const bool use_poly = m.poly().enabled;
+    XPlan pl = plan_x(m, rows_max, fp64, use_poly ? m.host.n_linear() : m.host.n_features());
     pl.P.ld = ld;
     pl.P.st0 = s0;
     pl.P.st_end = s1;
@@ -598,6 +602,9 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
                      work.x.ptr, x_stride, a0, work.pd.ptr, R, X, roww, xe_sum, xe_sq);
     }
     check_hip(hipGetLastError(), "xrows launch");
+    if (use_poly)
+        build_poly_rows(m, b, s0, s1, fp64, X, ld, roww, xe_sum, xe_sq, work, pb, R, w_stride,
+                        x_stride);
 }
 
 size_t atom_work_bytes(const ModelHandle& m, bool fp64) {

@@ -12,6 +12,7 @@
 #include "device_tables.hpp"
 #include "eval_tables.hpp"
 #include "kernels.hpp"
+#include "pair_data.hpp"
 #include "model.hpp"
 
 namespace plm {
@@ -62,6 +63,12 @@ struct ColumnTables {
     int need_max = 0, wz_max = 0, CB = 0, nkey_max = 0;
 };
 
+struct PolyTables {
+    Store store;
+    PolyDev dev{};
+    bool enabled = false;
+};
+
 struct ModelHandle {
     Model host;
     Store store;
@@ -80,7 +87,11 @@ struct ModelHandle {
     int max_heads = 0, max_lin = 0;
     explicit ModelHandle(const ModelInput& in);
     void build_device();
-    const ColumnTables& columns(int CB);
+    // column-block tables for blocks of CB columns over columns [0, cover)
+    const ColumnTables& columns(int CB, int cover);
+    // tables of the polynomial columns built by the factor-derivative GEMM
+    std::unique_ptr<PolyTables> poly_tables;
+    const PolyTables& poly();
 };
 
 struct BatchHandle {
@@ -114,12 +125,13 @@ struct XPlan {
     const ColumnTables* ct;
     size_t smem;
 };
-XPlan plan_x(ModelHandle& m, int rows_max, bool fp64);
+XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover);
 
 // Work buffers of the X build (grown as needed): order parameters, W' and
 // feature values per atom, pair records.
 struct XWork {
     DevBuf a, w, x, pd;
+    DevBuf xf, m, d, doff, z;  // polynomial columns: factor values, monomials, D, Z
 };
 
 // X rows of structures [s0, s1) into X (row-major, leading dimension ld),
@@ -128,6 +140,12 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
                   const double* roww, double* xe_sum, double* xe_sq, XWork& work);
 // device bytes per atom of the X build's per-atom work buffers
 size_t atom_work_bytes(const ModelHandle& m, bool fp64);
+// polynomial columns of structures [s0, s1) (after the linear columns of the
+// same sub-batch have been built, which leaves the W', x and pair buffers)
+void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
+                     const double* roww, double* xe_sum, double* xe_sq, XWork& work,
+                     long pair_base, const PairRec& R, int w_stride, int x_stride);
+size_t poly_bytes(ModelHandle& m, const BatchHandle& b, int s0, int s1, bool fp64);
 size_t pair_record_bytes(const ModelHandle& m, bool fp64);
 
 struct NormalHandle {

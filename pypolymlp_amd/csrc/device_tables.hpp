@@ -80,6 +80,19 @@ struct ColumnsDev {
     const int* blk_nkey;              // [n_blocks] Y keys the block's nonzeros use
 };
 
+// Polynomial columns built by the factor-derivative GEMM (kernels_poly.hip).
+struct PolyDev {
+    int K;                   // factors (distinct linear features of the products)
+    int U;                   // monomials: K of degree 1, then K(K+1)/2 of degree 2
+    int n_cols;              // polynomial columns
+    int c0;                  // first polynomial column (= number of linear features)
+    const int* fac_local;    // [n_type][K] local linear feature of each factor, -1 if none
+    const int* col_phi;      // [n_cols][3] factor of each slot (-1 unused)
+    const int* col_u;        // [n_cols][3] monomial of the other factors of each slot
+    const int* mono_a;       // [U] first factor of each monomial
+    const int* mono_b;       // [U] second factor (-1 for degree 1)
+};
+
 // A batch of structures resident in device memory.
 struct BatchDev {
     int n_st;

@@ -102,6 +102,19 @@ void launch_gram_quantize(hipStream_t st, const float* X, int ld, int rows_pad, 
 void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int F_pad, const void* quant, double* C,
                     int ldc);
 
+// Polynomial columns (kernels_poly.hip)
+void launch_poly_factor(hipStream_t st, bool fp64, const ModelDev& md, const PolyDev& pd,
+                        const BatchDev& bd, int atom0, int n_atoms, const void* xall, int x_stride,
+                        void* XF, void* M);
+void launch_poly_deriv(hipStream_t st, bool fp64, const ModelDev& md, const PolyDev& pd,
+                       const BatchDev& bd, int atom0, int n_atoms, long pair_base,
+                       const void* prec, const PairRec& R, const void* wall, int w_stride, int k0,
+                       int kc, int rd_max, void* D, int st0, const long* d_off);
+void launch_poly_gather(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,
+                        int s_first, int n_st, int atom_base, const void* Z, long z_stride,
+                        const void* XF, void* X, int ld, int row_base, const double* roww,
+                        double* xe_sum, double* xe_sq, int rd_max);
+
 // Surrogate network of the hyperparameter search (FP32 MLP 5-64-64-2, SiLU).
 constexpr int kSurrIn = 5, kSurrHidden = 64, kSurrOut = 2;
 constexpr int kSurrParams = kSurrHidden * kSurrIn + kSurrHidden + kSurrHidden * kSurrHidden +

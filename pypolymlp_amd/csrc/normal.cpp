@@ -19,6 +19,7 @@ namespace plm {
 static constexpr size_t kXBudget = 3ull << 30;   // bytes of X per sub-batch
 static constexpr size_t kABudget = 8ull << 30;   // bytes of per-atom work (a, W', x) per sub-batch
 static constexpr size_t kPBudget = 6ull << 30;   // bytes of pair records per sub-batch
+static constexpr size_t kDBudget = 4ull << 30;   // bytes of factor derivatives per sub-batch
 
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
                        bool fp64) {
@@ -41,7 +42,8 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
             const size_t atoms = (size_t)(b.atom_off[s_end] - b.atom_off[s0]);
             const size_t pairs = (size_t)(b.nb_off[b.atom_off[s_end]] - b.nb_off[b.atom_off[s0]]);
             return (rows + kstep) * row_bytes <= kXBudget &&
-                   atoms * atom_bytes <= kABudget && pairs * rec_bytes <= kPBudget;
+                   atoms * atom_bytes <= kABudget && pairs * rec_bytes <= kPBudget &&
+                   poly_bytes(m, b, s0, s_end, fp64) <= kDBudget;
         };
         while (s1 < b.n_st && fits(s1 + 1)) ++s1;
         const long r0 = b.row_off[s0], r1 = b.row_off[s1];

@@ -1,0 +1,186 @@
+// Polynomial columns of X by the factor-derivative GEMM: host tables and the
+// per-sub-batch sequence (factor values and monomials, factor derivatives,
+// one rocBLAS strided-batched GEMM per group of equal-size structures,
+// gather into the X rows). The column order and the products follow
+// Model::model_polynomial / model_order{2,3} (src/cxx/src/compute/model.cpp:159-267).
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <map>
+#include <stdexcept>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "pair_data.hpp"
+#include "profile.hpp"
+
+namespace plm {
+
+static rocblas_handle blas_handle() {
+    static rocblas_handle h = nullptr;
+    if (!h) {
+        if (rocblas_create_handle(&h) != rocblas_status_success)
+            throw DeviceError("rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(h, device().stream) != rocblas_status_success)
+        throw DeviceError("rocblas_set_stream failed");
+    return h;
+}
+
+const PolyTables& ModelHandle::poly() {
+    if (poly_tables) return *poly_tables;
+    auto pt = std::make_unique<PolyTables>();
+    const Model& M = host;
+    const int nlin = M.n_linear(), F = M.n_features(), nt = M.n_type();
+    std::vector<std::vector<int>> cols;
+    for (const auto& c : M.comb2()) cols.push_back({c[0], c[1]});
+    for (const auto& c : M.comb3()) cols.push_back({c[0], c[1], c[2]});
+    if ((int)cols.size() != F - nlin) throw std::logic_error("polynomial column count");
+    std::map<int, int> phi;
+    for (const auto& c : cols)
+        for (int f : c) phi.emplace(f, 0);
+    int K = 0;
+    for (auto& kv : phi) kv.second = K++;
+    bool deg2 = false;
+    for (const auto& c : cols) deg2 = deg2 || c.size() == 3;
+    std::vector<int> ma, mb;
+    for (int a = 0; a < K; ++a) ma.push_back(a), mb.push_back(-1);
+    std::map<std::pair<int, int>, int> pair_u;
+    if (deg2)
+        for (int a = 0; a < K; ++a)
+            for (int b = a; b < K; ++b) {
+                pair_u[{a, b}] = (int)ma.size();
+                ma.push_back(a), mb.push_back(b);
+            }
+    std::vector<int> cphi(cols.size() * 3, -1), cu(cols.size() * 3, -1);
+    for (size_t c = 0; c < cols.size(); ++c) {
+        std::vector<int> q;
+        for (int f : cols[c]) q.push_back(phi.at(f));
+        for (size_t s = 0; s < q.size(); ++s) {
+            cphi[3 * c + s] = q[s];
+            std::vector<int> rest;
+            for (size_t k = 0; k < q.size(); ++k)
+                if (k != s) rest.push_back(q[k]);
+            if (rest.size() == 1) cu[3 * c + s] = rest[0];
+            else cu[3 * c + s] = pair_u.at({std::min(rest[0], rest[1]), std::max(rest[0], rest[1])});
+        }
+    }
+    std::vector<int> floc((size_t)nt * std::max(K, 1), -1);
+    for (int t = 0; t < nt; ++t) {
+        const auto& lg = M.type(t).linear_global;
+        for (size_t k = 0; k < lg.size(); ++k) {
+            auto it = phi.find(lg[k]);
+            if (it != phi.end()) floc[(size_t)t * K + it->second] = (int)k;
+        }
+    }
+    PolyDev& d = pt->dev;
+    d.K = K;
+    d.U = (int)ma.size();
+    d.n_cols = (int)cols.size();
+    d.c0 = nlin;
+    d.fac_local = pt->store.upload(floc.data(), floc.size() * 4);
+    d.col_phi = pt->store.upload(cphi.data(), cphi.size() * 4);
+    d.col_u = pt->store.upload(cu.data(), cu.size() * 4);
+    d.mono_a = pt->store.upload(ma.data(), ma.size() * 4);
+    d.mono_b = pt->store.upload(mb.data(), mb.size() * 4);
+    pt->enabled = d.n_cols > 0 && K > 0;
+    poly_tables = std::move(pt);
+    return *poly_tables;
+}
+
+size_t poly_bytes(ModelHandle& m, const BatchHandle& b, int s0, int s1, bool fp64) {
+    const PolyTables& P = m.poly();
+    if (!P.enabled) return 0;
+    size_t n = 0;
+    for (int s = s0; s < s1; ++s) {
+        const size_t N = b.atom_off[s + 1] - b.atom_off[s];
+        n += N * P.dev.K * (6 + 3 * N) + N * (P.dev.K + P.dev.U);
+    }
+    return n * (fp64 ? 8 : 4);
+}
+
+void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
+                     const double* roww, double* xe_sum, double* xe_sq, XWork& work,
+                     long pair_base, const PairRec& R, int w_stride, int x_stride) {
+    const PolyTables& P = m.poly();
+    const PolyDev& pd = P.dev;
+    Device& dev = device();
+    hipStream_t st = dev.stream;
+    const int elem = fp64 ? 8 : 4;
+    const int a0 = b.atom_off[s0], na = b.atom_off[s1] - a0;
+    const BatchDev bd = b.view();
+    work.xf.alloc((size_t)na * pd.K * elem);
+    work.m.alloc((size_t)na * pd.U * elem);
+    // D offsets per structure of the sub-batch
+    std::vector<long> doff(s1 - s0 + 1, 0);
+    int nmax = 0;
+    for (int s = s0; s < s1; ++s) {
+        const long N = b.atom_off[s + 1] - b.atom_off[s];
+        nmax = std::max<int>(nmax, (int)N);
+        doff[s - s0 + 1] = doff[s - s0] + N * pd.K * (6 + 3 * N);
+    }
+    const int rd_max = 6 + 3 * nmax;
+    work.d.alloc((size_t)doff.back() * elem);
+    work.doff.alloc(doff.size() * sizeof(long));
+    check_hip(hipMemcpyAsync(work.doff.ptr, doff.data(), doff.size() * sizeof(long),
+                             hipMemcpyHostToDevice, st), "copy");
+    {
+        ProfScope ps("poly_deriv", st);
+        launch_poly_factor(st, fp64, m.md, pd, bd, a0, na, work.x.ptr, x_stride, work.xf.ptr,
+                           work.m.ptr);
+        // factor chunks whose per-atom rows fit 64 KB of LDS
+        const int kc = std::max(1, std::min(pd.K, (int)((64 * 1024) / ((size_t)rd_max * elem))));
+        if ((size_t)kc * rd_max * elem > 160 * 1024)
+            throw std::runtime_error("structure too large for the factor-derivative rows");
+        for (int k0 = 0; k0 < pd.K; k0 += kc)
+            launch_poly_deriv(st, fp64, m.md, pd, bd, a0, na, pair_base, work.pd.ptr, R,
+                              work.w.ptr, w_stride, k0, std::min(kc, pd.K - k0), rd_max,
+                              work.d.ptr, s0, (const long*)work.doff.ptr);
+    }
+    check_hip(hipGetLastError(), "poly launch");
+    rocblas_handle h = blas_handle();
+    const size_t z_budget = 2ull << 30;
+    int s = s0;
+    while (s < s1) {
+        const int N = b.atom_off[s + 1] - b.atom_off[s];
+        const long RD = 6 + 3 * N;
+        const long KR = (long)pd.K * RD;
+        const size_t z_one = (size_t)pd.U * KR * elem;
+        int g = 1;
+        while (s + g < s1 && b.atom_off[s + g + 1] - b.atom_off[s + g] == N &&
+               (size_t)(g + 1) * z_one <= z_budget)
+            ++g;
+        work.z.alloc(z_one * g);
+        const size_t aoff = (size_t)(b.atom_off[s] - a0) * pd.U;
+        const size_t boff = (size_t)doff[s - s0];
+        {
+            ProfScope ps("poly_gemm", st);
+            rocblas_status rs;
+            if (fp64) {
+                const double one = 1.0, zero = 0.0;
+                rs = rocblas_dgemm_strided_batched(
+                    h, rocblas_operation_none, rocblas_operation_transpose, pd.U, (int)KR, N, &one,
+                    reinterpret_cast<const double*>(work.m.ptr) + aoff, pd.U, (long)N * pd.U,
+                    reinterpret_cast<const double*>(work.d.ptr) + boff, (int)KR, (long)N * KR,
+                    &zero, reinterpret_cast<double*>(work.z.ptr), pd.U, (long)pd.U * KR, g);
+            } else {
+                const float one = 1.f, zero = 0.f;
+                rs = rocblas_sgemm_strided_batched(
+                    h, rocblas_operation_none, rocblas_operation_transpose, pd.U, (int)KR, N, &one,
+                    reinterpret_cast<const float*>(work.m.ptr) + aoff, pd.U, (long)N * pd.U,
+                    reinterpret_cast<const float*>(work.d.ptr) + boff, (int)KR, (long)N * KR,
+                    &zero, reinterpret_cast<float*>(work.z.ptr), pd.U, (long)pd.U * KR, g);
+            }
+            if (rs != rocblas_status_success) throw DeviceError("rocBLAS GEMM of the polynomial columns failed");
+        }
+        {
+            ProfScope ps("poly_gather", st);
+            launch_poly_gather(st, fp64, pd, bd, s, g, a0, work.z.ptr, (long)pd.U * KR,
+                               work.xf.ptr, X, ld, b.row_off[s0], roww, xe_sum, xe_sq, (int)RD);
+        }
+        check_hip(hipGetLastError(), "poly gather launch");
+        s += g;
+    }
+}
+
+}  // namespace plm
