@@ -540,7 +540,8 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover) {
         const int ppw = 64 / CB;
         const int slot = ppw * R.prefix(ct.nkey_max);
         if (slot > xrows_stage_limit()) continue;
-        for (int NW : {8, 4}) {
+        for (int NW : {16, 8, 4}) {
+            if (NW > xrows_max_waves(fp64)) continue;
             XRowsParams P{};
             P.CB = CB, P.NW = NW, P.need_max = ct.need_max, P.wz_max = ct.wz_max;
             P.rows_max = rows_max, P.ntp = m.md.n_tp, P.slot = slot;

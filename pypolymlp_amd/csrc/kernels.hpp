@@ -75,6 +75,7 @@ void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev
                    void* x, int x_stride, int max_nlm);
 size_t xrows_smem(const XRowsParams& P, int elem);
 int xrows_stage_limit();
+int xrows_max_waves(bool fp64);  // waves per workgroup the kernel is built for
 void launch_xrows(hipStream_t st, bool fp64, const ModelDev& md, const ColumnsDev& cd,
                   const BatchDev& bd, const BlockWz& bw, const XRowsParams& P, const void* w,
                   int w_stride, const void* x, int x_stride, int atom_base, const void* pd,

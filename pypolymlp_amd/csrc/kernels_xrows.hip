@@ -176,39 +176,44 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // d x_f / d d for one pair record: W' nonzeros z in [za, zb) of feature f
-// restricted to the pair's type pair. f_n < 1e-20 is skipped (local.cpp:164).
+// restricted to the pair's type pair. Every nonzero of one linear feature has
+// the feature's radial index, so f_n and f_n' are read once; f_n < 1e-20
+// drops the pair (local.cpp:164).
 template <class T>
 __device__ __forceinline__ void q_feature(const T* rec, const PairRec& R,
                                           const Cpx<T>* __restrict__ Wv,
                                           const int* __restrict__ nk, int za, int zb, T& q0,
                                           T& q1, T& q2) {
-    const T* fn = rec + R.fn();
-    const T* fd = rec + R.fd();
+    q0 = q1 = q2 = T(0);
+    if (za >= zb) return;
+    const int nid = nk[za] & 255;
+    const T fv = rec[R.fn() + nid];
+    if (fv < T(1e-20)) return;
+    const T fdv = rec[R.fd() + nid];
     const T ir = T(1) / rec[3];
     const T ux = rec[0] * ir, uy = rec[1] * ir, uz = rec[2] * ir;
-    q0 = q1 = q2 = T(0);
+    T a0 = T(0), a1 = T(0), a2 = T(0), ry = T(0);
     for (int z = za; z < zb; ++z) {
-        const int pk = nk[z];
-        const int nid = pk & 255, key = (pk >> 8) - 1;
-        const T fv = fn[nid];
-        if (fv < T(1e-20)) continue;
-        const T fdv = fd[nid];
+        const int key = (nk[z] >> 8) - 1;
         const Cpx<T> W = Wv[z];
         if (key < 0) {
-            const T r0 = fdv * W.re;
-            q0 += r0 * ux, q1 += r0 * uy, q2 += r0 * uz;
+            ry += W.re;
         } else {
             const T* y = rec + R.ylm(key);
-            const T ry = (W.re * y[0] - W.im * y[1]) * fdv;
-            q0 += ry * ux + fv * (W.re * y[2] - W.im * y[3]);
-            q1 += ry * uy + fv * (W.re * y[4] - W.im * y[5]);
-            q2 += ry * uz + fv * (W.re * y[6] - W.im * y[7]);
+            ry += W.re * y[0] - W.im * y[1];
+            a0 += W.re * y[2] - W.im * y[3];
+            a1 += W.re * y[4] - W.im * y[5];
+            a2 += W.re * y[6] - W.im * y[7];
         }
     }
+    ry *= fdv;
+    q0 = ry * ux + fv * a0;
+    q1 = ry * uy + fv * a1;
+    q2 = ry * uz + fv * a2;
 }
 
 template <class T>
-__global__ void __launch_bounds__(512) xrows_kernel(ModelDev md, ColumnsDev cd, BatchDev bd,
+__global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) xrows_kernel(ModelDev md, ColumnsDev cd, BatchDev bd,
                                                     BlockWz bw, XRowsParams P,
                                                     const Cpx<T>* __restrict__ wall, int w_stride,
                                                     const T* __restrict__ xall, int x_stride,
@@ -438,6 +443,8 @@ void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchD
 size_t xrows_smem(const XRowsParams& P, int elem) { return XRowsLds(P, elem).total; }
 
 int xrows_stage_limit() { return 64 * kStageRegs; }
+
+int xrows_max_waves(bool fp64) { return fp64 ? 8 : 16; }
 
 void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,
                    const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
