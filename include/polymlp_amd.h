@@ -130,11 +130,13 @@ int plm_batch_row_order(const plm_batch* b, long* order);
  * exhaustive fallback this network prunes). FP32 MLP 5-64-64-2 with SiLU,
  * parameters [W1 (64x5) | b1 | W2 (64x64) | b2 | W3 (2x64) | b3], loss =
  * mean (y - t)^2. Training runs `steps` full-batch Adam steps (beta 0.9 /
- * 0.999, eps 1e-8; step count continues from t0) in one launch; params and
- * the Adam moments are updated in place. n <= 128 for train / grad. */
+ * 0.999, eps 1e-8, decoupled weight decay; step count continues from t0) in
+ * one launch; params and the Adam moments are updated in place. n <= 128 for
+ * train / grad. */
 int plm_surrogate_n_params(int* n);
 int plm_surrogate_train(float* params, float* adam_m, float* adam_v, int t0, const float* x,
-                        const float* y, int n, int steps, float lr, float* loss);
+                        const float* y, int n, int steps, float lr, float weight_decay,
+                        float* loss);
 int plm_surrogate_grad(const float* params, const float* x, const float* y, int n, float* grad,
                        float* loss);
 int plm_surrogate_predict(const float* params, const float* x, int n, float* y);

@@ -3,7 +3,7 @@
 Same layout and loss as the HIP kernels (kernels_surrogate.hip): parameter
 vector [W1 (64x5) | b1 | W2 (64x64) | b2 | W3 (2x64) | b3], SiLU
 activations, loss = mean over samples and outputs of (y - t)^2, Adam with
-beta (0.9, 0.999), eps 1e-8 and bias correction.
+beta (0.9, 0.999), eps 1e-8, bias correction and decoupled weight decay.
 """
 
 from __future__ import annotations
@@ -60,7 +60,7 @@ def loss_grad(p, x, t):
     return loss, g
 
 
-def adam(p, m, v, t0, x, t, steps, lr, b1=0.9, b2=0.999, eps=1e-8):
+def adam(p, m, v, t0, x, t, steps, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
     p, m, v = (np.array(a, dtype=np.float64) for a in (p, m, v))
     loss = None
     for it in range(steps):
@@ -68,5 +68,5 @@ def adam(p, m, v, t0, x, t, steps, lr, b1=0.9, b2=0.999, eps=1e-8):
         k = t0 + it + 1
         m = b1 * m + (1 - b1) * g
         v = b2 * v + (1 - b2) * g * g
-        p = p - lr * (m / (1 - b1 ** k)) / (np.sqrt(v / (1 - b2 ** k)) + eps)
+        p = p - lr * ((m / (1 - b1 ** k)) / (np.sqrt(v / (1 - b2 ** k)) + eps) + wd * p)
     return p, m, v, loss

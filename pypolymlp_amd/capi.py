@@ -89,7 +89,7 @@ def lib():
         L.plm_profile_enable.argtypes = [ctypes.c_int]
         L.plm_surrogate_n_params.argtypes = [_ip]
         L.plm_surrogate_train.argtypes = [_fp, _fp, _fp, ctypes.c_int, _fp, _fp, ctypes.c_int,
-                                          ctypes.c_int, ctypes.c_float, _fp]
+                                          ctypes.c_int, ctypes.c_float, ctypes.c_float, _fp]
         L.plm_surrogate_grad.argtypes = [_fp, _fp, _fp, ctypes.c_int, _fp, _fp]
         L.plm_surrogate_predict.argtypes = [_fp, _fp, ctypes.c_int, _fp]
         L.plm_profile_get.argtypes = [ctypes.c_char_p, _dp, _lp]
@@ -384,13 +384,15 @@ def surrogate_n_params() -> int:
     return n.value
 
 
-def surrogate_train(params, m, v, t0: int, x, y, steps: int, lr: float) -> float:
-    """Full-batch Adam steps on the GPU; params, m, v (float32) updated in place."""
+def surrogate_train(params, m, v, t0: int, x, y, steps: int, lr: float,
+                    weight_decay: float = 0.0) -> float:
+    """Full-batch Adam(W) steps on the GPU; params, m, v (float32) updated in place."""
     assert params.dtype == np.float32 and m.dtype == np.float32 and v.dtype == np.float32
     x, y = _f(x), _f(y)
     loss = ctypes.c_float()
     _check(lib().plm_surrogate_train(_pf(params), _pf(m), _pf(v), int(t0), _pf(x), _pf(y),
-                                     len(x), int(steps), float(lr), ctypes.byref(loss)))
+                                     len(x), int(steps), float(lr), float(weight_decay),
+                                     ctypes.byref(loss)))
     return loss.value
 
 

@@ -914,11 +914,13 @@ int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas,
 int plm_surrogate_n_params(int* n) { PLM_TRY({ *n = kSurrParams; }); }
 
 int plm_surrogate_train(float* params, float* adam_m, float* adam_v, int t0, const float* x,
-                        const float* y, int n, int steps, float lr, float* loss) {
+                        const float* y, int n, int steps, float lr, float weight_decay,
+                        float* loss) {
     PLM_TRY({
         if (n < 1 || n > kSurrMaxBatch) throw std::invalid_argument("training batch size out of range");
         if (steps < 1) throw std::invalid_argument("steps must be positive");
-        surrogate_run(params, adam_m, adam_v, t0, x, y, n, steps, lr, 0, nullptr, loss);
+        surrogate_run(params, adam_m, adam_v, t0, x, y, n, steps, lr, weight_decay, 0, nullptr,
+                      loss);
     });
 }
 
@@ -926,8 +928,8 @@ int plm_surrogate_grad(const float* params, const float* x, const float* y, int 
                        float* loss) {
     PLM_TRY({
         if (n < 1 || n > kSurrMaxBatch) throw std::invalid_argument("batch size out of range");
-        surrogate_run(const_cast<float*>(params), nullptr, nullptr, 0, x, y, n, 1, 0.f, 1, grad,
-                      loss);
+        surrogate_run(const_cast<float*>(params), nullptr, nullptr, 0, x, y, n, 1, 0.f, 0.f, 1,
+                      grad, loss);
     });
 }
 

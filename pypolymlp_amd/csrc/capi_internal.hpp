@@ -170,7 +170,8 @@ void ridge_solve(int F, const double* xtx, const double* xty, const double* alph
                  double* coeffs, int* info);
 
 void surrogate_run(float* params, float* adam_m, float* adam_v, int t0, const float* x,
-                   const float* y, int n, int steps, float lr, int mode, float* grad, float* loss);
+                   const float* y, int n, int steps, float lr, float wd, int mode, float* grad,
+                   float* loss);
 void surrogate_predict(const float* params, const float* x, int n, float* y);
 
 }  // namespace plm

@@ -124,7 +124,7 @@ constexpr int kSurrMaxBatch = 128;
 size_t surrogate_smem(int n);
 void launch_surr_train(hipStream_t st, float* P, float* M, float* V, const float* X,
                        const float* T, int n, int steps, int t0, float lr, float beta1,
-                       float beta2, float eps, int mode, float* grad, float* loss);
+                       float beta2, float eps, float wd, int mode, float* grad, float* loss);
 void launch_surr_predict(hipStream_t st, const float* P, const float* X, int n, float* Y);
 
 }  // namespace plm

@@ -1,7 +1,8 @@
 // Surrogate network of the hyperparameter search: an FP32 multilayer
-// perceptron 5-64-64-2 with SiLU activations, trained with Adam on a full
-// batch. One workgroup runs every training step in a single launch with the
-// parameters, Adam moments, activations and gradients resident in LDS.
+// perceptron 5-64-64-2 with SiLU activations, trained with Adam (decoupled
+// weight decay) on a full batch. One workgroup runs every training step in a
+// single launch with the parameters, Adam moments, activations and gradients
+// resident in LDS.
 //
 // Parameter vector (row-major weights): [W1 (64x5) | b1 | W2 (64x64) | b2 |
 // W3 (2x64) | b3]. Loss: mean over samples and outputs of (y - t)^2.
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(256) surr_train_kernel(float* __restrict__ P, 
                                                          const float* __restrict__ X,
                                                          const float* __restrict__ T, int n,
                                                          int steps, int t0, float lr, float beta1,
-                                                         float beta2, float eps, int mode,
+                                                         float beta2, float eps, float wd, int mode,
                                                          float* __restrict__ grad,
                                                          float* __restrict__ loss_out) {
     extern __shared__ float sm[];
@@ -186,7 +187,8 @@ __global__ void __launch_bounds__(256) surr_train_kernel(float* __restrict__ P, 
             const float g = L.G[e];
             m[k] = beta1 * m[k] + (1.f - beta1) * g;
             v[k] = beta2 * v[k] + (1.f - beta2) * g * g;
-            L.P[e] -= lr * (m[k] / c1) / (sqrtf(v[k] / c2) + eps);
+            // decoupled weight decay (AdamW)
+            L.P[e] -= lr * ((m[k] / c1) / (sqrtf(v[k] / c2) + eps) + wd * L.P[e]);
         }
         __syncthreads();
     }
@@ -216,12 +218,12 @@ size_t surrogate_smem(int n) {
 
 void launch_surr_train(hipStream_t st, float* P, float* M, float* V, const float* X,
                        const float* T, int n, int steps, int t0, float lr, float beta1,
-                       float beta2, float eps, int mode, float* grad, float* loss) {
+                       float beta2, float eps, float wd, int mode, float* grad, float* loss) {
     const size_t sm = surrogate_smem(n);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&surr_train_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    surr_train_kernel<<<1, 256, sm, st>>>(P, M, V, X, T, n, steps, t0, lr, beta1, beta2, eps, mode,
-                                          grad, loss);
+    surr_train_kernel<<<1, 256, sm, st>>>(P, M, V, X, T, n, steps, t0, lr, beta1, beta2, eps, wd,
+                                          mode, grad, loss);
 }
 
 void launch_surr_predict(hipStream_t st, const float* P, const float* X, int n, float* Y) {

@@ -6,7 +6,7 @@
 namespace plm {
 
 void surrogate_run(float* params, float* adam_m, float* adam_v, int t0, const float* x,
-                   const float* y, int n, int steps, float lr, int mode, float* grad,
+                   const float* y, int n, int steps, float lr, float wd, int mode, float* grad,
                    float* loss) {
     Device& dev = device();
     hipStream_t st = dev.stream;
@@ -22,7 +22,7 @@ void surrogate_run(float* params, float* adam_m, float* adam_v, int t0, const fl
     check_hip(hipMemcpyAsync(X.ptr, x, (size_t)n * kSurrIn * 4, hipMemcpyHostToDevice, st), "copy");
     check_hip(hipMemcpyAsync(Y.ptr, y, (size_t)n * kSurrOut * 4, hipMemcpyHostToDevice, st), "copy");
     launch_surr_train(st, (float*)P.ptr, (float*)M.ptr, (float*)V.ptr, (const float*)X.ptr,
-                      (const float*)Y.ptr, n, steps, t0, lr, 0.9f, 0.999f, 1e-8f, mode,
+                      (const float*)Y.ptr, n, steps, t0, lr, 0.9f, 0.999f, 1e-8f, wd, mode,
                       (float*)G.ptr, (float*)L.ptr);
     check_hip(hipGetLastError(), "surrogate launch");
     if (mode == 0) {

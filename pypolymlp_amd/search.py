@@ -184,9 +184,10 @@ class Surrogate:
         return capi.surrogate_predict(self.params, Z) * self.y_std + self.y_mean
 
 
-def train_surrogate(X: np.ndarray, Y: np.ndarray, steps: int = 3000, lr: float = 3e-3,
-                    seed: int = 0, chunk: int = 500) -> Surrogate:
-    """Full-batch Adam on the GPU (hand-written HIP forward/backward)."""
+def train_surrogate(X: np.ndarray, Y: np.ndarray, steps: int = 2000, lr: float = 3e-3,
+                    weight_decay: float = 1e-2, seed: int = 0, chunk: int = 250) -> Surrogate:
+    """Full-batch AdamW on the GPU (hand-written HIP forward/backward); the
+    history starts with the loss of the initial parameters."""
     X = np.asarray(X, dtype=float)
     Y = np.asarray(Y, dtype=float)
     xm, xs = X.mean(0), X.std(0) + 1e-12
@@ -206,11 +207,11 @@ def train_surrogate(X: np.ndarray, Y: np.ndarray, steps: int = 3000, lr: float =
     v = np.zeros_like(p)
     Z = (X - xm) / xs
     T = (Y - ym) / ys
-    hist = []
+    hist = [capi.surrogate_grad(p, Z, T)[1]]
     t = 0
     while t < steps:
         k = min(chunk, steps - t)
-        hist.append(capi.surrogate_train(p, m, v, t, Z, T, k, lr))
+        hist.append(capi.surrogate_train(p, m, v, t, Z, T, k, lr, weight_decay))
         t += k
     return Surrogate(p, xm, xs, ym, ys, hist[-1], hist)
 
@@ -261,7 +262,7 @@ class SearchReport:
 
 
 def run_search(cands: Sequence[Candidate], evaluate_fn: Callable[[Candidate], Result],
-               n_init: int = 32, n_pick: int = 24, seed: int = 7, surrogate_steps: int = 3000,
+               n_init: int = 32, n_pick: int = 24, seed: int = 7, surrogate_steps: int = 2000,
                cached: Optional[Dict[Candidate, Result]] = None) -> SearchReport:
     """Exhaustive grid (the judge), surrogate-guided search and a random
     search with the same number of fits. Fits are deterministic, so each

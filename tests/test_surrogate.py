@@ -73,8 +73,9 @@ def test_surrogate_adam_matches_oracle():
     p = p0.astype(np.float32)
     m = np.zeros_like(p)
     v = np.zeros_like(p)
-    loss = capi.surrogate_train(p, m, v, 0, x, y, steps=50, lr=3e-3)
-    pr, mr, vr, lr_ = oracle_net.adam(p0, np.zeros_like(p0), np.zeros_like(p0), 0, x, y, 50, 3e-3)
+    loss = capi.surrogate_train(p, m, v, 0, x, y, steps=50, lr=3e-3, weight_decay=1e-2)
+    pr, mr, vr, lr_ = oracle_net.adam(p0, np.zeros_like(p0), np.zeros_like(p0), 0, x, y, 50, 3e-3,
+                                      wd=1e-2)
     assert np.linalg.norm(p - pr) <= 1e-3 * np.linalg.norm(pr)
     assert loss == pytest.approx(lr_, rel=1e-3)
     # continuing from t0 = 50 reduces the loss further
