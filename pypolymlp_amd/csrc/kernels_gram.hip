@@ -226,14 +226,20 @@ __global__ void __launch_bounds__(256) gram_i8_kernel(const signed char* __restr
                         B + ((s * GI_RB + rbo) * GI_TJ + 32 * n + (lane & 31)) * GI_CS +
                         16 * (lane >> 5));
             }
+            // issue order: the k-th product of every (block, level) accumulator
+            // before any (k+1)-th, so dependent MFMAs are 6-10 apart
+            constexpr int kS[3][5] = {{0, 0, 0, 1, 2}, {-1, 1, 1, 2, -1}, {-1, -1, 2, -1, -1}};
 #pragma unroll
-            for (int n = 0; n < 2; ++n)
+            for (int k = 0; k < 3; ++k)
 #pragma unroll
-                for (int s = 0; s < 3; ++s)
+                for (int lvl = 0; lvl < 5; ++lvl)
 #pragma unroll
-                    for (int t = 0; t < 3; ++t)
-                        acc[n][s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b[n][t],
-                                                                              acc[n][s + t], 0, 0, 0);
+                    for (int n = 0; n < 2; ++n) {
+                        const int s = kS[k][lvl];
+                        if (s < 0) continue;
+                        acc[n][lvl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b[n][lvl - s],
+                                                                            acc[n][lvl], 0, 0, 0);
+                    }
         }
     };
     // x~_i x~_j = 2^(E_i + E_j - 14) sum_k acc_k 2^(-7k): exact int32 sums
