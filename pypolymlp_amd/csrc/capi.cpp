@@ -539,7 +539,7 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover) {
         const PairRec R(m.md.n_fn, m.md.n_lmh);
         const int ppw = 64 / CB;
         const int slot = ppw * R.prefix(ct.nkey_max);
-        if (slot > xrows_stage_limit()) continue;
+        if (slot > xrows_stage_limit(fp64)) continue;
         for (int NW : {16, 8, 4}) {
             if (NW > xrows_max_waves(fp64)) continue;
             XRowsParams P{};

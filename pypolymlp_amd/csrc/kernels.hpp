@@ -74,7 +74,7 @@ void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev
                    const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
                    void* x, int x_stride, int max_nlm);
 size_t xrows_smem(const XRowsParams& P, int elem);
-int xrows_stage_limit();
+int xrows_stage_limit(bool fp64);
 int xrows_max_waves(bool fp64);  // waves per workgroup the kernel is built for
 void launch_xrows(hipStream_t st, bool fp64, const ModelDev& md, const ColumnsDev& cd,
                   const BatchDev& bd, const BlockWz& bw, const XRowsParams& P, const void* w,

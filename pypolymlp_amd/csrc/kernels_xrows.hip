@@ -31,7 +31,9 @@
 namespace plm {
 
 constexpr int kPdPairs = 16;   // pairs per LDS chunk in pairdata_kernel
-constexpr int kStageRegs = 24; // staged record elements per lane (64 lanes)
+// staged record elements per lane (64 lanes): FP32 groups are one record
+// (<= 768 values), FP64 groups two
+template <class T> struct StageRegs { static constexpr int n = sizeof(T) == 8 ? 24 : 12; };
 
 // ---------------------------------------------------------------------------
 // Pair records (X path direction d = r_j - r_i + T, local.cpp:149).
@@ -175,6 +177,26 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// one nonzero of W': key (packed), weight W. The four increments are formed
+// first and added once, so the accumulators stay in registers.
+template <class T>
+__device__ __forceinline__ void q_term(const T* rec, const PairRec& R, int pk, Cpx<T> W, T& ry,
+                                       T& a0, T& a1, T& a2) {
+    const int key = (pk >> 8) - 1;
+    T d0 = W.re, d1 = T(0), d2 = T(0), d3 = T(0);
+    if (key >= 0) {
+        const T* y = rec + R.ylm(key);
+        d0 = W.re * y[0] - W.im * y[1];
+        d1 = W.re * y[2] - W.im * y[3];
+        d2 = W.re * y[4] - W.im * y[5];
+        d3 = W.re * y[6] - W.im * y[7];
+    }
+    ry += d0;
+    a0 += d1;
+    a1 += d2;
+    a2 += d3;
+}
+
 // d x_f / d d for one pair record: W' nonzeros z in [za, zb) of feature f
 // restricted to the pair's type pair. Every nonzero of one linear feature has
 // the feature's radial index, so f_n and f_n' are read once; f_n < 1e-20
@@ -193,19 +215,17 @@ __device__ __forceinline__ void q_feature(const T* rec, const PairRec& R,
     const T ir = T(1) / rec[3];
     const T ux = rec[0] * ir, uy = rec[1] * ir, uz = rec[2] * ir;
     T a0 = T(0), a1 = T(0), a2 = T(0), ry = T(0);
-    for (int z = za; z < zb; ++z) {
-        const int key = (nk[z] >> 8) - 1;
-        const Cpx<T> W = Wv[z];
-        if (key < 0) {
-            ry += W.re;
-        } else {
-            const T* y = rec + R.ylm(key);
-            ry += W.re * y[0] - W.im * y[1];
-            a0 += W.re * y[2] - W.im * y[3];
-            a1 += W.re * y[4] - W.im * y[5];
-            a2 += W.re * y[6] - W.im * y[7];
-        }
+    // four nonzeros per step: their keys and weights are loaded together
+    int z = za;
+    for (; z + 4 <= zb; z += 4) {
+        const int k0 = nk[z], k1 = nk[z + 1], k2 = nk[z + 2], k3 = nk[z + 3];
+        const Cpx<T> w0 = Wv[z], w1 = Wv[z + 1], w2 = Wv[z + 2], w3 = Wv[z + 3];
+        q_term<T>(rec, R, k0, w0, ry, a0, a1, a2);
+        q_term<T>(rec, R, k1, w1, ry, a0, a1, a2);
+        q_term<T>(rec, R, k2, w2, ry, a0, a1, a2);
+        q_term<T>(rec, R, k3, w3, ry, a0, a1, a2);
     }
+    for (; z < zb; ++z) q_term<T>(rec, R, nk[z], Wv[z], ry, a0, a1, a2);
     ry *= fdv;
     q0 = ry * ux + fv * a0;
     q1 = ry * uy + fv * a1;
@@ -317,6 +337,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) xrows_kernel(Mode
 
         // records of a group of PPW consecutive pairs are staged through
         // registers: the next group is loaded while the current one is used
+        constexpr int kStageRegs = StageRegs<T>::n;
         T stage[kStageRegs];
         auto load_group = [&](int pc) {
             const int nel = min(PPW, p1 - pc) * pre;
@@ -442,7 +463,9 @@ void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchD
 
 size_t xrows_smem(const XRowsParams& P, int elem) { return XRowsLds(P, elem).total; }
 
-int xrows_stage_limit() { return 64 * kStageRegs; }
+int xrows_stage_limit(bool fp64) {
+    return 64 * (fp64 ? StageRegs<double>::n : StageRegs<float>::n);
+}
 
 int xrows_max_waves(bool fp64) { return fp64 ? 8 : 16; }
 
