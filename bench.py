@@ -175,9 +175,9 @@ def main():
             acc.accumulate(batch, w, y, prec)
         t1 = time.perf_counter()
         capi.profile_enable(False)
-        kern = {k: capi.profile_get(k) for k in ("anlm", "wprime", "pairdata", "xrows",
-                                                 "poly_deriv", "poly_gemm", "poly_gather",
-                                                 "quantize", "gram")}
+        kern = {k: capi.profile_get(k) for k in ("anlm", "wprime", "pairdata", "xrows", "xgroup",
+                                                 "xenergy", "poly_deriv", "poly_gemm",
+                                                 "poly_gather", "quantize", "gram")}
         return (t1 - t0) / steps, kern
 
     log("timing mixed ...")

@@ -45,6 +45,8 @@ SOURCES = [
     ("kernels_surrogate.hip", []),
     ("kernels_poly.hip", []),
     ("poly.cpp", []),
+    ("kernels_xgroup.hip", []),
+    ("xgroup.cpp", []),
     ("surrogate.cpp", []),
 ]
 

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -555,8 +556,27 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover) {
     throw std::runtime_error("structure too large for the LDS row tile");
 }
 
+// Signature-group path for the linear columns: tasks per workgroup (TB) and
+// waves per task (A) for the largest structure, or TB = 0 when its rows do
+// not fit the LDS tile (or the path is switched off: PLM_XROWS=generic).
+static void plan_group(ModelHandle& m, int rows_max, bool fp64, int& TB, int& A) {
+    TB = 0, A = 0;
+    const char* env = std::getenv("PLM_XROWS");
+    if (env && std::string(env) == "generic") return;
+    if (!m.groups().enabled) return;
+    const int NSW = xgroup_slots(fp64), elem = 8;  // the row tile is FP64 in both precisions
+    const size_t per_col = (size_t)(rows_max - 1) * elem;
+    const int cols = (int)((kLdsBytes - 1024) / per_col) - 1;
+    const int waves = xgroup_max_waves(fp64);
+    int tb = std::min(8, cols / NSW);
+    while (tb > 1 && waves % tb) --tb;  // the waves split evenly over the tasks
+    if (tb < 1) return;
+    TB = tb;
+    A = waves / TB;
+}
+
 void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
-                  const double* roww, double* xe_sum, double* xe_sq, XWork& work) {
+                  const double* roww, double* xe_sum, double* xe_sq, XWork& work, bool x_is_zero) {
     Device& dev = device();
     hipStream_t st = dev.stream;
     const int a0 = b.atom_off[s0], a1 = b.atom_off[s1];
@@ -571,33 +591,69 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     }
     check_hip(hipGetLastError(), "anlm launch");
     const int w_stride = std::max(m.wz_max_type, 1), x_stride = std::max(m.lin_max_type, 1);
+    int rows_max = 0;
+    for (int s = s0; s < s1; ++s) rows_max = std::max(rows_max, 7 + 3 * (b.atom_off[s + 1] - b.atom_off[s]));
+    const bool use_poly = m.poly().enabled;
+    int TB = 0, A = 0;
+    plan_group(m, rows_max, fp64, TB, A);
+    const GroupDev* gd = TB > 0 ? &m.groups().dev : nullptr;
     work.w.alloc((size_t)na * w_stride * 2 * elem);
     work.x.alloc((size_t)na * x_stride * elem);
+    if (gd) work.wg.alloc((size_t)na * gd->wg_stride * 2 * elem);
     {
         ProfScope ps("wprime", st);
         launch_wprime(st, fp64, m.md, bd, work.a.ptr, a_stride, a0, na, work.w.ptr, w_stride,
-                      work.x.ptr, x_stride, m.max_nlm);
+                      work.x.ptr, x_stride, m.max_nlm, gd, gd ? work.wg.ptr : nullptr);
     }
     check_hip(hipGetLastError(), "wprime launch");
     const long pb = b.nb_off[a0], pe = b.nb_off[a1];
     const PairRec R(m.md.n_fn, m.md.n_lmh);
     work.pd.alloc((size_t)std::max<long>(pe - pb, 1) * R.stride * elem);
+    if (gd) work.tpo.alloc((size_t)na * (m.md.n_tp + 1) * sizeof(int));
     {
         ProfScope ps("pairdata", st);
-        launch_pairdata(st, fp64, m.md, bd, a0, na, pb, work.pd.ptr, R, m.md.max_l);
+        launch_pairdata(st, fp64, m.md, bd, a0, na, pb, work.pd.ptr, R, m.md.max_l,
+                        gd ? (int*)work.tpo.ptr : nullptr);
     }
     check_hip(hipGetLastError(), "pairdata launch");
-    int rows_max = 0;
-    for (int s = s0; s < s1; ++s) rows_max = std::max(rows_max, 7 + 3 * (b.atom_off[s + 1] - b.atom_off[s]));
-    const bool use_poly = m.poly().enabled;
-    XPlan pl = plan_x(m, rows_max, fp64, use_poly ? m.host.n_linear() : m.host.n_features());
-    pl.P.ld = ld;
-    pl.P.st0 = s0;
-    pl.P.st_end = s1;
-    pl.P.row_base = b.row_off[s0];
-    pl.P.xcd_map = (s1 - s0) >= 16 ? 1 : 0;
-    pl.P.pair_base = pb;
-    {
+    if (gd) {
+        // linear columns by signature groups: energy rows from the feature
+        // values, then one launch per atom type adding its stress and force rows
+        const GroupTables& G = m.groups();
+        const int nlin = m.host.n_linear();
+        const long rows = b.row_off[s1] - b.row_off[s0];
+        ProfScope ps("xrows", st);
+        if (!x_is_zero)
+            check_hip(hipMemset2DAsync(X, (size_t)ld * elem, 0, (size_t)nlin * elem, (size_t)rows, st),
+                      "memset");
+        {
+            ProfScope pe("xenergy", st);
+            launch_xenergy(st, fp64, *gd, bd, nlin, s0, s1 - s0, b.row_off[s0], a0, work.x.ptr,
+                           x_stride, X, ld, roww, xe_sum, xe_sq);
+        }
+        const int H = kGroupSlots / xgroup_slots(fp64);
+        ProfScope pg("xgroup", st);
+        for (int t = 0; t < m.host.n_type(); ++t) {
+            XGroupParams P{};
+            P.TB = TB, P.A = A;
+            P.vt0 = G.type_task0[t] * H, P.vt1 = G.type_task0[t + 1] * H;
+            if (P.vt1 <= P.vt0) continue;
+            P.n_blocks = (P.vt1 - P.vt0 + TB - 1) / TB;
+            P.ld = ld, P.st0 = s0, P.st_end = s1, P.row_base = b.row_off[s0];
+            P.xcd_map = (s1 - s0) >= 16 ? 1 : 0;
+            P.atom_base = a0;
+            P.pair_base = pb;
+            launch_xgroup(st, fp64, *gd, bd, P, rows_max, work.wg.ptr, work.pd.ptr, R,
+                          (const int*)work.tpo.ptr, X, roww);
+        }
+    } else {
+        XPlan pl = plan_x(m, rows_max, fp64, use_poly ? m.host.n_linear() : m.host.n_features());
+        pl.P.ld = ld;
+        pl.P.st0 = s0;
+        pl.P.st_end = s1;
+        pl.P.row_base = b.row_off[s0];
+        pl.P.xcd_map = (s1 - s0) >= 16 ? 1 : 0;
+        pl.P.pair_base = pb;
         ProfScope ps("xrows", st);
         launch_xrows(st, fp64, m.md, pl.ct->cd, bd, pl.ct->bw, pl.P, work.w.ptr, w_stride,
                      work.x.ptr, x_stride, a0, work.pd.ptr, R, X, roww, xe_sum, xe_sq);
@@ -608,10 +664,12 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
                         x_stride);
 }
 
-size_t atom_work_bytes(const ModelHandle& m, bool fp64) {
+size_t atom_work_bytes(ModelHandle& m, bool fp64) {
     const size_t elem = fp64 ? 8 : 4;
+    const GroupTables& G = m.groups();
+    const size_t grouped = G.enabled ? 2 * (size_t)G.dev.wg_stride * elem + (m.md.n_tp + 1) * 4 : 0;
     return elem * (2 * (size_t)m.max_nlm + 2 * (size_t)std::max(m.wz_max_type, 1) +
-                   (size_t)std::max(m.lin_max_type, 1));
+                   (size_t)std::max(m.lin_max_type, 1)) + grouped;
 }
 
 size_t pair_record_bytes(const ModelHandle& m, bool fp64) {

@@ -69,6 +69,13 @@ struct PolyTables {
     bool enabled = false;
 };
 
+struct GroupTables {
+    Store store;
+    GroupDev dev{};
+    bool enabled = false;
+    std::vector<int> type_task0;  // [n_type + 1] first task of each atom type
+};
+
 struct ModelHandle {
     Model host;
     Store store;
@@ -92,6 +99,9 @@ struct ModelHandle {
     // tables of the polynomial columns built by the factor-derivative GEMM
     std::unique_ptr<PolyTables> poly_tables;
     const PolyTables& poly();
+    // signature groups of the linear features (kernels_xgroup.hip)
+    std::unique_ptr<GroupTables> group_tables;
+    const GroupTables& groups();
 };
 
 struct BatchHandle {
@@ -131,15 +141,18 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover);
 // feature values per atom, pair records.
 struct XWork {
     DevBuf a, w, x, pd;
+    DevBuf wg, tpo;            // grouped W', per-atom type-pair ranges of the pair records
     DevBuf xf, m, d, doff, z;  // polynomial columns: factor values, monomials, D, Z
 };
 
 // X rows of structures [s0, s1) into X (row-major, leading dimension ld),
 // weighted by roww when given.
+// x_is_zero: X's rows of the range are already zero.
 void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
-                  const double* roww, double* xe_sum, double* xe_sq, XWork& work);
+                  const double* roww, double* xe_sum, double* xe_sq, XWork& work,
+                  bool x_is_zero = false);
 // device bytes per atom of the X build's per-atom work buffers
-size_t atom_work_bytes(const ModelHandle& m, bool fp64);
+size_t atom_work_bytes(ModelHandle& m, bool fp64);
 // polynomial columns of structures [s0, s1) (after the linear columns of the
 // same sub-batch have been built, which leaves the W', x and pair buffers)
 void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,

@@ -93,6 +93,31 @@ struct PolyDev {
     const int* mono_b;       // [U] second factor (-1 for degree 1)
 };
 
+// Signature groups of the linear features: features of one atom type whose
+// derivative weights W' use the same Y keys for every type pair (they differ
+// in the radial index and the coefficients). A task is up to kGroupSlots such
+// features (slots); tasks are sorted by atom type. The atom's W' entries of a
+// task and type pair are stored [key][slot] (re, im), so one key's weights of
+// all slots are contiguous.
+constexpr int kGroupSlots = 16;
+struct GroupDev {
+    int n_tasks;
+    int ntp;
+    const int* task_type;   // [n_tasks]
+    const int* task_gcol;   // [n_tasks][16] global column of each slot, -1 when empty
+    const int* tt_nkey;     // [n_tasks][ntp] Y keys of the task for the type pair
+    const int* tt_keyoff;   // [n_tasks][ntp] offset into keys
+    const int* keys;        // Y keys (-1: the constant Y_00 folded into W')
+    const int* tt_woff;     // [n_tasks][ntp] complex offset of the task's W' block per atom
+    const int* tt_nid;      // [n_tasks][ntp][16] radial index of each slot, -1 when empty
+    const int* g2l;         // [n_type][n_lin] local linear feature of a global column, -1 if none
+    const int* wg_pos;      // per type, concatenated: grouped position of each W' nonzero
+    const int* wg_pos_off;  // [n_type + 1]
+    const int* wg_hole;     // per type, concatenated: grouped positions without a nonzero
+    const int* wg_hole_off; // [n_type + 1]
+    int wg_stride;          // complex entries of the grouped W' per atom (max over types)
+};
+
 // A batch of structures resident in device memory.
 struct BatchDev {
     int n_st;

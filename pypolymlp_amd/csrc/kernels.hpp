@@ -65,14 +65,46 @@ struct BlockWz {
 
 struct PairRec;
 size_t pairdata_smem(const ModelDev& md, int elem);
+// tpo (optional): the records of each atom are then stored grouped by type
+// pair (stable within a type pair), and tpo[(i - atom0) * (n_tp + 1) + tp] is
+// the first pair index of type pair tp of atom i (the last entry its end).
 void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, int atom0,
-                     int n_atoms, long pair_base, void* pd, const PairRec& R, int lmax);
+                     int n_atoms, long pair_base, void* pd, const PairRec& R, int lmax,
+                     int* tpo = nullptr);
 // W'_i[g] for every nonzero g of the atom's type and the feature values
 // x_f(i), for atoms [atom0, atom0 + n_atoms): w[(i - atom0) * w_stride + g],
 // x[(i - atom0) * x_stride + f].
+// With wg given, W' is also written in the signature-group layout of gd
+// (wg[(i - atom0) * gd.wg_stride + position], holes zero).
 void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,
                    const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
-                   void* x, int x_stride, int max_nlm);
+                   void* x, int x_stride, int max_nlm, const GroupDev* gd = nullptr,
+                   void* wg = nullptr);
+
+// Linear columns by signature groups (kernels_xgroup.hip).
+struct XGroupParams {
+    int TB;         // tasks (column groups of NSW slots) per workgroup
+    int A;          // waves per task: atom streams
+    int vt0, vt1;   // virtual task range of the launch (one atom type)
+    int n_blocks;   // workgroups per structure
+    int ld;         // leading dimension of X
+    int st0, st_end;
+    int row_base;   // first X row of the structure range
+    int xcd_map;    // 1: all blocks of a structure on one XCD
+    int atom_base;  // first atom of the work buffers
+    long pair_base;
+};
+int xgroup_slots(bool fp64);      // slots per wave (NSW): 16 in FP32, 8 in FP64
+int xgroup_max_waves(bool fp64);  // waves per workgroup the kernel is built for
+size_t xgroup_smem(const XGroupParams& P, int rows_max, bool fp64);
+void launch_xgroup(hipStream_t st, bool fp64, const GroupDev& gd, const BatchDev& bd,
+                   const XGroupParams& P, int rows_max, const void* wg, const void* pd,
+                   const PairRec& R, const int* tpo, void* X, const double* roww);
+// energy rows of the linear columns [0, n_lin) of structures [st0, st0 + n_st)
+// from the feature values x (wprime), with the scale sums of the raw rows
+void launch_xenergy(hipStream_t st, bool fp64, const GroupDev& gd, const BatchDev& bd, int n_lin,
+                    int st0, int n_st, int row_base, int atom_base, const void* x, int x_stride,
+                    void* X, int ld, const double* roww, double* xe_sum, double* xe_sq);
 size_t xrows_smem(const XRowsParams& P, int elem);
 int xrows_stage_limit(bool fp64);
 int xrows_max_waves(bool fp64);  // waves per workgroup the kernel is built for

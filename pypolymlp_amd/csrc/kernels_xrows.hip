@@ -23,6 +23,8 @@
 //   PyModel row layout (energy, 6 stress, 3N forces)  src/cxx/src/compute/py_model.cpp:58-106
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "device_math.hpp"
 #include "device_tables.hpp"
 #include "kernels.hpp"
@@ -31,6 +33,7 @@
 namespace plm {
 
 constexpr int kPdPairs = 16;   // pairs per LDS chunk in pairdata_kernel
+constexpr int kMaxPairTypes = 64;  // type pairs the grouped record order supports
 // staged record elements per lane (64 lanes): FP32 groups are one record
 // (<= 768 values), FP64 groups two
 template <class T> struct StageRegs { static constexpr int n = sizeof(T) == 8 ? 24 : 12; };
@@ -39,11 +42,17 @@ template <class T> struct StageRegs { static constexpr int n = sizeof(T) == 8 ? 
 // Pair records (X path direction d = r_j - r_i + T, local.cpp:149).
 // One workgroup per atom.
 // ---------------------------------------------------------------------------
+// One wave per atom (64 threads). With tpo, the records are stored grouped by
+// type pair: lane tp keeps the next free index of type pair tp, and each
+// chunk's destinations come from ballots (stable, deterministic).
 template <class T>
-__global__ void __launch_bounds__(256) pairdata_kernel(ModelDev md, BatchDev bd, int atom0,
-                                                       int n_atoms, long pair_base,
-                                                       T* __restrict__ pd, PairRec R, int lmax) {
+__global__ void __launch_bounds__(64) pairdata_kernel(ModelDev md, BatchDev bd, int atom0,
+                                                      int n_atoms, long pair_base,
+                                                      T* __restrict__ pd, PairRec R, int lmax,
+                                                      int* __restrict__ tpo) {
     extern __shared__ unsigned char smem_raw[];
+    __shared__ int cnt[kMaxPairTypes];
+    __shared__ int dest[kPdPairs];
     const int ia = blockIdx.x;
     if (ia >= n_atoms) return;
     const int i = atom0 + ia;
@@ -54,10 +63,37 @@ __global__ void __launch_bounds__(256) pairdata_kernel(ModelDev md, BatchDev bd,
     const int p0 = bd.nb_off[i], p1 = bd.nb_off[i + 1];
     const int nfn = md.n_fn;
     const int pre = R.prefix(R.nkey);
+    const int tid = threadIdx.x, ntp = md.n_tp;
+    int run = 0;  // lane tp: next free pair index of type pair tp
+    if (tpo) {
+        if (tid < ntp) cnt[tid] = 0;
+        __syncthreads();
+        for (int k = p0 + tid; k < p1; k += 64)
+            atomicAdd(&cnt[md.tp_of[t1 * md.n_type + bd.types[bd.nb_j[k]]]], 1);
+        __syncthreads();
+        if (tid < ntp) {
+            run = p0;
+            for (int t = 0; t < tid; ++t) run += cnt[t];
+            tpo[(size_t)ia * (ntp + 1) + tid] = run;
+        }
+        if (tid == 0) tpo[(size_t)ia * (ntp + 1) + ntp] = p1;
+    }
     for (int c0 = p0; c0 < p1; c0 += kPdPairs) {
         const int np = min(kPdPairs, p1 - c0);
         __syncthreads();
         pairs_coop<T>(md, L, pairs, np, c0, bd, t1, a0, true, 1.0, lmax);
+        if (tpo) {
+            const int myt = tid < np ? int(L.misc(pairs + (size_t)tid * L.stride)[5]) : -1;
+            int d = 0;
+            for (int t = 0; t < ntp; ++t) {
+                const unsigned long long m = __ballot(myt == t);
+                const int base = __shfl(run, t);
+                if (myt == t) d = base + __popcll(m & ((1ull << tid) - 1ull));
+                if (tid == t) run += __popcll(m);
+            }
+            if (tid < np) dest[tid] = d;
+            __syncthreads();
+        }
         for (int e = threadIdx.x; e < np * pre; e += blockDim.x) {
             const int p = e / pre, o = e % pre;
             T* slot = pairs + (size_t)p * L.stride;
@@ -81,7 +117,8 @@ __global__ void __launch_bounds__(256) pairdata_kernel(ModelDev md, BatchDev bd,
             }
             // pairs beyond the cutoff keep only misc (valid = 0)
             if (o >= 8 && L.misc(slot)[4] == T(0)) v = T(0);
-            pd[(size_t)(c0 + p - pair_base) * R.stride + o] = v;
+            const long dp = tpo ? dest[p] : c0 + p;
+            pd[(size_t)(dp - pair_base) * R.stride + o] = v;
         }
     }
 }
@@ -111,13 +148,15 @@ __global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
                                                      const Cpx<T>* __restrict__ a_full,
                                                      int a_stride, int atom0, int n_atoms,
                                                      Cpx<T>* __restrict__ w, int w_stride,
-                                                     T* __restrict__ x, int x_stride) {
+                                                     T* __restrict__ x, int x_stride, GroupDev gd,
+                                                     Cpx<T>* __restrict__ wg) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     Cpx<T>* a = reinterpret_cast<Cpx<T>*>(smem_raw);
     const int ia = blockIdx.x;
     if (ia >= n_atoms) return;
     const int i = atom0 + ia;
-    const TypeDev& TT = md.t[bd.types[i]];
+    const int ti = bd.types[i];
+    const TypeDev& TT = md.t[ti];
     const Cpx<T>* ag = a_full + (size_t)ia * a_stride;
     for (int k = threadIdx.x; k < TT.n_nlm; k += blockDim.x) a[k] = ag[k];
     __syncthreads();
@@ -133,7 +172,11 @@ __global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
             v.im += cf * pr.im;
         }
         wo[g] = v;
+        if (wg) wg[(size_t)ia * gd.wg_stride + gd.wg_pos[gd.wg_pos_off[ti] + g]] = v;
     }
+    if (wg)
+        for (int e = gd.wg_hole_off[ti] + threadIdx.x; e < gd.wg_hole_off[ti + 1]; e += blockDim.x)
+            wg[(size_t)ia * gd.wg_stride + gd.wg_hole[e]] = Cpx<T>{T(0), T(0)};
     T* xo = x + (size_t)ia * x_stride;
     for (int f = threadIdx.x; f < TT.n_lin; f += blockDim.x) {
         T v = T(0);
@@ -446,18 +489,19 @@ size_t pairdata_smem(const ModelDev& md, int elem) {
 }
 
 void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, int atom0,
-                     int n_atoms, long pair_base, void* pd, const PairRec& R, int lmax) {
+                     int n_atoms, long pair_base, void* pd, const PairRec& R, int lmax, int* tpo) {
+    if (tpo && md.n_tp > kMaxPairTypes) throw std::invalid_argument("too many type pairs");
     const size_t sm = pairdata_smem(md, fp64 ? 8 : 4);
     if (fp64) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pairdata_kernel<double>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         pairdata_kernel<double><<<n_atoms, 64, sm, st>>>(md, bd, atom0, n_atoms, pair_base,
-                                                         reinterpret_cast<double*>(pd), R, lmax);
+                                                         reinterpret_cast<double*>(pd), R, lmax, tpo);
     } else {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pairdata_kernel<float>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         pairdata_kernel<float><<<n_atoms, 64, sm, st>>>(md, bd, atom0, n_atoms, pair_base,
-                                                        reinterpret_cast<float*>(pd), R, lmax);
+                                                        reinterpret_cast<float*>(pd), R, lmax, tpo);
     }
 }
 
@@ -471,20 +515,24 @@ int xrows_max_waves(bool fp64) { return fp64 ? 8 : 16; }
 
 void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,
                    const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
-                   void* x, int x_stride, int max_nlm) {
+                   void* x, int x_stride, int max_nlm, const GroupDev* gd, void* wg) {
     const size_t sm = (size_t)max_nlm * 2 * (fp64 ? 8 : 4);
+    const GroupDev g = gd ? *gd : GroupDev{};
+    if (!gd) wg = nullptr;
     if (fp64) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&wprime_kernel<double>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         wprime_kernel<double><<<n_atoms, 256, sm, st>>>(
             md, bd, reinterpret_cast<const Cpx<double>*>(a_full), a_stride, atom0, n_atoms,
-            reinterpret_cast<Cpx<double>*>(w), w_stride, reinterpret_cast<double*>(x), x_stride);
+            reinterpret_cast<Cpx<double>*>(w), w_stride, reinterpret_cast<double*>(x), x_stride, g,
+            reinterpret_cast<Cpx<double>*>(wg));
     } else {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&wprime_kernel<float>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
         wprime_kernel<float><<<n_atoms, 256, sm, st>>>(
             md, bd, reinterpret_cast<const Cpx<float>*>(a_full), a_stride, atom0, n_atoms,
-            reinterpret_cast<Cpx<float>*>(w), w_stride, reinterpret_cast<float*>(x), x_stride);
+            reinterpret_cast<Cpx<float>*>(w), w_stride, reinterpret_cast<float*>(x), x_stride, g,
+            reinterpret_cast<Cpx<float>*>(wg));
     }
 }
 

@@ -56,7 +56,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
         check_hip(hipMemcpyAsync(acc.wbuf.ptr, row_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
         check_hip(hipMemcpyAsync(acc.ybuf.ptr, y_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
         build_x_rows(m, b, s0, s1, fp64, acc.xbuf.ptr, ld, (const double*)acc.wbuf.ptr, xe_sum, xe_sq,
-                     acc.work);
+                     acc.work, true);
         launch_set_column(st, fp64, acc.xbuf.ptr, ld, F, (const double*)acc.ybuf.ptr, (int)rows);
         check_hip(hipGetLastError(), "target column");
         if (fp64) {

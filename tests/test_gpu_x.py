@@ -110,27 +110,50 @@ def test_neighbor_full_and_half_bit_exact():
             assert np.array_equal(d, rd), name
 
 
+def _use_path(path, monkeypatch):
+    """The linear columns are built by the signature-group kernel unless
+    PLM_XROWS=generic selects the column-block kernel."""
+    if path == "generic":
+        monkeypatch.setenv("PLM_XROWS", "generic")
+    else:
+        monkeypatch.delenv("PLM_XROWS", raising=False)
+
+
+def _features_x(m, b, prec, path):
+    capi.profile_reset()
+    capi.profile_enable(True)
+    X = capi.features_x(m, b, prec)
+    capi.profile_enable(False)
+    n_group = capi.profile_get("xgroup")[1]
+    assert (n_group > 0) == (path == "group"), f"path {path}: {n_group} signature-group launches"
+    return X
+
+
+@pytest.mark.parametrize("path", ["group", "generic"])
 @pytest.mark.parametrize("model", ["polymlp.yaml.gtinv.MgO", "polymlp.yaml.pair.MgO"])
-def test_x_rows_mgo(model):
+def test_x_rows_mgo(model, path, monkeypatch):
+    _use_path(path, monkeypatch)
     spec, _ = load_model(model)
     m = capi.Model(spec.as_dict())
     sts = [poscar("POSCAR-00001.MgO"), poscar("POSCAR-00002.MgO")]
     axes, pos, types = _structs(sts)
-    say(f"x rows model={model} FP64")
+    say(f"x rows model={model} path={path} FP64")
     b = capi.Batch(m, axes, pos, types)
-    X = capi.features_x(m, b, capi.PLM_FP64)
+    X = _features_x(m, b, capi.PLM_FP64, path)
     Xr, noise = oracle_x_and_noise(ref.RefModel(ref_file(model)), axes, pos, types)
     assert X.shape == Xr.shape
     strict, loose, n_loose = x_parity(X, Xr, noise)
     say(f"  fp64 column-relative err {strict:.2e} (resolved), {loose:.2e} over {n_loose} cancelling columns")
     assert strict <= 1e-10
     assert loose <= 1e-6
-    say(f"x rows model={model} MIXED")
-    check_mixed(capi.features_x(m, b, capi.PLM_MIXED), Xr, noise)
+    say(f"x rows model={model} path={path} MIXED")
+    check_mixed(_features_x(m, b, capi.PLM_MIXED, path), Xr, noise)
 
 
+@pytest.mark.parametrize("path", ["group", "generic"])
 @pytest.mark.parametrize("cfg,n_st", [("cfg1", 3), ("cfg2", 2), ("cfg3", 4)])
-def test_x_rows_configs(cfg, n_st, tmp_path):
+def test_x_rows_configs(cfg, n_st, path, tmp_path, monkeypatch):
+    _use_path(path, monkeypatch)
     spec = getattr(datasets, f"config{cfg[-1]}_model")()
     if cfg == "cfg1":
         ds = datasets.config1_dataset(n_st)
@@ -141,18 +164,18 @@ def test_x_rows_configs(cfg, n_st, tmp_path):
         ds.structures = sorted(ds.structures, key=lambda s: s.n_total)[:n_st]
     m = capi.Model(spec.as_dict())
     axes, pos, types = _structs(ds.structures)
-    say(f"x rows {cfg} n_st={n_st} atoms={[len(t) for t in types]}")
+    say(f"x rows {cfg} path={path} n_st={n_st} atoms={[len(t) for t in types]}")
     b = capi.Batch(m, axes, pos, types)
-    X = capi.features_x(m, b, capi.PLM_FP64)
-    path = write_model_yaml(spec, str(tmp_path / "model.yaml"), np.ones(m.n_features))
-    Xr, noise = oracle_x_and_noise(ref.RefModel(path), axes, pos, types)
+    X = _features_x(m, b, capi.PLM_FP64, path)
+    yaml_path = write_model_yaml(spec, str(tmp_path / "model.yaml"), np.ones(m.n_features))
+    Xr, noise = oracle_x_and_noise(ref.RefModel(yaml_path), axes, pos, types)
     strict, loose, n_loose = x_parity(X, Xr, noise)
     say(f"  fp64 column-relative err {strict:.2e} (resolved), {loose:.2e} over {n_loose} cancelling columns")
     report_worst(X, Xr, noise)
     assert np.linalg.norm(X - Xr) <= 1e-10 * np.linalg.norm(Xr)
     assert strict <= 1e-10
     assert loose <= 1e-6
-    check_mixed(capi.features_x(m, b, capi.PLM_MIXED), Xr, noise)
+    check_mixed(_features_x(m, b, capi.PLM_MIXED, path), Xr, noise)
 
 
 def test_normal_equations_cfg1():
