@@ -137,21 +137,24 @@ __global__ void quantize_kernel(const float* __restrict__ X, int ld, int rows_pa
 }
 
 // ---------------------------------------------------------------------------
-// Mixed: 128 (i) x 64 (j) output tile per 256-thread workgroup, each wave a
-// 32x64 strip as two 32x32 blocks with five int32 accumulators each (levels
-// s + t = 0..4). Stages of 64 rows: each thread's share of a stage is loaded
-// into registers one stage ahead and stored into one of two LDS buffers
-// (48-byte column stride, b128 reads without bank conflicts). Tiles (bi, bj) with bj >= 2 bi cover
-// the upper triangle.
+// Mixed: 128 (i) x 64 (j) output tile per 512-thread workgroup; each of the 8
+// waves owns one 32x32 block with five int32 accumulators (levels s + t =
+// 0..4), so two waves run per SIMD. Stages of 64 rows: each thread's share of
+// a stage is loaded into registers one stage ahead and stored into one of two
+// LDS buffers (48-byte column stride, b128 reads without bank conflicts).
+// Tiles (bi, bj) with bj >= 2 bi cover the upper triangle.
 // ---------------------------------------------------------------------------
 constexpr int GI_TI = 128;
 constexpr int GI_TJ = 64;
+constexpr int GI_NT = 512;                       // threads per workgroup
 constexpr int GI_CS = 48;                        // LDS bytes per column and 32-row block
 constexpr int GI_RB = 2;                         // 32-row blocks per stage
 constexpr int GI_A = 3 * GI_RB * GI_TI * GI_CS;  // A panel bytes per stage
 constexpr int GI_B = 3 * GI_RB * GI_TJ * GI_CS;  // B panel bytes per stage
-constexpr int GI_LA = 3 * GI_RB * GI_TI * 2 / 256;  // 16-byte A pieces per thread per stage
-constexpr int GI_LB = 3 * GI_RB * GI_TJ * 2 / 256;  // 16-byte B pieces per thread per stage
+constexpr int GI_PA = 3 * GI_RB * GI_TI * 2;     // 16-byte A pieces per stage
+constexpr int GI_PB = 3 * GI_RB * GI_TJ * 2;     // 16-byte B pieces per stage
+constexpr int GI_LA = (GI_PA + GI_NT - 1) / GI_NT;  // A pieces per thread
+constexpr int GI_LB = (GI_PB + GI_NT - 1) / GI_NT;  // B pieces per thread
 
 __device__ __forceinline__ void tile_index_i8(int t, int nbi, int& bi, int& bj) {
     const int nbj = 2 * nbi;
@@ -162,25 +165,23 @@ __device__ __forceinline__ void tile_index_i8(int t, int nbi, int& bi, int& bj) 
 
 size_t gram_i8_smem() { return 2 * (size_t)(GI_A + GI_B); }
 
-__global__ void __launch_bounds__(256) gram_i8_kernel(const signed char* __restrict__ Q, int ld,
-                                                      int rows_pad, int nbi,
-                                                      const int* __restrict__ E,
-                                                      double* __restrict__ C, int ldc) {
+__global__ void __launch_bounds__(GI_NT) gram_i8_kernel(const signed char* __restrict__ Q, int ld,
+                                                        int rows_pad, int nbi,
+                                                        const int* __restrict__ E,
+                                                        double* __restrict__ C, int ldc) {
     extern __shared__ __attribute__((aligned(16))) signed char smem_i8[];
     int bi, bj;
     tile_index_i8(blockIdx.x, nbi, bi, bj);
     const int I0 = bi * GI_TI, J0 = bj * GI_TJ;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = wave * 32;
+    const int wi = (wave & 3) * 32, wj = (wave >> 2) * 32;
     const int nrb = rows_pad / kQK;
     const int nst = nrb / GI_RB;
     const size_t plane = (size_t)nrb * ld * kQK;
-    v16i acc[2][5];
+    v16i acc[5];
 #pragma unroll
-    for (int n = 0; n < 2; ++n)
-#pragma unroll
-        for (int k = 0; k < 5; ++k) acc[n][k] = v16i{};
-    // piece p = tid + 256 r -> ((s * GI_RB + rbo) * cols + col) * 2 + half
+    for (int k = 0; k < 5; ++k) acc[k] = v16i{};
+    // piece p -> ((s * GI_RB + rbo) * cols + col) * 2 + half
     auto g_off = [&](int p, int cols, int col0, int st) -> size_t {
         const int half = p & 1, col = (p >> 1) % cols, sr = (p >> 1) / cols;
         const int s = sr / GI_RB, rbo = sr % GI_RB;
@@ -191,23 +192,31 @@ __global__ void __launch_bounds__(256) gram_i8_kernel(const signed char* __restr
         return (sr * cols + col) * GI_CS + 16 * half;
     };
     v4i ra[GI_LA], rb[GI_LB];
-    auto load = [&](int st, v4i* ra, v4i* rb) {
+    auto load = [&](int st) {
 #pragma unroll
-        for (int r = 0; r < GI_LA; ++r)
-            ra[r] = *reinterpret_cast<const v4i*>(Q + g_off(tid + 256 * r, GI_TI, I0, st));
+        for (int r = 0; r < GI_LA; ++r) {
+            const int p = tid + GI_NT * r;
+            if (p < GI_PA) ra[r] = *reinterpret_cast<const v4i*>(Q + g_off(p, GI_TI, I0, st));
+        }
 #pragma unroll
-        for (int r = 0; r < GI_LB; ++r)
-            rb[r] = *reinterpret_cast<const v4i*>(Q + g_off(tid + 256 * r, GI_TJ, J0, st));
+        for (int r = 0; r < GI_LB; ++r) {
+            const int p = tid + GI_NT * r;
+            if (p < GI_PB) rb[r] = *reinterpret_cast<const v4i*>(Q + g_off(p, GI_TJ, J0, st));
+        }
     };
-    auto store = [&](int buf, const v4i* ra, const v4i* rb) {
+    auto store = [&](int buf) {
         signed char* A = smem_i8 + (size_t)buf * (GI_A + GI_B);
         signed char* B = A + GI_A;
 #pragma unroll
-        for (int r = 0; r < GI_LA; ++r)
-            *reinterpret_cast<v4i*>(A + l_off(tid + 256 * r, GI_TI)) = ra[r];
+        for (int r = 0; r < GI_LA; ++r) {
+            const int p = tid + GI_NT * r;
+            if (p < GI_PA) *reinterpret_cast<v4i*>(A + l_off(p, GI_TI)) = ra[r];
+        }
 #pragma unroll
-        for (int r = 0; r < GI_LB; ++r)
-            *reinterpret_cast<v4i*>(B + l_off(tid + 256 * r, GI_TJ)) = rb[r];
+        for (int r = 0; r < GI_LB; ++r) {
+            const int p = tid + GI_NT * r;
+            if (p < GI_PB) *reinterpret_cast<v4i*>(B + l_off(p, GI_TJ)) = rb[r];
+        }
     };
     auto compute = [&](int buf) {
         const signed char* A = smem_i8 + (size_t)buf * (GI_A + GI_B);
@@ -215,67 +224,60 @@ __global__ void __launch_bounds__(256) gram_i8_kernel(const signed char* __restr
 #pragma unroll 1
         for (int rbo = 0; rbo < GI_RB; ++rbo) {
             // lane -> column (lane % 32) of its block, k half (lane / 32)
-            v4i a[3], b[2][3];
+            v4i a[3], b[3];
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
                 a[s] = *reinterpret_cast<const v4i*>(
                     A + ((s * GI_RB + rbo) * GI_TI + wi + (lane & 31)) * GI_CS + 16 * (lane >> 5));
-#pragma unroll
-                for (int n = 0; n < 2; ++n)
-                    b[n][s] = *reinterpret_cast<const v4i*>(
-                        B + ((s * GI_RB + rbo) * GI_TJ + 32 * n + (lane & 31)) * GI_CS +
-                        16 * (lane >> 5));
+                b[s] = *reinterpret_cast<const v4i*>(
+                    B + ((s * GI_RB + rbo) * GI_TJ + wj + (lane & 31)) * GI_CS + 16 * (lane >> 5));
             }
-            // issue order: the k-th product of every (block, level) accumulator
-            // before any (k+1)-th, so dependent MFMAs are 6-10 apart
+            // issue order: the k-th product of every level before any (k+1)-th
             constexpr int kS[3][5] = {{0, 0, 0, 1, 2}, {-1, 1, 1, 2, -1}, {-1, -1, 2, -1, -1}};
 #pragma unroll
             for (int k = 0; k < 3; ++k)
 #pragma unroll
-                for (int lvl = 0; lvl < 5; ++lvl)
-#pragma unroll
-                    for (int n = 0; n < 2; ++n) {
-                        const int s = kS[k][lvl];
-                        if (s < 0) continue;
-                        acc[n][lvl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b[n][lvl - s],
-                                                                            acc[n][lvl], 0, 0, 0);
-                    }
+                for (int lvl = 0; lvl < 5; ++lvl) {
+                    const int s = kS[k][lvl];
+                    if (s < 0) continue;
+                    acc[lvl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b[lvl - s], acc[lvl],
+                                                                     0, 0, 0);
+                }
         }
     };
     // x~_i x~_j = 2^(E_i + E_j - 14) sum_k acc_k 2^(-7k): exact int32 sums
     // combined in FP64; D[(v%4) + 8(v/4) + 4(l/32)][l%32]
     auto flush = [&](int chunk) {
         const int* Ec = E + (size_t)chunk * ld;
+        const int i0 = I0 + wi + 4 * (lane >> 5);
+        const int j = J0 + wj + (lane & 31);
+        const int ej = Ec[j];
+        double* cb = C + (size_t)i0 * ldc + j;
 #pragma unroll
-        for (int n = 0; n < 2; ++n) {
-            const int j = J0 + 32 * n + (lane & 31);
-            const int ej = Ec[j];
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int i = I0 + wi + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
-                double t = double(acc[n][4][v]);
-                t = t * 0.0078125 + double(acc[n][3][v]);
-                t = t * 0.0078125 + double(acc[n][2][v]);
-                t = t * 0.0078125 + double(acc[n][1][v]);
-                t = t * 0.0078125 + double(acc[n][0][v]);
-                C[(size_t)i * ldc + j] += ldexp(t, Ec[i] + ej - 14);
-            }
-#pragma unroll
-            for (int k = 0; k < 5; ++k) acc[n][k] = v16i{};
+        for (int v = 0; v < 16; ++v) {
+            const int di = (v & 3) + 8 * (v >> 2);
+            double t = double(acc[4][v]);
+            t = t * 0.0078125 + double(acc[3][v]);
+            t = t * 0.0078125 + double(acc[2][v]);
+            t = t * 0.0078125 + double(acc[1][v]);
+            t = t * 0.0078125 + double(acc[0][v]);
+            cb[(size_t)di * ldc] += ldexp(t, Ec[i0 + di] + ej - 14);
         }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[k] = v16i{};
     };
     const int st_per_chunk = kQChunk / (kQK * GI_RB);
     // stage st + 1 is stored to LDS at the start of iteration st (its loads
     // were issued one iteration earlier), then stage st + 2 is loaded into the
     // freed registers while stage st is multiplied
-    load(0, ra, rb);
-    store(0, ra, rb);
-    if (nst > 1) load(1, ra, rb);
+    load(0);
+    store(0);
+    if (nst > 1) load(1);
     __syncthreads();
     for (int st = 0; st < nst; ++st) {
         if (st + 1 < nst) {
-            store((st + 1) & 1, ra, rb);
-            if (st + 2 < nst) load(st + 2, ra, rb);
+            store((st + 1) & 1);
+            if (st + 2 < nst) load(st + 2);
         }
         compute(st & 1);
         if ((st + 1) % st_per_chunk == 0 || st + 1 == nst) flush(st / st_per_chunk);
@@ -335,7 +337,7 @@ void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int F_pad, const void*
     const size_t sm = gram_i8_smem();
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&gram_i8_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    gram_i8_kernel<<<nbi * (nbi + 1), 256, sm, st>>>(Q, ld, rows_pad, nbi, E, C, ldc);
+    gram_i8_kernel<<<nbi * (nbi + 1), GI_NT, sm, st>>>(Q, ld, rows_pad, nbi, E, C, ldc);
 }
 
 }  // namespace plm
