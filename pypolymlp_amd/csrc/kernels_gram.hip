@@ -142,7 +142,6 @@ __global__ void quantize_kernel(const float* __restrict__ X, int ld, int rows_pa
 // 0..4), so two waves run per SIMD. Stages of 64 rows: each thread's share of
 // a stage is loaded into registers one stage ahead and stored into one of two
 // LDS buffers (48-byte column stride, b128 reads without bank conflicts).
-// Tiles (bi, bj) with bj >= 2 bi cover the upper triangle.
 // ---------------------------------------------------------------------------
 constexpr int GI_TI = 128;
 constexpr int GI_TJ = 64;
@@ -156,11 +155,25 @@ constexpr int GI_PB = 3 * GI_RB * GI_TJ * 2;     // 16-byte B pieces per stage
 constexpr int GI_LA = (GI_PA + GI_NT - 1) / GI_NT;  // A pieces per thread
 constexpr int GI_LB = (GI_PB + GI_NT - 1) / GI_NT;  // B pieces per thread
 
-__device__ __forceinline__ void tile_index_i8(int t, int nbi, int& bi, int& bj) {
-    const int nbj = 2 * nbi;
-    int r = 0, rem = t;
-    while (rem >= nbj - 2 * r) rem -= nbj - 2 * r, ++r;
-    bi = r, bj = 2 * r + rem;
+// Tile order: workgroups are dealt to the 8 XCDs round robin; each XCD runs
+// one super-tile of 4 x 8 tiles (512 x 512 outputs) at a time, so the panels
+// of concurrently running tiles are shared in its L2. Super-tiles (SI, SJ),
+// SJ >= SI, cover the upper triangle; tiles below it return at once.
+__device__ __forceinline__ bool tile_index_i8(int b, int nbi, int& bi, int& bj) {
+    const int xcd = b & 7, k = b >> 3;
+    const int sup = (k >> 5) * 8 + xcd, loc = k & 31;
+    const int nsi = (nbi + 3) / 4;
+    int r = 0, rem = sup;
+    while (r < nsi && rem >= nsi - r) rem -= nsi - r, ++r;
+    if (r >= nsi) return false;
+    bi = r * 4 + (loc >> 3);
+    bj = (r + rem) * 8 + (loc & 7);
+    return bi < nbi && bj < 2 * nbi && bj >= 2 * bi;
+}
+
+static long gram_i8_grid(int nbi) {
+    const long nsi = (nbi + 3) / 4, nsup = nsi * (nsi + 1) / 2;
+    return (nsup + 7) / 8 * 8 * 32;
 }
 
 size_t gram_i8_smem() { return 2 * (size_t)(GI_A + GI_B); }
@@ -171,7 +184,7 @@ __global__ void __launch_bounds__(GI_NT) gram_i8_kernel(const signed char* __res
                                                         double* __restrict__ C, int ldc) {
     extern __shared__ __attribute__((aligned(16))) signed char smem_i8[];
     int bi, bj;
-    tile_index_i8(blockIdx.x, nbi, bi, bj);
+    if (!tile_index_i8(blockIdx.x, nbi, bi, bj)) return;
     const int I0 = bi * GI_TI, J0 = bj * GI_TJ;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 3) * 32, wj = (wave >> 2) * 32;
@@ -337,7 +350,7 @@ void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int F_pad, const void*
     const size_t sm = gram_i8_smem();
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&gram_i8_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    gram_i8_kernel<<<nbi * (nbi + 1), GI_NT, sm, st>>>(Q, ld, rows_pad, nbi, E, C, ldc);
+    gram_i8_kernel<<<gram_i8_grid(nbi), GI_NT, sm, st>>>(Q, ld, rows_pad, nbi, E, C, ldc);
 }
 
 }  // namespace plm
