@@ -583,13 +583,7 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     const int na = a1 - a0;
     const int elem = fp64 ? 8 : 4;
     const int a_stride = m.max_nlm;
-    work.a.alloc((size_t)na * a_stride * 2 * elem);
     BatchDev bd = b.view();
-    {
-        ProfScope ps("anlm", st);
-        launch_anlm(st, fp64, m.md, bd, work.a.ptr, a_stride, a0, na, m.max_noconj);
-    }
-    check_hip(hipGetLastError(), "anlm launch");
     const int w_stride = std::max(m.wz_max_type, 1), x_stride = std::max(m.lin_max_type, 1);
     int rows_max = 0;
     for (int s = s0; s < s1; ++s) rows_max = std::max(rows_max, 7 + 3 * (b.atom_off[s + 1] - b.atom_off[s]));
@@ -597,6 +591,24 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     int TB = 0, A = 0;
     plan_group(m, rows_max, fp64, TB, A);
     const GroupDev* gd = TB > 0 ? &m.groups().dev : nullptr;
+    // pair records (grouped by type pair), then the order parameters from them
+    const long pb = b.nb_off[a0], pe = b.nb_off[a1];
+    const PairRec R(m.md.n_fn, m.md.n_lmh);
+    work.pd.alloc((size_t)std::max<long>(pe - pb, 1) * R.stride * elem);
+    work.tpo.alloc((size_t)na * (m.md.n_tp + 1) * sizeof(int));
+    {
+        ProfScope ps("pairdata", st);
+        launch_pairdata(st, fp64, m.md, bd, a0, na, pb, work.pd.ptr, R, m.md.max_l,
+                        (int*)work.tpo.ptr);
+    }
+    check_hip(hipGetLastError(), "pairdata launch");
+    work.a.alloc((size_t)na * a_stride * 2 * elem);
+    {
+        ProfScope ps("anlm", st);
+        launch_anlm_rec(st, fp64, m.md, bd, a0, na, pb, work.pd.ptr, R, (const int*)work.tpo.ptr,
+                        work.a.ptr, a_stride, m.max_noconj);
+    }
+    check_hip(hipGetLastError(), "anlm launch");
     work.w.alloc((size_t)na * w_stride * 2 * elem);
     work.x.alloc((size_t)na * x_stride * elem);
     if (gd) work.wg.alloc((size_t)na * gd->wg_stride * 2 * elem);
@@ -606,16 +618,6 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
                       work.x.ptr, x_stride, m.max_nlm, gd, gd ? work.wg.ptr : nullptr);
     }
     check_hip(hipGetLastError(), "wprime launch");
-    const long pb = b.nb_off[a0], pe = b.nb_off[a1];
-    const PairRec R(m.md.n_fn, m.md.n_lmh);
-    work.pd.alloc((size_t)std::max<long>(pe - pb, 1) * R.stride * elem);
-    if (gd) work.tpo.alloc((size_t)na * (m.md.n_tp + 1) * sizeof(int));
-    {
-        ProfScope ps("pairdata", st);
-        launch_pairdata(st, fp64, m.md, bd, a0, na, pb, work.pd.ptr, R, m.md.max_l,
-                        gd ? (int*)work.tpo.ptr : nullptr);
-    }
-    check_hip(hipGetLastError(), "pairdata launch");
     if (gd) {
         // linear columns by signature groups: energy rows from the feature
         // values, then one launch per atom type adding its stress and force rows
@@ -627,7 +629,7 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
             check_hip(hipMemset2DAsync(X, (size_t)ld * elem, 0, (size_t)nlin * elem, (size_t)rows, st),
                       "memset");
         {
-            ProfScope pe("xenergy", st);
+            ProfScope px("xenergy", st);
             launch_xenergy(st, fp64, *gd, bd, nlin, s0, s1 - s0, b.row_off[s0], a0, work.x.ptr,
                            x_stride, X, ld, roww, xe_sum, xe_sq);
         }

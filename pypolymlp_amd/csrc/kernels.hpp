@@ -74,6 +74,11 @@ void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchD
 // W'_i[g] for every nonzero g of the atom's type and the feature values
 // x_f(i), for atoms [atom0, atom0 + n_atoms): w[(i - atom0) * w_stride + g],
 // x[(i - atom0) * x_stride + f].
+// Order parameters of the X path from the pair records (grouped by type pair,
+// tpo from launch_pairdata): a_full[(i - atom0) * a_stride + k].
+void launch_anlm_rec(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, int atom0,
+                     int n_atoms, long pair_base, const void* pd, const PairRec& R,
+                     const int* tpo, void* a_full, int a_stride, int max_noconj);
 // With wg given, W' is also written in the signature-group layout of gd
 // (wg[(i - atom0) * gd.wg_stride + position], holes zero).
 void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,

@@ -124,6 +124,77 @@ __global__ void __launch_bounds__(64) pairdata_kernel(ModelDev md, BatchDev bd, 
 }
 
 // ---------------------------------------------------------------------------
+// Order parameters of the X path from the pair records (records grouped by
+// type pair, tpo): a_h = sum over the pairs of head h's type pair of
+// f_n(r) Y_lm(d), pairs with f_n < 1e-20 skipped (local.cpp:164), in the
+// neighbour order within the type pair; then the atom's full component list
+// with the conjugates (Local::compute_anlmtp, local.cpp:62-110). One
+// workgroup per atom; chunks of pairs staged in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kArPairs = 32;  // pairs per LDS chunk in anlm_rec_kernel
+
+template <class T>
+__global__ void __launch_bounds__(256) anlm_rec_kernel(ModelDev md, BatchDev bd, int atom0,
+                                                       int n_atoms, long pair_base,
+                                                       const T* __restrict__ pd, PairRec R,
+                                                       const int* __restrict__ tpo,
+                                                       Cpx<T>* __restrict__ a_full, int a_stride) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int ia = blockIdx.x;
+    if (ia >= n_atoms) return;
+    const int i = atom0 + ia;
+    const TypeDev& TT = md.t[bd.types[i]];
+    const int nfn = md.n_fn, nlmh = md.n_lmh, ntp = md.n_tp;
+    const int W = nfn + 2 * nlmh;  // staged values per pair: f_n, then (Re, Im) Y per key
+    T* fL = reinterpret_cast<T*>(smem_raw);
+    Cpx<T>* acc = reinterpret_cast<Cpx<T>*>(fL + (size_t)kArPairs * W);
+    const int tid = threadIdx.x;
+    for (int h = tid; h < TT.n_noconj; h += blockDim.x) acc[h] = {T(0), T(0)};
+    const int* to = tpo + (size_t)ia * (ntp + 1);
+    for (int tp = 0; tp < ntp; ++tp) {
+        const int h0 = TT.nc_tp_off[tp], h1 = TT.nc_tp_off[tp + 1];
+        if (h0 == h1) continue;
+        for (int c0 = to[tp]; c0 < to[tp + 1]; c0 += kArPairs) {
+            const int np = min(kArPairs, to[tp + 1] - c0);
+            __syncthreads();
+            for (int e = tid; e < np * W; e += blockDim.x) {
+                const int p = e / W, o = e % W;
+                const T* rec = pd + (size_t)(c0 + p - pair_base) * R.stride;
+                fL[e] = o < nfn ? rec[R.fn() + o] : rec[R.ylm((o - nfn) >> 1) + ((o - nfn) & 1)];
+            }
+            __syncthreads();
+            for (int h = h0 + tid; h < h1; h += blockDim.x) {
+                const int nid = TT.nc_tp_nid[h], key = TT.nc_tp_ylm[h];
+                Cpx<T> v = acc[TT.nc_tp_head[h]];
+                for (int p = 0; p < np; ++p) {
+                    const T* q = fL + (size_t)p * W;
+                    const T f = q[nid];
+                    if (f < T(1e-20)) continue;
+                    if (key < 0) {
+                        v.re += f;
+                    } else {
+                        v.re += f * q[nfn + 2 * key];
+                        v.im += f * q[nfn + 2 * key + 1];
+                    }
+                }
+                acc[TT.nc_tp_head[h]] = v;
+            }
+        }
+    }
+    __syncthreads();
+    Cpx<T>* out = a_full + (size_t)ia * a_stride;
+    for (int k = tid; k < TT.n_nlm; k += blockDim.x) {
+        const Cpx<T> v = acc[TT.nlm_src[k]];
+        if (TT.nlm_conj[k]) {
+            const T cc = T(TT.nlm_sign[k]);
+            out[k] = {cc * v.re, -cc * v.im};
+        } else {
+            out[k] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Derivative weights W'_i[g] of every nonzero g of the atom's type and the
 // feature values x_f(i), once per atom (one workgroup per atom, the order
 // parameters staged in LDS).
@@ -506,6 +577,26 @@ void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchD
 }
 
 size_t xrows_smem(const XRowsParams& P, int elem) { return XRowsLds(P, elem).total; }
+
+void launch_anlm_rec(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, int atom0,
+                     int n_atoms, long pair_base, const void* pd, const PairRec& R,
+                     const int* tpo, void* a_full, int a_stride, int max_noconj) {
+    const int elem = fp64 ? 8 : 4;
+    const size_t sm = (size_t)kArPairs * (md.n_fn + 2 * md.n_lmh) * elem + (size_t)max_noconj * 2 * elem;
+    if (fp64) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_rec_kernel<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        anlm_rec_kernel<double><<<n_atoms, 256, sm, st>>>(
+            md, bd, atom0, n_atoms, pair_base, reinterpret_cast<const double*>(pd), R, tpo,
+            reinterpret_cast<Cpx<double>*>(a_full), a_stride);
+    } else {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_rec_kernel<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        anlm_rec_kernel<float><<<n_atoms, 256, sm, st>>>(
+            md, bd, atom0, n_atoms, pair_base, reinterpret_cast<const float*>(pd), R, tpo,
+            reinterpret_cast<Cpx<float>*>(a_full), a_stride);
+    }
+}
 
 int xrows_stage_limit(bool fp64) {
     return 64 * (fp64 ? StageRegs<double>::n : StageRegs<float>::n);
