@@ -217,6 +217,86 @@ void ModelHandle::build_device() {
         D.wt_coeff = up(*this, wtc);
         D.wt_keys = up(*this, wtk);
         D.wt_conj = up(*this, wtj);
+        {
+            // interleaved W' terms: items sorted by decreasing term count
+            const int nnz = (int)wzn.size();
+            std::vector<int> cnt(nnz), order(nnz);
+            for (int g = 0; g < nnz; ++g) cnt[g] = wto[g + 1] - wto[g];
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+            const int nblk = (nnz + 63) / 64;
+            std::vector<int> boff(nblk + 1, 0), item((size_t)nblk * 64, -1), keys;
+            std::vector<double> cc;
+            for (int bk = 0; bk < nblk; ++bk) {
+                int rows = 0;
+                for (int l = 0; l < 64 && bk * 64 + l < nnz; ++l) rows = std::max(rows, cnt[order[bk * 64 + l]]);
+                boff[bk + 1] = boff[bk] + rows * 64;
+                keys.resize((size_t)boff[bk + 1] * 4, -1);
+                cc.resize((size_t)boff[bk + 1] * 2, 0.0);
+                for (int l = 0; l < 64 && bk * 64 + l < nnz; ++l) {
+                    const int g = order[bk * 64 + l];
+                    item[(size_t)bk * 64 + l] = g;
+                    for (int j = 0; j < cnt[g]; ++j) {
+                        const int e = wto[g] + j;
+                        const size_t r = (size_t)boff[bk] + (size_t)j * 64 + l;
+                        int nf = 0;
+                        for (int k = 0; k < 3; ++k) {
+                            keys[r * 4 + k] = wtk[(size_t)e * 3 + k];
+                            nf += wtk[(size_t)e * 3 + k] >= 0;
+                        }
+                        w_maxf = std::max(w_maxf, nf);
+                        cc[r * 2] = wtc[e];
+                        cc[r * 2 + 1] = wtj[e] ? -wtc[e] : wtc[e];
+                    }
+                }
+            }
+            std::vector<float> ccf(cc.begin(), cc.end());
+            D.wq_nblk = nblk;
+            D.wq_boff = up(*this, boff);
+            D.wq_item = up(*this, item);
+            D.wq_keys = up(*this, keys);
+            D.wq_cc = up(*this, cc);
+            D.wq_ccf = up(*this, ccf);
+        }
+        {
+            // interleaved value terms
+            std::vector<int> cnt(L), order(L);
+            for (int f = 0; f < L; ++f) cnt[f] = fvo[f + 1] - fvo[f];
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cnt[x] > cnt[y]; });
+            const int nblk = (L + 63) / 64;
+            std::vector<int> boff(nblk + 1, 0), item((size_t)nblk * 64, -1), keys;
+            std::vector<double> c;
+            for (int bk = 0; bk < nblk; ++bk) {
+                int rows = 0;
+                for (int l = 0; l < 64 && bk * 64 + l < L; ++l) rows = std::max(rows, cnt[order[bk * 64 + l]]);
+                boff[bk + 1] = boff[bk] + rows * 64;
+                keys.resize((size_t)boff[bk + 1] * 4, -1);
+                c.resize((size_t)boff[bk + 1], 0.0);
+                for (int l = 0; l < 64 && bk * 64 + l < L; ++l) {
+                    const int f = order[bk * 64 + l];
+                    item[(size_t)bk * 64 + l] = f;
+                    for (int j = 0; j < cnt[f]; ++j) {
+                        const int e = fvo[f] + j;
+                        const size_t r = (size_t)boff[bk] + (size_t)j * 64 + l;
+                        int nv = 0;
+                        for (int k = 0; k < 4; ++k) {
+                            keys[r * 4 + k] = fvk[(size_t)e * 4 + k];
+                            nv += fvk[(size_t)e * 4 + k] >= 0;
+                        }
+                        v_maxf = std::max(v_maxf, nv);
+                        c[r] = fvc[e];
+                    }
+                }
+            }
+            std::vector<float> cf(c.begin(), c.end());
+            D.fq_nblk = nblk;
+            D.fq_boff = up(*this, boff);
+            D.fq_item = up(*this, item);
+            D.fq_keys = up(*this, keys);
+            D.fq_c = up(*this, c);
+            D.fq_cf = up(*this, cf);
+        }
         D.lin_global = up(*this, T.linear_global);
         host_wz_off.push_back(wzo);
         host_wz_ylm.push_back(wzy);
@@ -615,7 +695,8 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     {
         ProfScope ps("wprime", st);
         launch_wprime(st, fp64, m.md, bd, work.a.ptr, a_stride, a0, na, work.w.ptr, w_stride,
-                      work.x.ptr, x_stride, m.max_nlm, gd, gd ? work.wg.ptr : nullptr);
+                      work.x.ptr, x_stride, m.max_nlm, m.w_maxf, m.v_maxf, gd,
+                      gd ? work.wg.ptr : nullptr);
     }
     check_hip(hipGetLastError(), "wprime launch");
     if (gd) {

@@ -86,6 +86,7 @@ struct ModelHandle {
     std::vector<std::vector<int>> host_wz_ylm;  // Y key of each derivative-weight nonzero
     std::vector<std::vector<int>> host_wz_nid;  // radial index of each nonzero
     int wz_max_type = 0, lin_max_type = 0;      // nonzeros / linear features, max over types
+    int w_maxf = 0, v_maxf = 0;  // factors per W' term / per value term, max over types
     std::map<int, std::unique_ptr<ColumnTables>> col_cache;
     // evaluation tables
     Store pot_store;

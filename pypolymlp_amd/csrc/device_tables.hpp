@@ -38,6 +38,23 @@ struct TypeDev {
     const double* wt_coeff;    // folded coefficient (cc applied for conjugate heads)
     const int* wt_keys;        // [nterm*3], -1 padded: product of all but the head
     const unsigned char* wt_conj;  // product enters conjugated
+    // The same terms laid out for the W' / value kernel: items (W' nonzeros,
+    // features) in order of decreasing term count, in blocks of 64 (one per
+    // lane); term j of the block's lane l at row j, so a wave's loads of one
+    // row are contiguous. Rows past an item's count are padding (keys -1,
+    // coefficient 0). Keys [row*64 + l][4], -1 padded; W' coefficients as
+    // (c, +-c) for (Re, Im) with the conjugation folded in.
+    int wq_nblk, fq_nblk;
+    const int* wq_boff;        // [wq_nblk+1] first term row of each block, times 64
+    const int* wq_item;        // [wq_nblk*64] nonzero of each lane, -1 past the end
+    const int* wq_keys;
+    const double* wq_cc;
+    const float* wq_ccf;
+    const int* fq_boff;
+    const int* fq_item;        // feature of each lane
+    const int* fq_keys;
+    const double* fq_c;
+    const float* fq_cf;
     const int* lin_global;     // [n_lin] global linear feature id
     // eval: potential terms per noconj head
     const int* pot_off;        // [n_noconj+1]

@@ -83,8 +83,8 @@ void launch_anlm_rec(hipStream_t st, bool fp64, const ModelDev& md, const BatchD
 // (wg[(i - atom0) * gd.wg_stride + position], holes zero).
 void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,
                    const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
-                   void* x, int x_stride, int max_nlm, const GroupDev* gd = nullptr,
-                   void* wg = nullptr);
+                   void* x, int x_stride, int max_nlm, int w_maxf, int v_maxf,
+                   const GroupDev* gd = nullptr, void* wg = nullptr);
 
 // Linear columns by signature groups (kernels_xgroup.hip).
 struct XGroupParams {

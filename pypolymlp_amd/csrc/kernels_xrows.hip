@@ -203,18 +203,27 @@ __global__ void __launch_bounds__(256) anlm_rec_kernel(ModelDev md, BatchDev bd,
 //   x_f = sum c * Re prod, dropping |Re prod| <= 1e-20 where the reference
 //   does (polymlp_features.cpp:184-193).
 // ---------------------------------------------------------------------------
-template <class T>
-__device__ __forceinline__ Cpx<T> prod_terms(const Cpx<T>* a, const int* keys, int nk) {
-    Cpx<T> v = {T(1), T(0)};
-    for (int k = 0; k < nk; ++k) {
-        const int id = keys[k];
-        if (id < 0) break;
-        v = cmul(v, a[id]);
-    }
+template <class T> __device__ __forceinline__ const T* wq_coeff(const TypeDev& TT);
+template <> __device__ __forceinline__ const double* wq_coeff<double>(const TypeDev& TT) { return TT.wq_cc; }
+template <> __device__ __forceinline__ const float* wq_coeff<float>(const TypeDev& TT) { return TT.wq_ccf; }
+template <class T> __device__ __forceinline__ const T* fq_coeff(const TypeDev& TT);
+template <> __device__ __forceinline__ const double* fq_coeff<double>(const TypeDev& TT) { return TT.fq_c; }
+template <> __device__ __forceinline__ const float* fq_coeff<float>(const TypeDev& TT) { return TT.fq_cf; }
+
+// Products of NK order-parameter components (keys -1 padded). a[one] holds
+// 1 + 0i, so padded keys multiply by one and the loop has no branch; the
+// factors are taken in key order, as the reference multiplies them.
+template <class T, int NK>
+__device__ __forceinline__ Cpx<T> prod_keys(const Cpx<T>* a, int4 k, int one) {
+    Cpx<T> v = a[k.x < 0 ? one : k.x];
+    if (NK > 1) v = cmul(v, a[k.y < 0 ? one : k.y]);
+    if (NK > 2) v = cmul(v, a[k.z < 0 ? one : k.z]);
+    if (NK > 3) v = cmul(v, a[k.w < 0 ? one : k.w]);
     return v;
 }
 
-template <class T>
+// NF: factors per W' term, NV: per value term (maxima over the model's types)
+template <class T, int NF, int NV>
 __global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
                                                      const Cpx<T>* __restrict__ a_full,
                                                      int a_stride, int atom0, int n_atoms,
@@ -229,34 +238,51 @@ __global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
     const int ti = bd.types[i];
     const TypeDev& TT = md.t[ti];
     const Cpx<T>* ag = a_full + (size_t)ia * a_stride;
+    const int one = TT.n_nlm;
     for (int k = threadIdx.x; k < TT.n_nlm; k += blockDim.x) a[k] = ag[k];
+    if (threadIdx.x == 0) a[one] = {T(1), T(0)};
     __syncthreads();
-    const int nzt = TT.wz_off[TT.n_lin * md.n_tp];
     Cpx<T>* wo = w + (size_t)ia * w_stride;
-    for (int g = threadIdx.x; g < nzt; g += blockDim.x) {
-        Cpx<T> v = {T(0), T(0)};
-        for (int e = TT.wz_term_off[g]; e < TT.wz_term_off[g + 1]; ++e) {
-            Cpx<T> pr = prod_terms<T>(a, TT.wt_keys + 3 * (size_t)e, 3);
-            if (TT.wt_conj[e]) pr.im = -pr.im;
-            const T cf = T(TT.wt_coeff[e]);
-            v.re += cf * pr.re;
-            v.im += cf * pr.im;
+    const int lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+    // W': one block of 64 nonzeros per wave at a time, lane l takes term row
+    // j of its nonzero (rows are contiguous across the lanes)
+    {
+        const T* cc = wq_coeff<T>(TT);
+        const int4* k4 = reinterpret_cast<const int4*>(TT.wq_keys);
+        for (int bk = threadIdx.x >> 6; bk < TT.wq_nblk; bk += nwave) {
+            const int g = TT.wq_item[bk * 64 + lane];
+            T vr = T(0), vi = T(0);
+            for (int r = TT.wq_boff[bk] + lane; r < TT.wq_boff[bk + 1]; r += 64) {
+                const Cpx<T> pr = prod_keys<T, NF>(a, k4[r], one);
+                vr += cc[2 * r] * pr.re;
+                vi += cc[2 * r + 1] * pr.im;
+            }
+            if (g >= 0) {
+                const Cpx<T> v = {vr, vi};
+                wo[g] = v;
+                if (wg) wg[(size_t)ia * gd.wg_stride + gd.wg_pos[gd.wg_pos_off[ti] + g]] = v;
+            }
         }
-        wo[g] = v;
-        if (wg) wg[(size_t)ia * gd.wg_stride + gd.wg_pos[gd.wg_pos_off[ti] + g]] = v;
     }
     if (wg)
         for (int e = gd.wg_hole_off[ti] + threadIdx.x; e < gd.wg_hole_off[ti + 1]; e += blockDim.x)
             wg[(size_t)ia * gd.wg_stride + gd.wg_hole[e]] = Cpx<T>{T(0), T(0)};
+    // feature values, same layout; products with |Re| <= 1e-20 dropped where
+    // the reference drops them
     T* xo = x + (size_t)ia * x_stride;
-    for (int f = threadIdx.x; f < TT.n_lin; f += blockDim.x) {
-        T v = T(0);
-        for (int e = TT.fv_off[f]; e < TT.fv_off[f + 1]; ++e) {
-            const T pr = prod_terms<T>(a, TT.fv_keys + 4 * (size_t)e, 4).re;
-            if (TT.fv_skip && fabs(pr) <= T(1e-20)) continue;
-            v += T(TT.fv_coeff[e]) * pr;
+    {
+        const T* c = fq_coeff<T>(TT);
+        const int4* k4 = reinterpret_cast<const int4*>(TT.fq_keys);
+        for (int bk = threadIdx.x >> 6; bk < TT.fq_nblk; bk += nwave) {
+            const int f = TT.fq_item[bk * 64 + lane];
+            T v = T(0);
+            for (int r = TT.fq_boff[bk] + lane; r < TT.fq_boff[bk + 1]; r += 64) {
+                const T pr = prod_keys<T, NV>(a, k4[r], one).re;
+                if (TT.fv_skip && fabs(pr) <= T(1e-20)) continue;
+                v += c[r] * pr;
+            }
+            if (f >= 0) xo[f] = v;
         }
-        xo[f] = v;
     }
 }
 
@@ -604,27 +630,45 @@ int xrows_stage_limit(bool fp64) {
 
 int xrows_max_waves(bool fp64) { return fp64 ? 8 : 16; }
 
+
+template <class T, int NF, int NV>
+static void launch_wprime_t(hipStream_t st, const ModelDev& md, const BatchDev& bd,
+                            const void* a_full, int a_stride, int atom0, int n_atoms, void* w,
+                            int w_stride, void* x, int x_stride, int max_nlm, const GroupDev& g,
+                            void* wg) {
+    const size_t sm = (size_t)(max_nlm + 1) * 2 * sizeof(T);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&wprime_kernel<T, NF, NV>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    wprime_kernel<T, NF, NV><<<n_atoms, 256, sm, st>>>(
+        md, bd, reinterpret_cast<const Cpx<T>*>(a_full), a_stride, atom0, n_atoms,
+        reinterpret_cast<Cpx<T>*>(w), w_stride, reinterpret_cast<T*>(x), x_stride, g,
+        reinterpret_cast<Cpx<T>*>(wg));
+}
+
 void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,
                    const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
-                   void* x, int x_stride, int max_nlm, const GroupDev* gd, void* wg) {
-    const size_t sm = (size_t)max_nlm * 2 * (fp64 ? 8 : 4);
+                   void* x, int x_stride, int max_nlm, int w_maxf, int v_maxf, const GroupDev* gd,
+                   void* wg) {
     const GroupDev g = gd ? *gd : GroupDev{};
     if (!gd) wg = nullptr;
+    if (w_maxf > 3 || v_maxf > 4) throw std::invalid_argument("invariant order above 4");
+    // instantiations: W' terms with <= 2 or 3 factors, value terms with <= 3 or 4
+    const int nf = w_maxf <= 2 ? 2 : 3, nv = v_maxf <= 3 ? 3 : 4;
+#define PLM_WP(T, NF, NV)                                                                       \
+    launch_wprime_t<T, NF, NV>(st, md, bd, a_full, a_stride, atom0, n_atoms, w, w_stride, x, \
+                               x_stride, max_nlm, g, wg)
     if (fp64) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&wprime_kernel<double>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        wprime_kernel<double><<<n_atoms, 256, sm, st>>>(
-            md, bd, reinterpret_cast<const Cpx<double>*>(a_full), a_stride, atom0, n_atoms,
-            reinterpret_cast<Cpx<double>*>(w), w_stride, reinterpret_cast<double*>(x), x_stride, g,
-            reinterpret_cast<Cpx<double>*>(wg));
+        if (nf == 2 && nv == 3) PLM_WP(double, 2, 3);
+        else if (nf == 2) PLM_WP(double, 2, 4);
+        else if (nv == 3) PLM_WP(double, 3, 3);
+        else PLM_WP(double, 3, 4);
     } else {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&wprime_kernel<float>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        wprime_kernel<float><<<n_atoms, 256, sm, st>>>(
-            md, bd, reinterpret_cast<const Cpx<float>*>(a_full), a_stride, atom0, n_atoms,
-            reinterpret_cast<Cpx<float>*>(w), w_stride, reinterpret_cast<float*>(x), x_stride, g,
-            reinterpret_cast<Cpx<float>*>(wg));
+        if (nf == 2 && nv == 3) PLM_WP(float, 2, 3);
+        else if (nf == 2) PLM_WP(float, 2, 4);
+        else if (nv == 3) PLM_WP(float, 3, 3);
+        else PLM_WP(float, 3, 4);
     }
+#undef PLM_WP
 }
 
 template <class T>
