@@ -222,7 +222,18 @@ def main():
     xrows["frac"] = xrows["achieved"] / HBM_PEAK_GBS
     by_name = {"gram": gram_i8, "xrows": xrows}
     roof = dict(by_name.get(dom, gram_i8))
+    # HBM bytes per launch of the dominant kernel: rocprofv3 FETCH_SIZE and
+    # WRITE_SIZE of this workload (scaled by a 1 GiB copy), committed with the
+    # profiles; the counters cannot be read from inside this process
     roof["traffic"] = None
+    tpath = os.path.join(ROOT, "profiles", "fit_binary_alloy_traffic.json")
+    if os.path.exists(tpath) and len(sts) == 1000 and roof["kernel"] == "gram_i8_kernel":
+        with open(tpath) as f:
+            tr = json.load(f)
+        k = tr.get(roof["kernel"])
+        if k:
+            roof["traffic"] = k["fetch_bytes_per_launch"] + k["write_bytes_per_launch"]
+            roof["traffic_source"] = "profiles/fit_binary_alloy_traffic.json (rocprofv3 --pmc)"
 
     ev = eval_leg(args.eval_steps) if args.eval_steps > 0 else {}
 
