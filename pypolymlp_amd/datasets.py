@@ -33,6 +33,8 @@ class Dataset:
     weight: float = 1.0
     include_force: bool = True
     include_stress: bool = True
+    exist_force: bool = True   # the source held forces (DatasetDFT.exist_force)
+    exist_stress: bool = True  # the source held stresses; zeros otherwise
 
     @property
     def total_n_atoms(self) -> np.ndarray:
@@ -47,6 +49,30 @@ class Dataset:
     def stresses_flat(self) -> np.ndarray:
         return self.stresses.reshape(-1)
 
+    def subset(self, ids, name=None) -> "Dataset":
+        """Structures ``ids`` (an index sequence or a slice) with their targets."""
+        ids = np.arange(len(self.structures))[ids] if isinstance(ids, slice) else np.asarray(ids)
+        return Dataset(structures=[self.structures[i] for i in ids], energies=self.energies[ids],
+                       forces=[self.forces[i] for i in ids], stresses=self.stresses[ids],
+                       name=self.name if name is None else name, weight=self.weight,
+                       include_force=self.include_force, include_stress=self.include_stress,
+                       exist_force=self.exist_force, exist_stress=self.exist_stress)
+
+    def split(self, train_ratio: float = 0.9):
+        """(train, test) by split_ids_train_test (DatasetDFT.split,
+        src/pypolymlp/core/dataset_utils.py:351-387; names as Dataset.split_dft)."""
+        tr, te = split_ids_train_test(len(self.energies), train_ratio=train_ratio)
+        train = self.subset(tr, "Train_" + self.name) if len(tr) else None
+        test = self.subset(te, "Test_" + self.name) if len(te) else None
+        return train, test
+
+    def apply_atomic_energy(self, atomic_energy):
+        """Subtract sum_t n_t e_t from each energy (DatasetDFT.apply_atomic_energy)."""
+        atomic_energy = np.asarray(atomic_energy, dtype=float)
+        self.energies = np.array([e - float(np.dot(s.n_atoms, atomic_energy[:len(s.n_atoms)]))
+                                  for e, s in zip(self.energies, self.structures)])
+        return self
+
     def sort_by_atoms(self):
         """Stable sort by atom count (Dataset.sort_dft, dataset_utils.py:337-349)."""
         order = np.argsort([s.n_total for s in self.structures], kind="stable")
@@ -55,6 +81,17 @@ class Dataset:
         self.forces = [self.forces[i] for i in order]
         self.stresses = self.stresses[order]
         return self
+
+
+def split_ids_train_test(n_data: int, train_ratio: float = 0.9):
+    """Test ids evenly spread over [0, n_data) by rounding a linspace, the rest
+    train (src/pypolymlp/core/utils.py:18-27)."""
+    n_train = round(n_data * train_ratio)
+    n_test = n_data - n_train
+    test_ids = np.round(np.linspace(0, n_data - 1, num=n_test)).astype(int)
+    train = np.ones(n_data, dtype=bool)
+    train[test_ids] = False
+    return np.where(train)[0], test_ids
 
 
 @dataclass

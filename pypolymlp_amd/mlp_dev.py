@@ -126,22 +126,31 @@ def calc_xtx_xty(model: capi.Model, datasets: Sequence[Dataset], precision=capi.
     acc = capi.Normal(model)
     for ds in datasets:
         ds.sort_by_atoms()
-        n = len(ds.structures)
-        for b0 in range(0, n, batch_size):
-            b1 = min(n, b0 + batch_size)
-            sub = Dataset(structures=ds.structures[b0:b1], energies=ds.energies[b0:b1],
-                          forces=ds.forces[b0:b1], stresses=ds.stresses[b0:b1], name=ds.name,
-                          weight=ds.weight, include_force=ds.include_force,
-                          include_stress=ds.include_stress)
-            w, yw = row_weights(sub, min_energy, weight_stress)
-            sts = sub.structures
-            batch = capi.Batch(model, [s.axis for s in sts], [s.positions_cartesian for s in sts],
-                               [s.types for s in sts])
-            acc.accumulate(batch, w, yw, precision)
-    xtx, xty, xe_sum, xe_sq, y_sq, n_rows = acc.result()
+        _accumulate(model, acc, ds, 0, len(ds.structures), min_energy, weight_stress,
+                    batch_size, precision)
     n_data = sum(len(d.energies) for d in datasets)
-    scales, zero = compute_scales(scales, xe_sum, xe_sq, n_data,
-                                  include_force=datasets[0].include_force,
+    return _finish_normal(acc.result(), n_data, scales, datasets[0].include_force,
+                          scale_threshold, min_energy)
+
+
+def _accumulate(model, acc, ds: Dataset, i0: int, i1: int, min_energy, weight_stress,
+                batch_size, precision):
+    """Add structures [i0, i1) of ds to the device accumulator in batches."""
+    for b0 in range(i0, i1, batch_size):
+        sub = ds.subset(slice(b0, min(i1, b0 + batch_size)))
+        w, yw = row_weights(sub, min_energy, weight_stress)
+        sts = sub.structures
+        batch = capi.Batch(model, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                           [s.types for s in sts])
+        acc.accumulate(batch, w, yw, precision)
+
+
+def _finish_normal(res, n_data, scales, include_force, scale_threshold, min_energy,
+                   ) -> NormalEquations:
+    """Scale the raw sums: zero-scale columns dropped, xtx / (s s^T), xty / s."""
+    xtx, xty, xe_sum, xe_sq, y_sq, n_rows = res
+    scales, zero = compute_scales(None if scales is None else np.array(scales, dtype=float),
+                                  xe_sum, xe_sq, n_data, include_force=include_force,
                                   threshold=scale_threshold)
     xtx[zero] = 0.0
     xtx[:, zero] = 0.0
@@ -151,6 +160,95 @@ def calc_xtx_xty(model: capi.Model, datasets: Sequence[Dataset], precision=capi.
     xty /= scales
     return NormalEquations(xtx=xtx, xty=xty, y_sq_norm=y_sq, total_n_data=n_rows, scales=scales,
                            min_energy=min_energy, xe_sum=xe_sum, xe_sq_sum=xe_sq)
+
+
+# --- the standard path: X in memory (features.py, data_standard.py) ----------
+@dataclass
+class Features:
+    x: np.ndarray
+    first_indices: list  # (ebegin, fbegin, sbegin) per dataset, -1 without forces
+    n_data: tuple        # (n energy, n force, n stress) rows
+
+
+def compute_features(model: capi.Model, datasets: Sequence[Dataset],
+                     precision=capi.PLM_FP64) -> Features:
+    """X of all datasets in the reference layout (Features, features.py:94-133;
+    PyModel::set_index, src/cxx/src/compute/py_model.cpp:58-104): energy rows
+    of every structure, then 6 stress rows and then 3N force rows of the
+    structures of datasets with forces. X is built on the GPU over one batch
+    of all structures; rows of datasets without forces are dropped."""
+    sts = [s for d in datasets for s in d.structures]
+    batch = capi.Batch(model, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                       [s.types for s in sts])
+    X = capi.features_x(model, batch, precision)
+    n_st = len(sts)
+    n_at = np.array([s.n_total for s in sts])
+    f_off = n_st * 7 + np.concatenate([[0], np.cumsum(3 * n_at)])
+    e_rows, s_rows, f_rows, first = [np.arange(n_st)], [], [], []
+    i0 = 0
+    ns = n_st
+    nf = n_st + 6 * sum(len(d.structures) for d in datasets if d.include_force)
+    for d in datasets:
+        n = len(d.structures)
+        if d.include_force:
+            first.append((i0, nf, ns))
+            s_rows.append(n_st + 6 * i0 + np.arange(6 * n))
+            f_rows.append(np.arange(f_off[i0], f_off[i0 + n]))
+            ns += 6 * n
+            nf += 3 * int(n_at[i0:i0 + n].sum())
+        else:
+            first.append((i0, -1, -1))
+        i0 += n
+    rows = np.concatenate(e_rows + s_rows + f_rows)
+    n_e, n_s = n_st, sum(len(r) for r in s_rows)
+    return Features(x=X[rows], first_indices=first,
+                    n_data=(n_e, len(rows) - n_e - n_s, n_s))
+
+
+@dataclass
+class DataXY:
+    x: np.ndarray
+    y: np.ndarray
+    weights: np.ndarray
+    scales: np.ndarray
+    min_energy: float
+    first_indices: list
+    n_structures: int
+
+
+def calc_xy(model: capi.Model, datasets: Sequence[Dataset], scales=None, min_energy=None,
+            weight_stress: float = 0.1, scale_threshold: float = 1e-10,
+            precision=capi.PLM_FP64) -> DataXY:
+    """Weighted, scaled X and y in memory (calc_xy, data_standard.py:16-77):
+    scales = population std of the energy rows, columns with zero scale are
+    zeroed, rows weighted as apply_weights (utils_weights.py:50-98)."""
+    feat = compute_features(model, datasets, precision)
+    x = feat.x
+    ne, nf, _ = feat.n_data
+    if scales is None:
+        scales = np.std(x[:ne], axis=0)
+    scales, zero = round_scales(np.array(scales, dtype=float), include_force=nf > 0,
+                                threshold=scale_threshold)
+    x[:, zero] = 0.0
+    x /= scales
+    if min_energy is None:
+        min_energy = get_min_energy(datasets)
+    y = np.zeros(x.shape[0])
+    w = np.ones(x.shape[0])
+    for d, (eb, fb, sb) in zip(datasets, feat.first_indices):
+        wd, yd = row_weights(d, min_energy, weight_stress)
+        n = len(d.energies)
+        rows = [np.arange(eb, eb + n)]
+        if d.include_force:
+            rows += [np.arange(sb, sb + 6 * n), np.arange(fb, fb + 3 * int(d.total_n_atoms.sum()))]
+        else:
+            wd, yd = wd[:n], yd[:n]
+        rows = np.concatenate(rows)
+        w[rows] = wd
+        y[rows] = yd
+        x[rows] *= wd[:, None]
+    return DataXY(x=x, y=y, weights=w, scales=scales, min_energy=float(min_energy),
+                  first_indices=feat.first_indices, n_structures=ne)
 
 
 def solver_ridge(xtx, xty, alphas):
@@ -194,3 +292,75 @@ def fit(model: capi.Model, train: List[Dataset], test: List[Dataset], alphas,
     k = int(np.argmin(rmse_test))
     return FitResult(coeffs=coefs[:, k], scales=tr.scales, alpha=float(alphas[k]),
                      rmse_train=float(rmse_train[k]), rmse_test=float(rmse_test[k]))
+
+
+# --- accuracy and learning curve (eval_accuracy.py, fit.py:115-179) ---------
+def _rmse(true, pred):
+    return float(np.sqrt(np.mean(np.square(true - pred))))
+
+
+def eval_accuracy(model: capi.Model, coeffs_unscaled, datasets: Sequence[Dataset],
+                  precision=capi.PLM_FP64) -> dict:
+    """RMSEs of the potential with ``coeffs_unscaled`` (coeffs / scales) over each
+    dataset: energy per atom (eV/atom), force (eV/A), stress per atom (eV/atom,
+    None without stress data), evaluated on the GPU
+    (PolymlpEvalAccuracy.compute_error_single, eval_accuracy.py:49-150)."""
+    model.set_coeffs(np.asarray(coeffs_unscaled, dtype=float))
+    out = {}
+    for ds in datasets:
+        sts = ds.structures
+        batch = capi.Batch(model, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                           [s.types for s in sts])
+        E, F, S = capi.eval_batch(model, batch, precision)
+        n = ds.total_n_atoms
+        err = {"energy": _rmse(ds.energies / n, E / n), "force": None, "stress": None}
+        if ds.exist_force:
+            err["force"] = _rmse(ds.forces_flat, np.concatenate([f.reshape(-1) for f in F]))
+        if ds.exist_stress:
+            nn = np.repeat(n, 6)
+            err["stress"] = _rmse(ds.stresses_flat / nn, S.reshape(-1) / nn)
+        out[ds.name] = err
+    return out
+
+
+def fit_learning_curve(model: capi.Model, train: List[Dataset], test: List[Dataset], alphas,
+                       precision=capi.PLM_FP64, weight_stress: float = 0.1,
+                       scale_threshold: float = 1e-10):
+    """Learning curve (fit_learning_curve, fit.py:115-179): for n = n_train/10,
+    2 n_train/10, ... training structures (in dataset order), ridge fits of the
+    first n structures with the scales and minimum energy of the whole training
+    set, the alpha chosen by test RMSE, and the test errors of that model.
+
+    One pass builds X once: the device accumulator's sums are read after each
+    prefix, so prefix k costs only its own n_train/10 structures.
+    Returns [[n, errors], ...], one entry per (n, test dataset)."""
+    if len(train) != 1:
+        raise RuntimeError("Number of training datasets must be one for learning curve.")
+    ds = train[0]
+    n_train = len(ds.structures)
+    step = n_train // 10
+    if step == 0:
+        raise ValueError("learning curve needs at least 10 training structures")
+    min_e = get_min_energy(train)
+    acc = capi.Normal(model)
+    prefixes, done = [], 0
+    for n in range(step, n_train + 1, step):
+        _accumulate(model, acc, ds, done, n, min_e, weight_stress, 4096, precision)
+        done = n
+        prefixes.append((n, acc.result()))
+    _accumulate(model, acc, ds, done, n_train, min_e, weight_stress, 4096, precision)
+    full = _finish_normal(acc.result(), n_train, None, ds.include_force, scale_threshold, min_e)
+    scales = full.scales
+    te = calc_xtx_xty(model, test, precision=precision, scales=scales.copy(), min_energy=min_e,
+                      weight_stress=weight_stress, scale_threshold=scale_threshold)
+    log = []
+    for n, res in prefixes:
+        ne = _finish_normal(res, n, scales, ds.include_force, scale_threshold, min_e)
+        coefs = solver_ridge(ne.xtx, ne.xty, alphas)
+        rmse_train = compute_rmse(coefs, ne)
+        rmse_test = compute_rmse(coefs, te)
+        k = int(np.argmin(rmse_test))
+        errors = eval_accuracy(model, coefs[:, k] / scales, test, precision=capi.PLM_FP64)
+        for v in errors.values():
+            log.append([n, v])
+    return log

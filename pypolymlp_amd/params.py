@@ -188,3 +188,75 @@ def load_mlp_yaml(filename: str):
         gtinv_version=int(yml.get("gtinv_version", 1)) if ft == "gtinv" else 1,
     )
     return spec, np.array(yml["coeffs"], dtype=float)
+
+
+# --- polymlp.in ------------------------------------------------------------
+@dataclass
+class PolymlpInput:
+    """Settings of a polymlp.in file that the fit path reads
+    (ParamsParser, src/pypolymlp/core/parser_polymlp_params.py:20-180)."""
+
+    spec: ModelSpec
+    alphas: np.ndarray
+    atomic_energy: tuple
+    include_force: bool = True
+    include_stress: bool = True
+    dataset_type: str = "vasp"
+    data: list = field(default_factory=list)  # [(tag, [strings])], tag in data/train_data/test_data
+
+
+def _tobool(s: str) -> bool:
+    if s.lower() in ("true", "t", "yes", "y", "1", ".true."):
+        return True
+    if s.lower() in ("false", "f", "no", "n", "0", ".false."):
+        return False
+    raise ValueError(f"not a boolean: {s}")
+
+
+def parse_polymlp_in(filename: str) -> PolymlpInput:
+    """Read a polymlp.in (one 'tag values...' per line, '#' comments) with the
+    reference's defaults: n_type 1, cutoff 6, model_type 1, max_p 1,
+    feature_type gtinv, gtinv_order 3, gtinv_maxl padded with 2 to order - 1
+    entries, gaussian_params1 (1, 1, 1), gaussian_params2 (0, cutoff - 1, 7),
+    reg_alpha_params (-3, 1, 5) giving alphas 10^linspace, atomic energies 0."""
+    tags, data = {}, []
+    for line in open(filename):
+        line = line.split("#")[0].split()
+        if not line:
+            continue
+        if line[0] in ("data", "train_data", "test_data", "data_md"):
+            data.append((line[0], line[1:]))
+        else:
+            tags[line[0]] = line[1:]
+    get = lambda k, d: tags.get(k, d)  # noqa: E731
+    n_type = int(get("n_type", ["1"])[0])
+    elements = get("elements", None)
+    cutoff = float(get("cutoff", ["6.0"])[0])
+    ftype = get("feature_type", ["gtinv"])[0]
+    order = int(get("gtinv_order", ["3"])[0])
+    maxl = [int(v) for v in get("gtinv_maxl", [])]
+    maxl += [2] * max(0, order - 1 - len(maxl))
+    p1 = [float(v) for v in get("gaussian_params1", ["1.0", "1.0", "1"])]
+    p2 = [float(v) for v in get("gaussian_params2", ["0.0", str(cutoff - 1.0), "7"])]
+    a = [float(v) for v in get("reg_alpha_params", ["-3", "1", "5"])]
+    atom_e = tuple(float(v) for v in get("atomic_energy", ["0.0"] * n_type))
+    include_force = _tobool(get("include_force", ["True"])[0])
+    include_stress = _tobool(get("include_stress", ["True"])[0]) if include_force else False
+    dataset_type = get("dataset_type", ["vasp"])[0]
+    # dataset types without stress data fit without stress rows
+    # (PolymlpParams._set_tags, src/pypolymlp/core/params.py:369-383)
+    if dataset_type in ("phono3py", "openmx"):
+        include_stress = False
+    elif dataset_type == "electron":
+        include_force = include_stress = False
+    spec = ModelSpec(n_type=n_type, cutoff=cutoff, feature_type=ftype,
+                     model_type=int(get("model_type", ["1"])[0]),
+                     max_p=int(get("max_p", ["1"])[0]), gtinv_order=order,
+                     gtinv_maxl=tuple(maxl[:order - 1]),
+                     gaussian_params1=(p1[0], p1[1], int(p1[2])),
+                     gaussian_params2=(p2[0], p2[1], int(p2[2])), elements=elements,
+                     gtinv_version=int(get("gtinv_version", ["1"])[0]))
+    alphas = np.array([10 ** x for x in np.linspace(a[0], a[1], int(a[2]))])
+    return PolymlpInput(spec=spec, alphas=alphas, atomic_energy=atom_e,
+                        include_force=include_force, include_stress=include_stress,
+                        dataset_type=dataset_type, data=data)

@@ -127,3 +127,81 @@ def test_product_path_fails_loudly_without_library(monkeypatch):
         mlp_dev.solver_ridge(np.eye(3), np.ones(3), np.array([0.1]))
     with pytest.raises(RuntimeError, match="not built"):
         capi.Model(datasets.config1_model().as_dict())
+
+
+# --- dataset ingestion: phono3py.yaml, train/test split, polymlp.in ---------
+def _si_dataset():
+    from pypolymlp_amd.interface_phono3py import parse_phono3py_yaml
+    from tests.helpers import ref_file
+    return parse_phono3py_yaml(ref_file("phonopy_training_dataset.yaml.xz"), elements=["Si"])
+
+
+def test_phono3py_yaml_reader_shapes():
+    """Shapes and the supercell of the reference's mp-149 fixture
+    (tests/test_core/test_interface_phono3py.py:19-42)."""
+    from pypolymlp_amd.interface_phono3py import Phono3pyYaml
+    from tests.helpers import ref_file
+    ph3 = Phono3pyYaml(ref_file("phonopy_training_dataset.yaml.xz"))
+    assert ph3.displacements.shape == (200, 3, 64)
+    assert ph3.forces.shape == (200, 3, 64)
+    assert len(ph3.supercells) == 200 and len(ph3.energies) == 200
+    assert ph3.supercell.axis[0, 0] == pytest.approx(10.86912324)
+    np.testing.assert_equal(ph3.phonon_dataset[0], ph3.displacements)
+    # displaced Cartesian positions = perfect positions + displacement
+    ax = ph3.supercell.axis
+    for k in (0, 77, 199):
+        st = ph3.supercells[k]
+        np.testing.assert_allclose(st.positions_cartesian,
+                                   ax @ ph3.supercell.positions + ph3.displacements[k], atol=1e-12)
+    ds = _si_dataset()
+    assert len(ds.energies) == 200
+    assert ds.forces_flat.shape[0] == 38400
+    assert not ds.exist_stress and np.all(ds.stresses == 0.0)
+    # the first snapshot's first displacement as printed in the file
+    np.testing.assert_allclose(ph3.displacements[0][:, 0],
+                               [-0.0150056047029767, -0.0255506665677614, -0.0046899110269901])
+
+
+def test_split_ids_train_test():
+    """10 % test ids evenly spread by a rounded linspace (src/pypolymlp/core/utils.py:18-27)."""
+    tr, te = datasets.split_ids_train_test(200, 0.9)
+    assert len(tr) == 180 and len(te) == 20
+    assert te[0] == 0 and te[-1] == 199
+    assert np.all(np.diff(te) >= 10) and np.all(np.diff(te) <= 11)
+    assert sorted(np.concatenate([tr, te]).tolist()) == list(range(200))
+    tr, te = datasets.split_ids_train_test(10, 0.9)
+    assert te.tolist() == [0] and tr.tolist() == list(range(1, 10))
+    tr, te = datasets.split_ids_train_test(5, 1.0)
+    assert len(te) == 0 and tr.tolist() == list(range(5))
+
+
+def test_polymlp_in_si_and_feature_count():
+    """polymlp.in.phono3py.Si gives the reference's 168-column Si model
+    (tests/test_mlp_dev/test_core_features.py:17-18)."""
+    from pypolymlp_amd.params import parse_polymlp_in
+    from tests.helpers import ref_file
+    inp = parse_polymlp_in(ref_file("polymlp.in.phono3py.Si"))
+    assert inp.dataset_type == "phono3py"
+    assert inp.data == [("data", ["phonopy_training_dataset.yaml.xz"])]
+    assert inp.spec.gtinv_maxl == (4, 4) and inp.spec.model_type == 3 and inp.spec.max_p == 2
+    np.testing.assert_allclose(inp.alphas, [1e-3, 1e-2, 1e-1, 1.0, 10.0])
+    assert len(inp.spec.pair_params) == 7
+    assert capi.Model(inp.spec.as_dict()).n_features == 168
+    spec_file, _ = load_model("polymlp.yaml.Si")
+    assert spec_file.pair_params == inp.spec.pair_params
+
+
+def test_load_datasets_phono3py_split():
+    """The single phono3py file is split 180 / 20 after parsing
+    (ParamsParser.set_datasets, parser_polymlp_params.py:229-280)."""
+    from pypolymlp_amd.dataset_io import load_datasets
+    from pypolymlp_amd.params import parse_polymlp_in
+    from tests.helpers import FILES, ref_file
+    inp = parse_polymlp_in(ref_file("polymlp.in.phono3py.Si"))
+    train, test = load_datasets(inp, prefix=FILES)
+    assert [d.name for d in train] == ["Train_Data_from_files"]
+    assert len(train[0].energies) == 180 and len(test[0].energies) == 20
+    full = _si_dataset()
+    _, te = datasets.split_ids_train_test(200, 0.9)
+    np.testing.assert_equal(test[0].energies, full.energies[te])
+    assert float(np.min(train[0].energies / 64)) == pytest.approx(-5.737324395625)

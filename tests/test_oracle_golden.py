@@ -101,3 +101,64 @@ def test_gtinv_tables_match_oracle_and_golden():
     assert c6[45][2] == pytest.approx(-0.18156825980064)
     for order, maxl in [(2, [4]), (3, [8, 4]), (4, [8, 4, 2])]:
         assert get_gtinv_attrs(order, maxl) == ref.gtinv(order, maxl)
+
+
+# --- Si mp-149 (phono3py dataset): oracle X and eval against the golden values
+@pytest.fixture(scope="module")
+def si_train():
+    from pypolymlp_amd.dataset_io import load_datasets
+    from pypolymlp_amd.params import parse_polymlp_in
+    from tests.helpers import FILES
+    inp = parse_polymlp_in(ref_file("polymlp.in.phono3py.Si"))
+    train, _ = load_datasets(inp, prefix=FILES)
+    return inp, train[0]
+
+
+def test_oracle_si_x_and_host_weights(si_train, tmp_path):
+    """The oracle's X of the 180 training snapshots has the reference's sums
+    (test_core_features.py:23-31); the host-side weights and scales applied to
+    it give the reference's calc_xy / calc_xtx_xty entries
+    (test_core_data.py:27-49)."""
+    from pypolymlp_amd import mlp_dev
+    from tests.helpers import write_model_yaml
+    inp, ds = si_train
+    rm = ref.RefModel(write_model_yaml(inp.spec, str(tmp_path / "si.yaml")))
+    sts = ds.structures
+    X = rm.x_batch([s.axis for s in sts], [s.positions_cartesian for s in sts],
+                   [s.types for s in sts])
+    assert X.shape == (35820, 168)
+    assert np.sum(X) == pytest.approx(5165294.450079148, rel=1e-6)
+    assert np.sum(X[:, 40:60]) == pytest.approx(93568.30539681061, rel=1e-6)
+    assert np.sum(X[:, -20:]) == pytest.approx(2008586.8132866116, rel=1e-6)
+    min_e = mlp_dev.get_min_energy([ds])
+    assert min_e == pytest.approx(-5.737324395625)
+    scales, zero = mlp_dev.round_scales(np.std(X[:180], axis=0))
+    w, yw = mlp_dev.row_weights(ds, min_e)
+    xw = X / scales * w[:, None]
+    assert xw[34500, 56] == pytest.approx(-1.6856145695486902, abs=1e-8)
+    assert yw[34500] == pytest.approx(0.11092374) and w[34500] == pytest.approx(1.0)
+    assert scales[56] == pytest.approx(0.0032488632685359524)
+    assert (xw.T @ yw)[56] == pytest.approx(6.899130774433e5, rel=1e-6)
+    # phono3py data fit without stress rows: ridge coefficients at alpha 1e2, 1e3
+    # (test_standard_solvers.py:22-28, rtol 1e-3); with stress rows weighted
+    # 0.1 the coefficients differ by more than 2x
+    assert not ds.include_stress
+    xtx = xw.T @ xw
+    c = np.stack([np.linalg.solve(xtx + a * np.eye(168), xw.T @ yw) for a in (1e2, 1e3)], axis=1)
+    np.testing.assert_allclose(c[20], [0.000854, 0.000642], rtol=1e-3)
+    np.testing.assert_allclose(c[30], [-0.000859, -0.000607], rtol=1e-3)
+
+
+def test_oracle_si_eval_accuracy(si_train):
+    """polymlp.yaml.Si on the training snapshots (test_core_eval_accuracy.py:12-19)."""
+    _, ds = si_train
+    rm = ref.RefModel(ref_file("polymlp.yaml.Si"), with_eval=True)
+    E, F = [], []
+    for s in ds.structures:
+        e, f, _ = rm.eval(s.axis, s.positions_cartesian, s.types)
+        E.append(e)
+        F.append(np.asarray(f).reshape(-1))
+    rmse_e = np.sqrt(np.mean(np.square((ds.energies - np.array(E)) / ds.total_n_atoms)))
+    rmse_f = np.sqrt(np.mean(np.square(ds.forces_flat - np.concatenate(F))))
+    assert rmse_e == pytest.approx(5.84708583252534e-06)
+    assert rmse_f == pytest.approx(0.0028245295894005276)
