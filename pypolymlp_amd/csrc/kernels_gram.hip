@@ -139,9 +139,13 @@ __global__ void quantize_kernel(const float* __restrict__ X, int ld, int rows_pa
 // ---------------------------------------------------------------------------
 // Mixed: 128 (i) x 64 (j) output tile per 512-thread workgroup; each of the 8
 // waves owns one 32x32 block with five int32 accumulators (levels s + t =
-// 0..4), so two waves run per SIMD. Stages of 64 rows: each thread's share of
-// a stage is loaded into registers one stage ahead and stored into one of two
-// LDS buffers (48-byte column stride, b128 reads without bank conflicts).
+// 0..4), so two waves run per SIMD. Stages of 64 rows through two LDS buffers
+// (48-byte column stride, b128 reads without bank conflicts). In each stage a
+// wave reads its operands, stores the next stage (loaded into registers one
+// stage earlier) into the other buffer and issues the loads of the stage after;
+// the LDS and global instructions are interleaved with the 18 MFMAs. Per
+// 4096-row chunk the int32 levels are combined into FP64 sums held in
+// registers; C is read and written once, at the end.
 // ---------------------------------------------------------------------------
 constexpr int GI_TI = 128;
 constexpr int GI_TJ = 64;
@@ -154,6 +158,7 @@ constexpr int GI_PA = 3 * GI_RB * GI_TI * 2;     // 16-byte A pieces per stage
 constexpr int GI_PB = 3 * GI_RB * GI_TJ * 2;     // 16-byte B pieces per stage
 constexpr int GI_LA = (GI_PA + GI_NT - 1) / GI_NT;  // A pieces per thread
 constexpr int GI_LB = (GI_PB + GI_NT - 1) / GI_NT;  // B pieces per thread
+constexpr int GI_MFMA = 9 * GI_RB;               // MFMAs per wave and stage
 
 // Tile order: workgroups are dealt to the 8 XCDs round robin; each XCD runs
 // one super-tile of 4 x 8 tiles (512 x 512 outputs) at a time, so the panels
@@ -194,6 +199,9 @@ __global__ void __launch_bounds__(GI_NT) gram_i8_kernel(const signed char* __res
     v16i acc[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) acc[k] = v16i{};
+    double sacc[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) sacc[v] = 0.0;
     // piece p -> ((s * GI_RB + rbo) * cols + col) * 2 + half
     auto g_off = [&](int p, int cols, int col0, int st) -> size_t {
         const int half = p & 1, col = (p >> 1) % cols, sr = (p >> 1) / cols;
@@ -204,68 +212,35 @@ __global__ void __launch_bounds__(GI_NT) gram_i8_kernel(const signed char* __res
         const int half = p & 1, col = (p >> 1) % cols, sr = (p >> 1) / cols;
         return (sr * cols + col) * GI_CS + 16 * half;
     };
+    // thread pieces past a panel's end wrap around (branch-free): such a
+    // thread loads and stores a copy of a piece another thread stores, the
+    // same value into the same slot
     v4i ra[GI_LA], rb[GI_LB];
     auto load = [&](int st) {
 #pragma unroll
-        for (int r = 0; r < GI_LA; ++r) {
-            const int p = tid + GI_NT * r;
-            if (p < GI_PA) ra[r] = *reinterpret_cast<const v4i*>(Q + g_off(p, GI_TI, I0, st));
-        }
+        for (int r = 0; r < GI_LA; ++r)
+            ra[r] = *reinterpret_cast<const v4i*>(Q + g_off((tid + GI_NT * r) % GI_PA, GI_TI, I0, st));
 #pragma unroll
-        for (int r = 0; r < GI_LB; ++r) {
-            const int p = tid + GI_NT * r;
-            if (p < GI_PB) rb[r] = *reinterpret_cast<const v4i*>(Q + g_off(p, GI_TJ, J0, st));
-        }
+        for (int r = 0; r < GI_LB; ++r)
+            rb[r] = *reinterpret_cast<const v4i*>(Q + g_off((tid + GI_NT * r) % GI_PB, GI_TJ, J0, st));
     };
     auto store = [&](int buf) {
         signed char* A = smem_i8 + (size_t)buf * (GI_A + GI_B);
         signed char* B = A + GI_A;
 #pragma unroll
-        for (int r = 0; r < GI_LA; ++r) {
-            const int p = tid + GI_NT * r;
-            if (p < GI_PA) *reinterpret_cast<v4i*>(A + l_off(p, GI_TI)) = ra[r];
-        }
+        for (int r = 0; r < GI_LA; ++r)
+            *reinterpret_cast<v4i*>(A + l_off((tid + GI_NT * r) % GI_PA, GI_TI)) = ra[r];
 #pragma unroll
-        for (int r = 0; r < GI_LB; ++r) {
-            const int p = tid + GI_NT * r;
-            if (p < GI_PB) *reinterpret_cast<v4i*>(B + l_off(p, GI_TJ)) = rb[r];
-        }
-    };
-    auto compute = [&](int buf) {
-        const signed char* A = smem_i8 + (size_t)buf * (GI_A + GI_B);
-        const signed char* B = A + GI_A;
-#pragma unroll 1
-        for (int rbo = 0; rbo < GI_RB; ++rbo) {
-            // lane -> column (lane % 32) of its block, k half (lane / 32)
-            v4i a[3], b[3];
-#pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                a[s] = *reinterpret_cast<const v4i*>(
-                    A + ((s * GI_RB + rbo) * GI_TI + wi + (lane & 31)) * GI_CS + 16 * (lane >> 5));
-                b[s] = *reinterpret_cast<const v4i*>(
-                    B + ((s * GI_RB + rbo) * GI_TJ + wj + (lane & 31)) * GI_CS + 16 * (lane >> 5));
-            }
-            // issue order: the k-th product of every level before any (k+1)-th
-            constexpr int kS[3][5] = {{0, 0, 0, 1, 2}, {-1, 1, 1, 2, -1}, {-1, -1, 2, -1, -1}};
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int lvl = 0; lvl < 5; ++lvl) {
-                    const int s = kS[k][lvl];
-                    if (s < 0) continue;
-                    acc[lvl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b[lvl - s], acc[lvl],
-                                                                     0, 0, 0);
-                }
-        }
+        for (int r = 0; r < GI_LB; ++r)
+            *reinterpret_cast<v4i*>(B + l_off((tid + GI_NT * r) % GI_PB, GI_TJ)) = rb[r];
     };
     // x~_i x~_j = 2^(E_i + E_j - 14) sum_k acc_k 2^(-7k): exact int32 sums
     // combined in FP64; D[(v%4) + 8(v/4) + 4(l/32)][l%32]
+    const int i0 = I0 + wi + 4 * (lane >> 5);
+    const int j = J0 + wj + (lane & 31);
     auto flush = [&](int chunk) {
         const int* Ec = E + (size_t)chunk * ld;
-        const int i0 = I0 + wi + 4 * (lane >> 5);
-        const int j = J0 + wj + (lane & 31);
         const int ej = Ec[j];
-        double* cb = C + (size_t)i0 * ldc + j;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int di = (v & 3) + 8 * (v >> 2);
@@ -274,28 +249,76 @@ __global__ void __launch_bounds__(GI_NT) gram_i8_kernel(const signed char* __res
             t = t * 0.0078125 + double(acc[2][v]);
             t = t * 0.0078125 + double(acc[1][v]);
             t = t * 0.0078125 + double(acc[0][v]);
-            cb[(size_t)di * ldc] += ldexp(t, Ec[i0 + di] + ej - 14);
+            sacc[v] += ldexp(t, Ec[i0 + di] + ej - 14);
         }
 #pragma unroll
         for (int k = 0; k < 5; ++k) acc[k] = v16i{};
     };
     const int st_per_chunk = kQChunk / (kQK * GI_RB);
-    // stage st + 1 is stored to LDS at the start of iteration st (its loads
-    // were issued one iteration earlier), then stage st + 2 is loaded into the
-    // freed registers while stage st is multiplied
+    const int nchunk = (nst + st_per_chunk - 1) / st_per_chunk;
     load(0);
     store(0);
-    if (nst > 1) load(1);
+    load(nst > 1 ? 1 : 0);
     __syncthreads();
-    for (int st = 0; st < nst; ++st) {
-        if (st + 1 < nst) {
+    for (int ch = 0; ch < nchunk; ++ch) {
+        const int s1 = min(nst, (ch + 1) * st_per_chunk);
+        for (int st = ch * st_per_chunk; st < s1; ++st) {
+            const signed char* A = smem_i8 + (size_t)(st & 1) * (GI_A + GI_B);
+            const signed char* B = A + GI_A;
+            // lane -> column (lane % 32) of its block, k half (lane / 32)
+            v4i a[GI_RB][3], b[GI_RB][3];
+#pragma unroll
+            for (int rbo = 0; rbo < GI_RB; ++rbo)
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    a[rbo][s] = *reinterpret_cast<const v4i*>(
+                        A + ((s * GI_RB + rbo) * GI_TI + wi + (lane & 31)) * GI_CS + 16 * (lane >> 5));
+                    b[rbo][s] = *reinterpret_cast<const v4i*>(
+                        B + ((s * GI_RB + rbo) * GI_TJ + wj + (lane & 31)) * GI_CS + 16 * (lane >> 5));
+                }
+            // stage st + 1 into the other buffer (last read in iteration st - 1),
+            // then the loads of stage st + 2 (past the end: the last stage again)
             store((st + 1) & 1);
-            if (st + 2 < nst) load(st + 2);
+            load(min(st + 2, nst - 1));
+            // issue order: the k-th product of every level before any (k+1)-th
+            constexpr int kS[3][5] = {{0, 0, 0, 1, 2}, {-1, 1, 1, 2, -1}, {-1, -1, 2, -1, -1}};
+#pragma unroll
+            for (int rbo = 0; rbo < GI_RB; ++rbo)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int lvl = 0; lvl < 5; ++lvl) {
+                        const int s = kS[k][lvl];
+                        if (s < 0) continue;
+                        acc[lvl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[rbo][s], b[rbo][lvl - s],
+                                                                         acc[lvl], 0, 0, 0);
+                    }
+            // schedule: the first row block's operands, then one MFMA per LDS
+            // read of the second row block, per LDS store, per global load
+            __builtin_amdgcn_sched_group_barrier(0x100, 3 * 2, 0);
+#pragma unroll
+            for (int i = 0; i < 3 * 2; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < GI_LA + GI_LB; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < GI_LA + GI_LB; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, GI_MFMA - 6 - 2 * (GI_LA + GI_LB), 0);
+            __syncthreads();
         }
-        compute(st & 1);
-        if ((st + 1) % st_per_chunk == 0 || st + 1 == nst) flush(st / st_per_chunk);
-        __syncthreads();
+        flush(ch);
     }
+    double* cb = C + (size_t)i0 * ldc + j;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) cb[(size_t)((v & 3) + 8 * (v >> 2)) * ldc] += sacc[v];
 }
 
 // y (FP64, one value per row) into column col of X (FP32 or FP64)
