@@ -168,6 +168,7 @@ class Features:
     x: np.ndarray
     first_indices: list  # (ebegin, fbegin, sbegin) per dataset, -1 without forces
     n_data: tuple        # (n energy, n force, n stress) rows
+    cumulative_n_features: Optional[list] = None  # hybrid models: end column of each sub-model
 
 
 def compute_features(model: capi.Model, datasets: Sequence[Dataset],
@@ -214,6 +215,7 @@ class DataXY:
     min_energy: float
     first_indices: list
     n_structures: int
+    cumulative_n_features: Optional[list] = None
 
 
 def calc_xy(model: capi.Model, datasets: Sequence[Dataset], scales=None, min_energy=None,

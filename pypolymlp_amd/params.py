@@ -63,6 +63,10 @@ class ModelSpec:
     elements: Optional[Sequence[str]] = None
     gtinv_version: int = 1
     extra: dict = field(default_factory=dict)
+    # hybrid sub-models: the atom types (of the full system) this model acts
+    # on, or every atom when type_full (PolymlpParamsSingle.type_full / type_indices)
+    type_full: bool = True
+    type_indices: Optional[Sequence[int]] = None
 
     def __post_init__(self):
         if self.pair_params is None:
@@ -74,6 +78,8 @@ class ModelSpec:
         if self.feature_type == "pair":
             self.gtinv_order = 0
             self.gtinv_maxl = ()
+        if self.type_indices is None:
+            self.type_indices = list(range(self.n_type))
 
     @property
     def max_l(self) -> int:
@@ -116,6 +122,8 @@ class ModelSpec:
             "print_memory": False,
             "include_force": True,
             "include_stress": True,
+            "type_full": bool(self.type_full),
+            "type_indices": [int(t) for t in self.type_indices],
             "model": model,
         }
 
@@ -157,8 +165,8 @@ def save_mlp_yaml(spec: ModelSpec, coeffs, scales, filename: str):
         lines.append("- atom_type_pair:      " + str(list(k)))
         lines.append("  pair_params_indices: " + str(list(v)))
     lines.append("")
-    lines.append("type_full:    1")
-    lines.append("type_indices: " + str(list(range(spec.n_type))))
+    lines.append(f"type_full:    {int(bool(spec.type_full))}")
+    lines.append("type_indices: " + str([int(t) for t in spec.type_indices]))
     lines.append("")
     c = coeffs / scales
     lines.append(f"n_coeffs: {len(c)}")
@@ -186,6 +194,8 @@ def load_mlp_yaml(filename: str):
         pair_params_conditional=cond,
         elements=elements,
         gtinv_version=int(yml.get("gtinv_version", 1)) if ft == "gtinv" else 1,
+        type_full=bool(yml.get("type_full", 1)),
+        type_indices=list(yml.get("type_indices", range(n_type))),
     )
     return spec, np.array(yml["coeffs"], dtype=float)
 

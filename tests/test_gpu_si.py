@@ -178,3 +178,30 @@ def test_si_learning_curve(si):
     assert log[-2][1]["energy"] == pytest.approx(2.7852516328822455e-06, rel=1e-2)
     assert log[-1][1]["energy"] == pytest.approx(2.7604438411037103e-06, rel=1e-2)
     assert log[-1][1]["stress"] is None and log[-1][1]["force"] > 0.0
+
+
+def test_si_hybrid_features_x_sums(si):
+    """Hybrid of the Si model with itself, type_indices [0], type_full
+    (tests/test_mlp_dev/test_core_features.py:34-56, rel 1e-6)."""
+    import copy
+
+    from pypolymlp_amd import hybrid
+    inp, _, train, _ = si
+    spec = copy.deepcopy(inp.spec)
+    spec.type_indices, spec.type_full = [0], True
+    subs = [hybrid.SubModel.from_spec(spec), hybrid.SubModel.from_spec(spec)]
+    say("si hybrid features_x")
+    feat = hybrid.compute_features_hybrid(subs, train, capi.PLM_FP64)
+    x = feat.x
+    assert x.shape == (35820, 336)
+    assert feat.n_data == (180, 34560, 180 * 6)
+    assert tuple(feat.cumulative_n_features) == (168, 336)
+    assert tuple(feat.first_indices[0]) == (0, 1260, 180)
+    assert np.sum(x) == pytest.approx(10330588.900158295, rel=1e-6)
+    assert np.sum(x[:, :40]) == pytest.approx(16251.02531073887, rel=1e-6)
+    assert np.sum(x[:, 40:80]) == pytest.approx(401890.1325695197, rel=1e-6)
+    assert np.sum(x[:, 80:120]) == pytest.approx(717054.2979810806, rel=1e-6)
+    assert np.sum(x[:, 120:160]) == pytest.approx(2804300.094916472, rel=1e-6)
+    assert np.sum(x[:, 160:200]) == pytest.approx(1242982.1653808185, rel=1e-6)
+    assert np.sum(x[:, 200:240]) == pytest.approx(393663.7339174809, rel=1e-6)
+    assert np.sum(x[:, 240:]) == pytest.approx(4754447.450082185, rel=1e-6)

@@ -205,3 +205,25 @@ def test_load_datasets_phono3py_split():
     _, te = datasets.split_ids_train_test(200, 0.9)
     np.testing.assert_equal(test[0].energies, full.energies[te])
     assert float(np.min(train[0].energies / 64)) == pytest.approx(-5.737324395625)
+
+
+def test_hybrid_active_atoms_and_yaml_round_trip(tmp_path):
+    """Active atoms of a partial sub-model are the atoms of its types, renumbered
+    by their position in type_indices (py_hybrid_model.cpp:107-143); type_full
+    and type_indices survive polymlp.yaml."""
+    from pypolymlp_amd.hybrid import active_atoms
+    from pypolymlp_amd.params import ModelSpec, load_mlp_yaml
+    from pypolymlp_amd.structure import PolymlpStructure
+    st = PolymlpStructure(axis=np.eye(3) * 5, positions=np.zeros((3, 6)), n_atoms=[2, 3, 1],
+                          elements=list("ABBABC"), types=[0, 1, 1, 0, 1, 2])
+    spec = ModelSpec(n_type=2, cutoff=5.0, feature_type="pair", model_type=2, max_p=1,
+                     elements=["C", "A"], type_full=False, type_indices=[2, 0])
+    sel, t = active_atoms(st, spec)
+    assert sel.tolist() == [0, 3, 5] and t.tolist() == [1, 1, 0]
+    spec.type_full = True
+    sel, t = active_atoms(st, spec)
+    assert sel.tolist() == list(range(6)) and t.tolist() == st.types
+    spec.type_full = False
+    path = write_model_yaml(spec, str(tmp_path / "h.yaml"), np.ones(capi.Model(spec.as_dict()).n_features))
+    spec2, _ = load_mlp_yaml(path)
+    assert spec2.type_full is False and list(spec2.type_indices) == [2, 0]
