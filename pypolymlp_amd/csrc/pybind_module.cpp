@@ -6,7 +6,9 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <algorithm>
 #include <cmath>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -177,6 +179,123 @@ class PotentialModel {
     vector1i xf_begin_, xs_begin_, n_data_;
 };
 
+// PotentialHybridModel (py_hybrid_model.cpp:11-199): X of several models side
+// by side; a model with type_full false sees only the atoms whose type is in
+// its type_indices (renumbered by position there), and its force rows of the
+// other atoms are zero. Row layout as PotentialModel.
+class PotentialHybridModel {
+  public:
+    PotentialHybridModel(const std::vector<py::dict>& params_array, const vector3d& axis,
+                         const vector3d& positions_c, const vector2i& types,
+                         const vector1i& n_st_dataset, const std::vector<bool>& force_dataset,
+                         const vector1i& n_atoms_all) {
+        const int n_st = (int)axis.size();
+        if ((int)positions_c.size() != n_st || (int)types.size() != n_st)
+            throw std::invalid_argument("axis, positions and types differ in length");
+        const int n_datasets = (int)n_st_dataset.size();
+        std::vector<bool> force_st(n_st, false);
+        xs_begin_ = xf_begin_ = vector1i(n_datasets, -1);
+        n_data_ = vector1i(3, 0);
+        n_data_[0] = n_st;
+        std::vector<int> srow(n_st, -1), frow(n_st, -1);
+        int id = 0, istress = n_st;
+        for (int d = 0; d < n_datasets; ++d) {
+            if (force_dataset[d]) xs_begin_[d] = istress, n_data_[2] += n_st_dataset[d] * 6;
+            for (int j = 0; j < n_st_dataset[d]; ++j, ++id)
+                if (force_dataset[d]) srow[id] = istress, istress += 6;
+        }
+        id = 0;
+        int iforce = istress;
+        for (int d = 0; d < n_datasets; ++d) {
+            if (force_dataset[d]) xf_begin_[d] = iforce;
+            for (int j = 0; j < n_st_dataset[d]; ++j, ++id)
+                if (force_dataset[d]) {
+                    force_st[id] = true, frow[id] = iforce;
+                    iforce += 3 * n_atoms_all[id];
+                    n_data_[1] += 3 * n_atoms_all[id];
+                }
+        }
+        std::vector<std::unique_ptr<ModelOwner>> models;
+        int F = 0;
+        for (const auto& pd : params_array) {
+            models.push_back(std::make_unique<ModelOwner>(pd));
+            F += models.back()->n_features();
+            cum_.push_back(F);
+        }
+        const long total = (long)n_data_[0] + n_data_[1] + n_data_[2];
+        x_ = py::array_t<double, py::array::f_style>({(py::ssize_t)total, (py::ssize_t)F});
+        auto X = x_.mutable_unchecked<2>();
+        for (long r = 0; r < total; ++r)
+            for (int c = 0; c < F; ++c) X(r, c) = 0.0;
+        for (size_t n = 0; n < models.size(); ++n) {
+            const py::dict& pd = params_array[n];
+            const bool full = !pd.contains("type_full") || pd["type_full"].cast<bool>();
+            const vector1i idx = pd.contains("type_indices") ? pd["type_indices"].cast<vector1i>()
+                                                             : vector1i();
+            const int c0 = n ? cum_[n - 1] : 0;
+            const int Fn = models[n]->n_features();
+            // active parts of the structures that have active atoms
+            vector3d ax_a, pos_a;
+            vector2i ty_a;
+            std::vector<int> keep;
+            std::vector<vector1i> atoms;
+            for (int s = 0; s < n_st; ++s) {
+                vector1i sel, t;
+                for (int i = 0; i < (int)types[s].size(); ++i) {
+                    if (full) {
+                        sel.push_back(i), t.push_back(types[s][i]);
+                        continue;
+                    }
+                    auto it = std::find(idx.begin(), idx.end(), types[s][i]);
+                    if (it != idx.end()) sel.push_back(i), t.push_back((int)(it - idx.begin()));
+                }
+                if (sel.empty()) continue;
+                vector2d p(3, vector1d(sel.size()));
+                for (int a = 0; a < 3; ++a)
+                    for (size_t j = 0; j < sel.size(); ++j) p[a][j] = positions_c[s][a][sel[j]];
+                keep.push_back(s), atoms.push_back(sel);
+                ax_a.push_back(axis[s]), pos_a.push_back(p), ty_a.push_back(t);
+            }
+            if (keep.empty()) continue;
+            Packed pk = pack(ax_a, pos_a, ty_a);
+            const int m = (int)keep.size();
+            plm_batch* b = nullptr;
+            check(plm_batch_create(models[n]->m, m, pk.axes.data(), pk.pos.data(), pk.types.data(),
+                                   pk.n_atoms.data(), &b));
+            long rows = 0;
+            plm_batch_n_rows(b, &rows);
+            std::vector<double> xs((size_t)rows * Fn);
+            const int rc = plm_features_x(models[n]->m, b, PLM_FP64, xs.data());
+            plm_batch_destroy(b);
+            check(rc);
+            auto copy_row = [&](long dst, long src) {
+                for (int c = 0; c < Fn; ++c) X(dst, c0 + c) = xs[(size_t)src * Fn + c];
+            };
+            long f_src = 7L * m;
+            for (int q = 0; q < m; ++q) {
+                const int s = keep[q];
+                copy_row(s, q);
+                if (force_st[s]) {
+                    for (int k = 0; k < 6; ++k) copy_row(srow[s] + k, m + 6L * q + k);
+                    for (size_t j = 0; j < atoms[q].size(); ++j)
+                        for (int a = 0; a < 3; ++a)
+                            copy_row(frow[s] + 3L * atoms[q][j] + a, f_src + 3L * j + a);
+                }
+                f_src += 3L * (long)atoms[q].size();
+            }
+        }
+    }
+    py::array_t<double, py::array::f_style>& get_x() { return x_; }
+    const vector1i& get_fbegin() const { return xf_begin_; }
+    const vector1i& get_sbegin() const { return xs_begin_; }
+    const vector1i& get_cumulative_n_features() const { return cum_; }
+    const vector1i& get_n_data() const { return n_data_; }
+
+  private:
+    py::array_t<double, py::array::f_style> x_;
+    vector1i xf_begin_, xs_begin_, n_data_, cum_;
+};
+
 // PotentialPropertiesFast (py_properties_fast.cpp:10-85)
 class PotentialPropertiesFast {
   public:
@@ -345,6 +464,19 @@ PYBIND11_MODULE(libmlpcpp, m) {
         .def("get_fbegin", &PotentialModel::get_fbegin, py::return_value_policy::reference_internal)
         .def("get_sbegin", &PotentialModel::get_sbegin, py::return_value_policy::reference_internal)
         .def("get_n_data", &PotentialModel::get_n_data, py::return_value_policy::reference_internal);
+    py::class_<PotentialHybridModel>(m, "PotentialHybridModel")
+        .def(py::init<const std::vector<py::dict>&, const vector3d&, const vector3d&,
+                      const vector2i&, const vector1i&, const std::vector<bool>&,
+                      const vector1i&>())
+        .def("get_x", &PotentialHybridModel::get_x, py::return_value_policy::reference_internal)
+        .def("get_fbegin", &PotentialHybridModel::get_fbegin,
+             py::return_value_policy::reference_internal)
+        .def("get_sbegin", &PotentialHybridModel::get_sbegin,
+             py::return_value_policy::reference_internal)
+        .def("get_cumulative_n_features", &PotentialHybridModel::get_cumulative_n_features,
+             py::return_value_policy::reference_internal)
+        .def("get_n_data", &PotentialHybridModel::get_n_data,
+             py::return_value_policy::reference_internal);
     py::class_<PotentialPropertiesFast>(m, "PotentialPropertiesFast")
         .def(py::init<const py::dict&, const vector1d&>())
         .def("eval", &PotentialPropertiesFast::eval)

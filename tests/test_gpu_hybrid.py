@@ -123,3 +123,47 @@ def test_hybrid_eval_matches_oracle_sum(models):
         assert E[k] == pytest.approx(e, rel=1e-10)
         assert np.abs(F[k] - f).max() <= 1e-9 * max(1.0, np.abs(f).max())
         assert np.abs(S[k] - s).max() <= 1e-9 * max(1.0, np.abs(s).max())
+
+
+def test_pybind_mirror_matches_host_paths(models):
+    """The pybind11 mirror of libmlpcpp (the drop-in the reference's
+    features.py binds): PotentialModel and PotentialHybridModel give the X of
+    the host paths, with the reference's getters; PotentialPropertiesFast gives
+    the same energies as the batch evaluation."""
+    from pypolymlp_amd import libmlpcpp, mlp_dev
+    subs, _, _ = models
+    sts = _structures()
+    n = len(sts)
+    ds = Dataset(structures=sts, energies=np.zeros(n),
+                 forces=[np.zeros((3, s.n_total)) for s in sts], stresses=np.zeros((n, 6)))
+    axes = [s.axis.tolist() for s in sts]
+    pos = [s.positions_cartesian.tolist() for s in sts]
+    types = [list(map(int, s.types)) for s in sts]
+    nat = [s.n_total for s in sts]
+    say("pybind PotentialModel")
+    pm = libmlpcpp.PotentialModel(subs[0].spec.as_dict(), axes, pos, types, [n], [True], nat)
+    ref_x = mlp_dev.compute_features(subs[0].model, [ds], capi.PLM_FP64)
+    # the X kernels sum with atomics, so two runs agree to rounding, not bitwise
+    px = np.asarray(pm.get_x())
+    assert px.shape == ref_x.x.shape
+    assert np.linalg.norm(px - ref_x.x) <= 1e-12 * np.linalg.norm(ref_x.x)
+    assert list(pm.get_n_data()) == list(ref_x.n_data)
+    assert (pm.get_fbegin()[0], pm.get_sbegin()[0]) == ref_x.first_indices[0][1:]
+    say("pybind PotentialHybridModel")
+    hm = libmlpcpp.PotentialHybridModel([s.spec.as_dict() for s in subs], axes, pos, types,
+                                        [n], [True], nat)
+    hx = hybrid.compute_features_hybrid(subs, [ds], capi.PLM_FP64)
+    hxx = np.asarray(hm.get_x())
+    assert hxx.shape == hx.x.shape
+    assert np.linalg.norm(hxx - hx.x) <= 1e-12 * np.linalg.norm(hx.x)
+    assert list(hm.get_cumulative_n_features()) == hx.cumulative_n_features
+    assert list(hm.get_n_data()) == list(hx.n_data)
+    say("pybind PotentialPropertiesFast")
+    _, c1 = load_model("polymlp.yaml.gtinv.MgO")
+    pp = libmlpcpp.PotentialPropertiesFast(subs[0].spec.as_dict(), c1.tolist())
+    pp.eval_multiple(axes, pos, types)
+    E, _, _ = capi.eval_batch(subs[0].model,
+                              capi.Batch(subs[0].model, [s.axis for s in sts],
+                                         [s.positions_cartesian for s in sts], types),
+                              capi.PLM_FP64)
+    np.testing.assert_allclose(np.asarray(pp.get_e_array()), E, rtol=1e-12)

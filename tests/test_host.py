@@ -227,3 +227,21 @@ def test_hybrid_active_atoms_and_yaml_round_trip(tmp_path):
     path = write_model_yaml(spec, str(tmp_path / "h.yaml"), np.ones(capi.Model(spec.as_dict()).n_features))
     spec2, _ = load_mlp_yaml(path)
     assert spec2.type_full is False and list(spec2.type_indices) == [2, 0]
+
+
+def test_pybind_mirror_exposes_reference_classes():
+    """The pybind11 module loads without a GPU and has the reference's classes
+    and getters (src/pypolymlp/cxx/src/python/pybind11_mlp.cpp)."""
+    from pypolymlp_amd import libmlpcpp
+    for cls, methods in {
+        "PotentialModel": ("get_x", "get_fbegin", "get_sbegin", "get_n_data"),
+        "PotentialHybridModel": ("get_x", "get_fbegin", "get_sbegin",
+                                 "get_cumulative_n_features", "get_n_data"),
+        "PotentialPropertiesFast": ("eval", "eval_multiple", "get_e", "get_f", "get_s",
+                                    "get_e_array", "get_f_array", "get_s_array"),
+        "NeighborFull": ("get_distances", "get_differences", "get_neighbor_indices"),
+        "NeighborHalf": ("get_differences", "get_neighbor_indices"),
+    }.items():
+        c = getattr(libmlpcpp, cls)
+        for mth in methods:
+            assert hasattr(c, mth), (cls, mth)
