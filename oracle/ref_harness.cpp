@@ -31,6 +31,7 @@
 #include "compute/polymlp_eval.h"
 #include "polymlp/polymlp_api.h"
 #include "polymlp/polymlp_parse_potential.h"
+#include "polymlp/polymlp_parse_potential_legacy.h"
 #include "polymlp/polymlp_read_gtinv.h"
 
 namespace {
@@ -161,7 +162,11 @@ void* ref_model_load(const char* yaml, int with_eval) {
         vector1d pot2;
         std::vector<std::string> ele;
         vector1d mass;
-        parse_polymlp(yaml, m->fp, pot2, ele, mass);  // pot2 = 2 * coeffs
+        // both formats give pot2 = 2 * coeffs / scales (polymlp_api.cpp:20-32)
+        if (check_polymlp_legacy(yaml))
+            parse_polymlp_legacy(yaml, m->fp, pot2, ele, mass);
+        else
+            parse_polymlp(yaml, m->fp, pot2, ele, mass);
         m->coeffs.resize(pot2.size());
         for (size_t i = 0; i < pot2.size(); ++i) m->coeffs[i] = 0.5 * pot2[i];
         m->feat.set_features(m->fp);

@@ -42,10 +42,21 @@ class SubModel:
         return cls(spec=spec, model=m)
 
 
-def active_atoms(st: PolymlpStructure, spec: ModelSpec):
-    """(atom indices, renumbered types) of the atoms the sub-model acts on
-    (PyHybridModel::find_active_atoms, py_hybrid_model.cpp:107-143)."""
+def active_atoms(st: PolymlpStructure, spec: ModelSpec, by_elements: bool = False):
+    """(atom indices, renumbered types) of the atoms the sub-model acts on.
+
+    By type (training, PyHybridModel::find_active_atoms,
+    py_hybrid_model.cpp:107-143): atoms whose type is in type_indices, typed
+    by position there. By element (evaluation, find_active_atoms and
+    update_types, properties_utils.py:9-44, compute_features.py:15-40): atoms
+    whose element is one of the model's, typed by the model's element order."""
     types = np.asarray(st.types)
+    if by_elements and st.elements is not None and len(st.elements) == len(types):
+        ele = list(spec.elements)
+        if spec.type_full and not set(st.elements) <= set(ele):
+            return np.arange(len(types)), types
+        sel = np.array([i for i, e in enumerate(st.elements) if e in ele], dtype=int)
+        return sel, np.array([ele.index(st.elements[i]) for i in sel], dtype=int)
     if spec.type_full:
         return np.arange(len(types)), types
     idx = list(spec.type_indices)
@@ -53,11 +64,11 @@ def active_atoms(st: PolymlpStructure, spec: ModelSpec):
     return sel, np.array([idx.index(int(types[i])) for i in sel], dtype=int)
 
 
-def _sub_batch(sub: SubModel, structures: Sequence[PolymlpStructure]):
+def _sub_batch(sub: SubModel, structures: Sequence[PolymlpStructure], by_elements=False):
     """Batch of the active parts of the structures that have active atoms."""
     keep, atoms, axes, pos, types = [], [], [], [], []
     for k, st in enumerate(structures):
-        sel, t = active_atoms(st, sub.spec)
+        sel, t = active_atoms(st, sub.spec, by_elements)
         if len(sel) == 0:
             continue
         keep.append(k)
@@ -161,7 +172,8 @@ def split_coeffs(coeffs, subs: Sequence[SubModel]) -> List[np.ndarray]:
 def eval_hybrid(subs: Sequence[SubModel], structures: Sequence[PolymlpStructure],
                 precision=capi.PLM_FP64):
     """(energies, forces [(N, 3)], stresses (n, 6)): the sum of the sub-models'
-    evaluations, each over its active atoms, forces scattered back
+    evaluations, each over its active atoms (selected by element when the
+    structures carry element names), forces scattered back
     (PropertiesHybrid.eval_multiple with PropertiesSingle.eval_multiple).
     Each sub-model's coefficients must be set on its Model."""
     n = len(structures)
@@ -169,7 +181,7 @@ def eval_hybrid(subs: Sequence[SubModel], structures: Sequence[PolymlpStructure]
     F = [np.zeros((s.n_total, 3)) for s in structures]
     S = np.zeros((n, 6))
     for sub in subs:
-        keep, atoms, batch = _sub_batch(sub, structures)
+        keep, atoms, batch = _sub_batch(sub, structures, by_elements=True)
         if batch is None:
             continue
         e, f, s = capi.eval_batch(sub.model, batch, precision)

@@ -270,3 +270,79 @@ def parse_polymlp_in(filename: str) -> PolymlpInput:
     return PolymlpInput(spec=spec, alphas=alphas, atomic_energy=atom_e,
                         include_force=include_force, include_stress=include_stress,
                         dataset_type=dataset_type, data=data)
+
+
+# --- legacy polymlp.lammps -------------------------------------------------
+def _legacy_var(line: str, dtype=int, return_list: bool = False):
+    vals = line.split("#")[0].split()
+    if return_list:
+        return [dtype(v) for v in vals]
+    return dtype(vals[0])
+
+
+def load_mlp_lammps(filename: str):
+    """Read the legacy polymlp.lammps format (load_mlp_lammps,
+    src/pypolymlp/core/io_polymlp_legacy.py:23-161): one value or list per
+    line with '#' comments, in the order elements, cutoff, pair_type,
+    feature_type, model_type, max_p, max_l, [gtinv_order, gtinv_maxl,
+    gtinv_sym], n_coeffs, coeffs, scales, n_params, (beta, mu) per line, mass,
+    electrostatic, [gtinv_version], [n_type_pairs and (type pair, indices)
+    line pairs], [type_full, type_indices]. Returns (ModelSpec, coeffs / scales);
+    without a type-pair section every radial function serves every pair."""
+    lines = open(filename).read().splitlines()
+    i = 0
+
+    def nxt(dtype=int, return_list=False):
+        nonlocal i
+        v = _legacy_var(lines[i], dtype, return_list)
+        i += 1
+        return v
+
+    elements = nxt(str, True)
+    n_type = len(elements)
+    cutoff = nxt(float)
+    nxt(str)  # pair_type (gaussian)
+    ftype = nxt(str)
+    model_type = nxt()
+    max_p = nxt()
+    nxt()  # max_l (derived from gtinv_maxl)
+    order, maxl = 0, ()
+    if ftype == "gtinv":
+        order = nxt()
+        maxl = tuple(nxt(int, True))
+        nxt(_tobool, True)  # gtinv_sym
+    nxt()  # n_coeffs
+    coeffs = np.array(nxt(float, True))
+    scales = np.array(nxt(float, True))
+    n_pp = nxt()
+    pair_params = [nxt(float, True) for _ in range(n_pp)]
+    nxt(float, True)  # mass
+    nxt(_tobool)  # electrostatic
+    version = 1
+    if ftype == "gtinv" and i < len(lines) and "gtinv_version" in lines[i]:
+        version = nxt()
+    cond = None
+    if i < len(lines) and "n_type_pairs" in lines[i]:
+        cond = {}
+        for _ in range(nxt()):
+            tp = tuple(nxt(int, True)[0:2])
+            cond[tp] = nxt(int, True)
+    type_full, type_indices = True, list(range(n_type))
+    if i < len(lines) and "type_full" in lines[i]:
+        type_full = nxt(_tobool)
+        type_indices = nxt(int, True)
+    spec = ModelSpec(n_type=n_type, cutoff=cutoff, feature_type=ftype, model_type=model_type,
+                     max_p=max_p, gtinv_order=order, gtinv_maxl=maxl, pair_params=pair_params,
+                     pair_params_conditional=cond, elements=elements, gtinv_version=version,
+                     type_full=type_full, type_indices=type_indices)
+    return spec, coeffs / scales
+
+
+def load_mlp(filename: str):
+    """polymlp.yaml or legacy polymlp.lammps, told apart by the first line
+    (the yaml format starts with an 'elements:' key)."""
+    with open(filename) as f:
+        first = f.readline()
+    if first.lstrip().startswith("elements:"):
+        return load_mlp_yaml(filename)
+    return load_mlp_lammps(filename)

@@ -104,3 +104,55 @@ def test_large_cell_binned_lists_and_eval(tmp_path):
     assert e == pytest.approx(er, rel=1e-10)
     assert np.linalg.norm(f - fr) <= 1e-10 * np.linalg.norm(fr)
     assert np.linalg.norm(s - sr) <= 1e-10 * np.linalg.norm(sr)
+
+
+# --- legacy-format models, hybrid "flexible" models, skewed cells ------------
+from tests.test_oracle_golden import MGO_HNF, MGO_HNF_ENERGY, SRTIO3_LEGACY  # noqa: E402
+
+
+@pytest.mark.parametrize("name,e_true,s_true", SRTIO3_LEGACY)
+def test_eval_legacy_srtio3_golden(name, e_true, s_true):
+    """polymlp.lammps models read by the legacy reader
+    (tests/test_calc/test_properties_legacy_SrTiO3.py:19-69)."""
+    from pypolymlp_amd.params import load_mlp
+    spec, coeffs = load_mlp(ref_file(name))
+    m = capi.Model(spec.as_dict())
+    m.set_coeffs(coeffs)
+    st = poscar("POSCAR.perovskite.SrTiO3")
+    say(f"eval legacy model={name}")
+    e, f, s = capi.eval_structure(m, st.axis, st.positions_cartesian, st.types)
+    assert e == pytest.approx(e_true, abs=1e-12)
+    assert f[0][0] == pytest.approx(0.0, abs=1e-12)
+    np.testing.assert_allclose(s[:3] * volume_gpa_factor(st), [s_true] * 3, atol=1e-5)
+    np.testing.assert_allclose(s[3:] * volume_gpa_factor(st), 0.0, atol=1e-5)
+
+
+def test_eval_flexible_hybrid_srtio3_golden():
+    """Hybrid of a full model and an O-only model
+    (test_properties_legacy_SrTiO3.py:72-81, rel 1e-8)."""
+    from pypolymlp_amd import hybrid
+    from pypolymlp_amd.params import load_mlp
+    subs = [hybrid.SubModel.from_spec(*load_mlp(ref_file(n)))
+            for n in ("polymlp.yaml.flexible.1.SrTiO3", "polymlp.yaml.flexible.2.SrTiO3")]
+    assert not subs[1].spec.type_full
+    st = poscar("POSCAR.perovskite.SrTiO3")
+    say("eval flexible hybrid")
+    E, F, _ = hybrid.eval_hybrid(subs, [st], capi.PLM_FP64)
+    assert E[0] == pytest.approx(-31.642657299368572, rel=1e-8)
+    assert F[0][0][0] == pytest.approx(0.0, abs=1e-12)
+
+
+def test_eval_skewed_cells_mgo_golden():
+    """25 shears of the ideal rocksalt cell in one batch: every energy equals
+    the reference's (tests/test_calc/test_check_neighbors.py:38-73, atol 1e-12)."""
+    from pypolymlp_amd.structure import supercell
+    spec, coeffs = load_model("polymlp.yaml.pair.MgO")
+    m = capi.Model(spec.as_dict())
+    m.set_coeffs(coeffs)
+    st = poscar("POSCAR.RS.idealMgO")
+    sts = [supercell(st, h) for h in MGO_HNF]
+    b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                   [s.types for s in sts])
+    say(f"eval skewed cells n={len(sts)}")
+    E, _, _ = capi.eval_batch(m, b, capi.PLM_FP64)
+    np.testing.assert_allclose(E, MGO_HNF_ENERGY, atol=1e-12)

@@ -162,3 +162,75 @@ def test_oracle_si_eval_accuracy(si_train):
     rmse_f = np.sqrt(np.mean(np.square(ds.forces_flat - np.concatenate(F))))
     assert rmse_e == pytest.approx(5.84708583252534e-06)
     assert rmse_f == pytest.approx(0.0028245295894005276)
+
+
+# --- legacy polymlp.lammps models, hybrid "flexible" models, skewed cells ----
+SRTIO3_LEGACY = [
+    # (file, energy, stress GPa xx) tests/test_calc/test_properties_legacy_SrTiO3.py:19-69
+    ("polymlp.lammps.pair.SrTiO3", -31.63203426437243, -1.9204671),
+    ("polymlp.lammps.pair.cond.SrTiO3", -31.64306562035659, -1.74745126),
+    ("polymlp.lammps.gtinv.SrTiO3", -31.64286166673613, -0.02560392),
+    ("polymlp.lammps.gtinv.cond.SrTiO3", -31.642024569145274, 0.21674893),
+]
+MGO_HNF = [
+    # tests/test_calc/test_check_neighbors.py:38-73: shears of the rocksalt cell
+    [[1, 0, 0], [0, 1, 0], [0, 0, 1]], [[1, 0, 0], [1, 1, 0], [1, 1, 1]],
+    [[1, 0, 0], [1, 1, 0], [0, 0, 1]], [[1, 1, 0], [0, 1, 0], [0, 0, 1]],
+    [[1, 0, 0], [0, 1, 0], [1, 0, 1]], [[1, 0, 0], [0, 1, 0], [0, 1, 1]],
+    [[1, 0, 0], [-1, 1, 0], [0, 0, 1]], [[1, 0, 0], [-1, 1, 0], [-1, -1, 1]],
+    [[1, 0, 0], [-3, 1, 0], [3, 3, 1]], [[1, 0, 0], [3, 1, 0], [3, 3, 1]],
+    [[1, 0, 0], [-3, 1, 0], [-3, -3, 1]], [[1, 0, 0], [5, 1, 0], [5, 5, 1]],
+    [[1, 0, 0], [-5, 1, 0], [-5, -5, 1]], [[1, 0, 0], [10, 1, 0], [10, 10, 1]],
+    [[1, 0, 0], [-10, 1, 0], [-10, -10, 1]], [[1, 3, 3], [0, 1, 3], [0, 0, 1]],
+    [[1, 4, 4], [0, 1, 4], [0, 0, 1]], [[1, 5, 5], [0, 1, 5], [0, 0, 1]],
+    [[1, -5, -5], [0, 1, -5], [0, 0, 1]], [[1, 10, 10], [0, 1, 10], [0, 0, 1]],
+    [[1, -10, -10], [0, 1, -10], [0, 0, 1]], [[1, 0, 0], [5, 1, 0], [0, 0, 1]],
+    [[1, 0, 0], [-5, 1, 0], [0, 0, 1]], [[1, 0, 0], [10, 1, 0], [0, 0, 1]],
+    [[1, 0, 0], [-10, 1, 0], [0, 0, 1]],
+]
+MGO_HNF_ENERGY = -40.225125687168706
+
+
+@pytest.mark.parametrize("name,e_true,s_true", SRTIO3_LEGACY)
+def test_oracle_legacy_srtio3(name, e_true, s_true):
+    """The legacy-format models on the perovskite cell; our legacy reader
+    gives the same feature count as the reference's C++ legacy parser."""
+    from pypolymlp_amd import capi
+    from pypolymlp_amd.params import load_mlp
+    st = poscar("POSCAR.perovskite.SrTiO3")
+    r = ref.RefModel(ref_file(name), with_eval=True)
+    e, f, s = r.eval(st.axis, st.positions_cartesian, st.types)
+    assert e == pytest.approx(e_true, abs=1e-12)
+    assert np.asarray(f)[0][0] == pytest.approx(0.0, abs=1e-12)
+    np.testing.assert_allclose(np.asarray(s)[:3] * volume_gpa_factor(st), [s_true] * 3, atol=1e-5)
+    spec, coeffs = load_mlp(ref_file(name))
+    assert capi.Model(spec.as_dict()).n_features == r.n_features == len(coeffs)
+
+
+def test_oracle_flexible_hybrid_srtio3():
+    """Hybrid of a full SrTiO3 model and an O-only model
+    (test_properties_legacy_SrTiO3.py:72-81, rel 1e-8): the O-only model
+    acts on the O atoms of the cell."""
+    from pypolymlp_amd.params import load_mlp
+    st = poscar("POSCAR.perovskite.SrTiO3")
+    e_tot = 0.0
+    for name in ("polymlp.yaml.flexible.1.SrTiO3", "polymlp.yaml.flexible.2.SrTiO3"):
+        spec, _ = load_mlp(ref_file(name))
+        sel = [i for i, e in enumerate(st.elements) if e in spec.elements]
+        types = [list(spec.elements).index(st.elements[i]) for i in sel]
+        r = ref.RefModel(ref_file(name), with_eval=True)
+        e, _, _ = r.eval(st.axis, st.positions_cartesian[:, sel], types)
+        e_tot += e
+    assert e_tot == pytest.approx(-31.642657299368572, rel=1e-8)
+
+
+def test_oracle_skewed_cells_mgo():
+    """25 shears of the ideal rocksalt cell give the same energy (atol 1e-12);
+    pins the supercell construction used by the GPU test."""
+    from pypolymlp_amd.structure import supercell
+    st = poscar("POSCAR.RS.idealMgO")
+    r = ref.RefModel(ref_file("polymlp.yaml.pair.MgO"), with_eval=True)
+    for h in MGO_HNF:
+        s2 = supercell(st, h)
+        e, _, _ = r.eval(s2.axis, s2.positions_cartesian, s2.types)
+        assert e == pytest.approx(MGO_HNF_ENERGY, abs=1e-12), h
