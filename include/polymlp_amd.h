@@ -149,6 +149,14 @@ int plm_surrogate_predict(const float* params, const float* x, int n, float* y);
 int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
               double* coeffs, int* info);
 
+/* Conjugate gradient (solver_cg, src/pypolymlp/mlp_dev/gradient/solvers_cg.py:32-93)
+ * on (xtx + alpha I) coef = xty from coef0 (zeros when NULL), device resident:
+ * the loop stops at the first iteration i > 100 with |r| / sqrt(F) < gtol, or
+ * after max_iter iterations. coef: F; *n_iter (may be NULL) receives the
+ * iteration at which the loop stopped. */
+int plm_solver_cg(int F, const double* xtx, const double* xty, double alpha, const double* coef0,
+                  double gtol, int max_iter, double* coef, int* n_iter);
+
 /* PotentialPropertiesFast.eval (py_properties_fast.cpp:19-28): energy,
  * forces (n x 3) and stress (xx, yy, zz, xy, yz, zx) in eV/cell. */
 int plm_eval(plm_model* m, const double* axis, const double* pos_c, const int* types, int n,

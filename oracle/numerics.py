@@ -32,3 +32,28 @@ def solver_ridge_posv(xtx, xty, alphas):
         coefs[:, i] = solve_linear_equation(A, xty)
         prev = a
     return coefs
+
+
+def solver_cg(xtx, xty, alpha, coef0=None, gtol=1e-2, max_iter=10000):
+    """Conjugate gradient as src/pypolymlp/mlp_dev/gradient/solvers_cg.py:32-93
+    restates it (xtx form): (coefficients the loop ends with, iterations).
+    The reference's coef_min aliases the in-place iterate, so the final
+    iterate is what it returns."""
+    F = len(xty)
+    coef = np.zeros(F) if coef0 is None else np.array(coef0, dtype=float)
+    r = xty - (xtx @ coef + alpha * coef)
+    d = r.copy()
+    nr = np.linalg.norm(r)
+    it = max_iter
+    for i in range(max_iter):
+        if i > 100 and nr / np.sqrt(F) < gtol:
+            it = i
+            break
+        t = xtx @ d + alpha * d
+        lr = nr ** 2 / (t @ d)
+        coef += lr * d
+        r -= lr * t
+        nr_new = np.linalg.norm(r)
+        d = r + (nr_new ** 2 / nr ** 2) * d
+        nr = nr_new
+    return coef, it

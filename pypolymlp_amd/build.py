@@ -48,6 +48,8 @@ SOURCES = [
     ("kernels_xgroup.hip", []),
     ("xgroup.cpp", []),
     ("surrogate.cpp", []),
+    ("kernels_cg.hip", []),
+    ("cg.cpp", []),
 ]
 
 

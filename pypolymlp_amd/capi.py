@@ -26,6 +26,7 @@ EXPORTS = [
     "plm_cell_translations", "plm_neighbor", "plm_batch_create", "plm_batch_destroy",
     "plm_batch_n_rows", "plm_batch_n_pairs", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
+    "plm_solver_cg",
     "plm_eval", "plm_eval_batch", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
     "plm_surrogate_n_params", "plm_surrogate_train", "plm_surrogate_grad",
     "plm_surrogate_predict",
@@ -347,6 +348,23 @@ def ridge(xtx, xty, alphas) -> np.ndarray:
     _check(lib().plm_ridge(F, _pd(A), _pd(b), _pd(al), len(al), _pd(out), _pi(info)))
     out[info != 0] = 1e30
     return out
+
+
+def solver_cg(xtx, xty, alpha: float, coef0=None, gtol: float = 1e-2, max_iter: int = 10000):
+    """(coef, iterations) of the device-resident CG on (xtx + alpha I) c = xty."""
+    xtx = _d(xtx)
+    xty = _d(xty)
+    F = len(xty)
+    if xtx.shape != (F, F):
+        raise ValueError("xtx must be F x F")
+    c0 = None if coef0 is None else _d(coef0)
+    if c0 is not None and c0.shape != (F,):
+        raise ValueError("coef0 must have F entries")
+    out = np.zeros(F)
+    n = ctypes.c_int()
+    _check(lib().plm_solver_cg(F, _pd(xtx), _pd(xty), ctypes.c_double(alpha), _pd(c0),
+                               ctypes.c_double(gtol), int(max_iter), _pd(out), ctypes.byref(n)))
+    return out, n.value
 
 
 def profile_enable(on: bool = True):

@@ -1052,6 +1052,14 @@ int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas,
     PLM_TRY({ ridge_solve(F, xtx, xty, alphas, n_alpha, coeffs, info); });
 }
 
+int plm_solver_cg(int F, const double* xtx, const double* xty, double alpha, const double* coef0,
+                  double gtol, int max_iter, double* coef, int* n_iter) {
+    PLM_TRY({
+        const int n = cg_solve(F, xtx, xty, alpha, coef0, gtol, max_iter, coef);
+        if (n_iter) *n_iter = n;
+    });
+}
+
 int plm_surrogate_n_params(int* n) { PLM_TRY({ *n = kSurrParams; }); }
 
 int plm_surrogate_train(float* params, float* adam_m, float* adam_v, int t0, const float* x,

@@ -182,6 +182,10 @@ void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, dou
                 double* stresses);
 void ridge_solve(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
                  double* coeffs, int* info);
+// CG on (xtx + alpha I) c = xty from c0 (zeros when null); returns the
+// iteration at which the stopping test ended the loop, or max_iter
+int cg_solve(int F, const double* xtx, const double* xty, double alpha, const double* coef0,
+             double gtol, int max_iter, double* coef);
 
 void surrogate_run(float* params, float* adam_m, float* adam_v, int t0, const float* x,
                    const float* y, int n, int steps, float lr, float wd, int mode, float* grad,

@@ -296,6 +296,40 @@ def fit(model: capi.Model, train: List[Dataset], test: List[Dataset], alphas,
                      rmse_train=float(rmse_train[k]), rmse_test=float(rmse_test[k]))
 
 
+# --- conjugate gradient (gradient/solvers_cg.py, gradient/fit_cg.py) --------
+def solver_cg(xtx, xty, alpha: float, coef0=None, gtol: float = 1e-2, max_iter: int = 10000):
+    """CG on (xtx + alpha I) c = xty from coef0 (zeros), on the GPU
+    (solver_cg, src/pypolymlp/mlp_dev/gradient/solvers_cg.py:32-93): the loop
+    ends at the first iteration i > 100 with |r| / sqrt(F) < gtol or after
+    max_iter iterations, and returns the coefficients it ends with (the
+    reference's coef_min aliases the iterate, which is updated in place)."""
+    return capi.solver_cg(xtx, xty, alpha, coef0=coef0, gtol=gtol, max_iter=max_iter)[0]
+
+
+def fit_cg(model: capi.Model, train: List[Dataset], test: List[Dataset], alphas,
+           precision=capi.PLM_FP64, gtol: float = 1e-2, max_iter: Optional[int] = None
+           ) -> FitResult:
+    """fit_cg (src/pypolymlp/mlp_dev/gradient/fit_cg.py:21-80): CG solves from the
+    largest alpha down, each warm-started from the previous solution, then the
+    alpha with the smallest test RMSE."""
+    tr = calc_xtx_xty(model, train, precision=precision)
+    if max_iter is None:
+        max_iter = max(model.n_features * 3, 50000)
+    alphas = np.asarray(alphas, dtype=float)
+    coefs, c0 = [], None
+    for a in alphas[::-1]:
+        c0 = solver_cg(tr.xtx, tr.xty, a, coef0=c0, gtol=gtol, max_iter=max_iter)
+        coefs.append(c0)
+    coefs = np.array(coefs)[::-1].T
+    rmse_train = compute_rmse(coefs, tr)
+    te = calc_xtx_xty(model, test, precision=precision, scales=tr.scales.copy(),
+                      min_energy=tr.min_energy)
+    rmse_test = compute_rmse(coefs, te)
+    k = int(np.argmin(rmse_test))
+    return FitResult(coeffs=coefs[:, k], scales=tr.scales, alpha=float(alphas[k]),
+                     rmse_train=float(rmse_train[k]), rmse_test=float(rmse_test[k]))
+
+
 # --- accuracy and learning curve (eval_accuracy.py, fit.py:115-179) ---------
 def _rmse(true, pred):
     return float(np.sqrt(np.mean(np.square(true - pred))))

@@ -205,3 +205,38 @@ def test_si_hybrid_features_x_sums(si):
     assert np.sum(x[:, 160:200]) == pytest.approx(1242982.1653808185, rel=1e-6)
     assert np.sum(x[:, 200:240]) == pytest.approx(393663.7339174809, rel=1e-6)
     assert np.sum(x[:, 240:]) == pytest.approx(4754447.450082185, rel=1e-6)
+
+
+def test_si_solver_cg(si):
+    """Device CG on the Si normal equations: the reference's golden
+    coefficients (tests/test_mlp_dev/test_gradient_solvers.py:22-28, alpha 1e2,
+    gtol 1e-4, rtol 1e-2); the loop stops past iteration 100 with a true
+    residual at the tolerance; the oracle's CG on the same matrix ends within
+    1e-4 of the largest coefficient (the two summation orders drift over ~1000
+    iterations and stop some tens of iterations apart)."""
+    from oracle.numerics import solver_cg as cg_oracle
+    _, m, train, _ = si
+    ne = mlp_dev.calc_xtx_xty(m, train, precision=capi.PLM_FP64)
+    F = len(ne.xty)
+    say("si solver_cg gtol 1e-4")
+    c, n_it = capi.solver_cg(ne.xtx, ne.xty, 1e2, gtol=1e-4)
+    np.testing.assert_allclose([c[20], c[30]], [0.000854, -0.000859], rtol=1e-2)
+    res = np.linalg.norm(ne.xty - (ne.xtx @ c + 1e2 * c)) / np.sqrt(F)
+    c_ref, n_ref = cg_oracle(ne.xtx, ne.xty, 1e2, gtol=1e-4)
+    say(f"  iterations: device {n_it}, oracle {n_ref}; true residual {res:.2e}")
+    assert 100 < n_it < 10000 and res < 2e-4
+    assert np.abs(c - c_ref).max() <= 1e-4 * np.abs(c_ref).max()
+    # warm start: from the converged solution the loop stops soon after its
+    # 101-iteration minimum
+    c2, n2 = capi.solver_cg(ne.xtx, ne.xty, 1e2, coef0=c, gtol=1e-4)
+    assert 101 <= n2 < 200
+    assert np.abs(c2 - c).max() <= 1e-3 * np.abs(c).max()
+
+
+def test_si_fit_cg(si):
+    """fit_cg over the default alphas with test = train picks alpha 1e-3
+    (tests/test_mlp_dev/test_gradient_fit.py:12-18)."""
+    inp, m, train, _ = si
+    say("si fit_cg")
+    res = mlp_dev.fit_cg(m, train, train, inp.alphas)
+    assert res.alpha == pytest.approx(0.001)

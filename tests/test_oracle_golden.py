@@ -147,6 +147,14 @@ def test_oracle_si_x_and_host_weights(si_train, tmp_path):
     c = np.stack([np.linalg.solve(xtx + a * np.eye(168), xw.T @ yw) for a in (1e2, 1e3)], axis=1)
     np.testing.assert_allclose(c[20], [0.000854, 0.000642], rtol=1e-3)
     np.testing.assert_allclose(c[30], [-0.000859, -0.000607], rtol=1e-3)
+    # CG restated from solvers_cg.py: the reference's golden coefficients at
+    # alpha 1e2, gtol 1e-4 (test_gradient_solvers.py:22-28, rtol 1e-2), and the
+    # direct solution once run to a tight tolerance
+    from oracle.numerics import solver_cg
+    c_cg, _ = solver_cg(xtx, xw.T @ yw, 1e2, gtol=1e-4)
+    np.testing.assert_allclose([c_cg[20], c_cg[30]], [0.000854, -0.000859], rtol=1e-2)
+    c_tight, _ = solver_cg(xtx, xw.T @ yw, 1e2, gtol=1e-12, max_iter=20000)
+    assert np.abs(c_tight - c[:, 0]).max() <= 1e-4 * np.abs(c[:, 0]).max()
 
 
 def test_oracle_si_eval_accuracy(si_train):
