@@ -149,6 +149,16 @@ int plm_surrogate_predict(const float* params, const float* x, int n, float* y);
 int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
               double* coeffs, int* info);
 
+/* FeaturesAttr (src/cxx/src/compute/py_features_attr.cpp:11-44): the
+ * attributes of the columns of X. Per linear feature f (n_linear of them):
+ * radial[f] its radial index, gtinv[f] its invariant (l-combination) index
+ * (-1 for pair features), tcomb[6 f .. 6 f + 5] its type-pair combination
+ * (-1 padded). Per polynomial column p (n_poly): poly[3 p ..] the linear
+ * features of the product (-1 padded). type_pairs: n_type x n_type type-pair
+ * index. Any output pointer may be NULL (call once with NULLs for the sizes). */
+int plm_model_feature_attrs(const plm_model* m, int* n_linear, int* n_poly, int* radial,
+                            int* gtinv, int* tcomb, int* poly, int* type_pairs);
+
 /* Conjugate gradient (solver_cg, src/pypolymlp/mlp_dev/gradient/solvers_cg.py:32-93)
  * on (xtx + alpha I) coef = xty from coef0 (zeros when NULL), device resident:
  * the loop stops at the first iteration i > 100 with |r| / sqrt(F) < gtol, or

@@ -26,7 +26,7 @@ EXPORTS = [
     "plm_cell_translations", "plm_neighbor", "plm_batch_create", "plm_batch_destroy",
     "plm_batch_n_rows", "plm_batch_n_pairs", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
-    "plm_solver_cg",
+    "plm_solver_cg", "plm_model_feature_attrs",
     "plm_eval", "plm_eval_batch", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
     "plm_surrogate_n_params", "plm_surrogate_train", "plm_surrogate_grad",
     "plm_surrogate_predict",
@@ -348,6 +348,26 @@ def ridge(xtx, xty, alphas) -> np.ndarray:
     _check(lib().plm_ridge(F, _pd(A), _pd(b), _pd(al), len(al), _pd(out), _pi(info)))
     out[info != 0] = 1e30
     return out
+
+
+def feature_attrs(model: "Model"):
+    """(radial ids, gtinv ids (-1 for pair features), type-pair combinations,
+    polynomial factor lists, type-pair table) of the columns of X
+    (FeaturesAttr, py_features_attr.cpp:11-44)."""
+    nl, npoly = ctypes.c_int(), ctypes.c_int()
+    _check(lib().plm_model_feature_attrs(model.handle, ctypes.byref(nl), ctypes.byref(npoly),
+                                         None, None, None, None, None))
+    L, P = nl.value, npoly.value
+    nt = model.n_type
+    radial, gt = np.zeros(max(L, 1), np.int32), np.zeros(max(L, 1), np.int32)
+    tc, poly = np.zeros(max(6 * L, 1), np.int32), np.zeros(max(3 * P, 1), np.int32)
+    tps = np.zeros(nt * nt, np.int32)
+    _check(lib().plm_model_feature_attrs(model.handle, None, None, _pi(radial), _pi(gt), _pi(tc),
+                                         _pi(poly), _pi(tps)))
+    tcomb = [[int(v) for v in tc[6 * f:6 * f + 6] if v >= 0] for f in range(L)]
+    polys = [[int(v) for v in poly[3 * p:3 * p + 3] if v >= 0] for p in range(P)]
+    return (radial[:L].tolist(), gt[:L].tolist(), tcomb, polys,
+            tps.reshape(nt, nt).tolist())
 
 
 def solver_cg(xtx, xty, alpha: float, coef0=None, gtol: float = 1e-2, max_iter: int = 10000):

@@ -1052,6 +1052,45 @@ int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas,
     PLM_TRY({ ridge_solve(F, xtx, xty, alphas, n_alpha, coeffs, info); });
 }
 
+int plm_model_feature_attrs(const plm_model* mp, int* n_linear, int* n_poly, int* radial,
+                            int* gtinv, int* tcomb, int* poly, int* type_pairs) {
+    PLM_TRY({
+        if (!mp) throw std::invalid_argument("null model");
+        const Model& M = reinterpret_cast<const ModelHandle*>(mp)->host;
+        const auto& lin = M.linear_terms();
+        const bool gt = M.input().gtinv;
+        const int nl = (int)lin.size();
+        const int np = (int)(M.comb2().size() + M.comb3().size());
+        if (n_linear) *n_linear = nl;
+        if (n_poly) *n_poly = np;
+        for (int f = 0; f < nl; ++f) {
+            if (radial) radial[f] = lin[f].n;
+            if (gtinv) gtinv[f] = gt ? lin[f].lm_comb_id : -1;
+            if (tcomb) {
+                for (int k = 0; k < 6; ++k) tcomb[6 * f + k] = -1;
+                if (gt) {
+                    const auto& c = M.tp_combs()[lin[f].order][lin[f].tp_comb_id];
+                    if (c.size() > 6) throw std::runtime_error("type-pair combination longer than 6");
+                    for (size_t k = 0; k < c.size(); ++k) tcomb[6 * f + k] = c[k];
+                } else {
+                    tcomb[6 * f] = lin[f].tp_comb_id;
+                }
+            }
+        }
+        if (poly) {
+            int p = 0;
+            for (const auto* cs : {&M.comb2(), &M.comb3()})
+                for (const auto& c : *cs) {
+                    for (int k = 0; k < 3; ++k) poly[3 * p + k] = k < (int)c.size() ? c[k] : -1;
+                    ++p;
+                }
+        }
+        if (type_pairs)
+            for (int a = 0; a < M.n_type(); ++a)
+                for (int b = 0; b < M.n_type(); ++b) type_pairs[a * M.n_type() + b] = M.type_pair(a, b);
+    });
+}
+
 int plm_solver_cg(int F, const double* xtx, const double* xty, double alpha, const double* coef0,
                   double gtol, int max_iter, double* coef, int* n_iter) {
     PLM_TRY({

@@ -109,6 +109,8 @@ class Model {
     const std::vector<LinearTerm>& linear_terms() const { return linear_; }
     const std::vector<std::vector<int>>& comb2() const { return comb2_; }
     const std::vector<std::vector<int>>& comb3() const { return comb3_; }
+    const std::vector<std::vector<std::vector<int>>>& tp_combs() const { return tp_combs_; }
+    const std::vector<std::vector<int>>& type_pairs() const { return type_pairs_; }
 
     // Fold fitted coefficients (as passed to PolymlpEval, i.e. coeffs/scales;
     // the factor 2 of polymlp_eval.cpp:17 is applied here) into per-head terms.

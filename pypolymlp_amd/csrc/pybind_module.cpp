@@ -296,6 +296,70 @@ class PotentialHybridModel {
     vector1i xf_begin_, xs_begin_, n_data_, cum_;
 };
 
+// FeaturesAttr (py_features_attr.cpp:11-66): column attributes of a model
+class FeaturesAttr {
+  public:
+    explicit FeaturesAttr(const py::dict& params) {
+        ModelOwner mo(params);
+        int nl = 0, np = 0;
+        check(plm_model_feature_attrs(mo.m, &nl, &np, nullptr, nullptr, nullptr, nullptr, nullptr));
+        const int nt = params["n_type"].cast<int>();
+        const bool gtinv = params["model"].cast<py::dict>()["feature_type"].cast<std::string>() == "gtinv";
+        vector1i radial(nl), gt(nl), tc(6 * (size_t)nl), poly(3 * (size_t)np), tps((size_t)nt * nt);
+        check(plm_model_feature_attrs(mo.m, nullptr, nullptr, radial.data(), gt.data(), tc.data(),
+                                      poly.data(), tps.data()));
+        radial_ = radial;
+        if (gtinv) gtinv_ = gt;
+        for (int f = 0; f < nl; ++f) {
+            vector1i c;
+            for (int k = 0; k < 6; ++k)
+                if (tc[6 * f + k] >= 0) c.push_back(tc[6 * f + k]);
+            tcomb_.push_back(c);
+        }
+        for (int p = 0; p < np; ++p) {
+            vector1i c;
+            for (int k = 0; k < 3; ++k)
+                if (poly[3 * p + k] >= 0) c.push_back(poly[3 * p + k]);
+            poly_.push_back(c);
+        }
+        type_pairs_.assign(nt, vector1i(nt));
+        for (int a = 0; a < nt; ++a)
+            for (int b = 0; b < nt; ++b) type_pairs_[a][b] = tps[a * nt + b];
+    }
+    const vector1i& get_radial_ids() const { return radial_; }
+    const vector1i& get_gtinv_ids() const { return gtinv_; }
+    const vector2i& get_tcomb_ids() const { return tcomb_; }
+    const vector2i& get_polynomial_ids() const { return poly_; }
+    const vector2i& get_type_pairs() const { return type_pairs_; }
+
+  private:
+    vector1i radial_, gtinv_;
+    vector2i tcomb_, poly_, type_pairs_;
+};
+
+// Readgtinv (polymlp_read_gtinv.cpp:23-59): the coupling tables kept for
+// (order, maxl, version), read from the same version-1 tables the params
+// dicts are built from (pypolymlp_amd.gtinv)
+class Readgtinv {
+  public:
+    Readgtinv(int order, const vector1i& maxl, int version) {
+        py::object r = py::module_::import("pypolymlp_amd.gtinv")
+                           .attr("get_gtinv_attrs")(order, maxl, version);
+        py::tuple t = r.cast<py::tuple>();
+        l_comb_ = t[0].cast<vector2i>();
+        lm_seq_ = t[1].cast<std::vector<vector2i>>();
+        lm_coeffs_ = t[2].cast<vector2d>();
+    }
+    const std::vector<vector2i>& get_lm_seq() const { return lm_seq_; }
+    const vector2i& get_l_comb() const { return l_comb_; }
+    const vector2d& get_lm_coeffs() const { return lm_coeffs_; }
+
+  private:
+    std::vector<vector2i> lm_seq_;
+    vector2i l_comb_;
+    vector2d lm_coeffs_;
+};
+
 // PotentialPropertiesFast (py_properties_fast.cpp:10-85)
 class PotentialPropertiesFast {
   public:
@@ -477,6 +541,19 @@ PYBIND11_MODULE(libmlpcpp, m) {
              py::return_value_policy::reference_internal)
         .def("get_n_data", &PotentialHybridModel::get_n_data,
              py::return_value_policy::reference_internal);
+    py::class_<FeaturesAttr>(m, "FeaturesAttr")
+        .def(py::init<const py::dict&>())
+        .def("get_radial_ids", &FeaturesAttr::get_radial_ids, py::return_value_policy::reference_internal)
+        .def("get_gtinv_ids", &FeaturesAttr::get_gtinv_ids, py::return_value_policy::reference_internal)
+        .def("get_tcomb_ids", &FeaturesAttr::get_tcomb_ids, py::return_value_policy::reference_internal)
+        .def("get_polynomial_ids", &FeaturesAttr::get_polynomial_ids,
+             py::return_value_policy::reference_internal)
+        .def("get_type_pairs", &FeaturesAttr::get_type_pairs, py::return_value_policy::reference_internal);
+    py::class_<Readgtinv>(m, "Readgtinv")
+        .def(py::init<int, const vector1i&, int>())
+        .def("get_lm_seq", &Readgtinv::get_lm_seq, py::return_value_policy::reference_internal)
+        .def("get_l_comb", &Readgtinv::get_l_comb, py::return_value_policy::reference_internal)
+        .def("get_lm_coeffs", &Readgtinv::get_lm_coeffs, py::return_value_policy::reference_internal);
     py::class_<PotentialPropertiesFast>(m, "PotentialPropertiesFast")
         .def(py::init<const py::dict&, const vector1d&>())
         .def("eval", &PotentialPropertiesFast::eval)

@@ -241,7 +241,50 @@ def test_pybind_mirror_exposes_reference_classes():
                                     "get_e_array", "get_f_array", "get_s_array"),
         "NeighborFull": ("get_distances", "get_differences", "get_neighbor_indices"),
         "NeighborHalf": ("get_differences", "get_neighbor_indices"),
+        "FeaturesAttr": ("get_radial_ids", "get_gtinv_ids", "get_tcomb_ids",
+                         "get_polynomial_ids", "get_type_pairs"),
+        "Readgtinv": ("get_lm_seq", "get_l_comb", "get_lm_coeffs"),
     }.items():
         c = getattr(libmlpcpp, cls)
         for mth in methods:
             assert hasattr(c, mth), (cls, mth)
+
+
+def test_features_attr_golden():
+    """Column attributes of the Si and MgO models
+    (tests/test_mlp_dev/test_core_features_attr.py:12-95)."""
+    from pypolymlp_amd.features_attr import get_features_attr, get_num_features
+    from pypolymlp_amd.params import parse_polymlp_in
+    from tests.helpers import ref_file
+    si = parse_polymlp_in(ref_file("polymlp.in.phono3py.Si")).spec
+    assert get_num_features(si) == 168
+    assert get_num_features([si, si]) == 336
+    feats, poly, pairs = get_features_attr(si)
+    assert feats[-1][0] == 6 and feats[-1][1] == 19 and feats[-1][2] == [0, 0, 0]
+    assert poly == [[a, b] for b in range(0, 140, 20) for a in range(0, b + 1, 20)]
+    assert pairs[0][0] == [0, 0]
+    mgo = parse_polymlp_in(ref_file("polymlp.in")).spec
+    assert get_num_features(mgo) == 1899
+    feats, poly, pairs = get_features_attr(mgo)
+    assert feats[-1] == (8, 19, [2, 2, 2]) and feats[-2] == (8, 19, [1, 2, 2])
+    assert poly[6:9] == [[1, 178], [178, 178], [0, 179]]
+    assert pairs[0][0] == [0, 0] and pairs[1] == [[0, 1], [1, 0]] and pairs[2][0] == [1, 1]
+
+
+def test_pybind_features_attr_and_readgtinv():
+    """FeaturesAttr and Readgtinv of the libmlpcpp mirror agree with the host
+    functions (and need no GPU)."""
+    from pypolymlp_amd import libmlpcpp
+    from pypolymlp_amd.features_attr import get_features_attr
+    from pypolymlp_amd.gtinv import get_gtinv_attrs
+    from pypolymlp_amd.params import parse_polymlp_in
+    from tests.helpers import ref_file
+    spec = parse_polymlp_in(ref_file("polymlp.in")).spec
+    obj = libmlpcpp.FeaturesAttr(spec.as_dict())
+    feats, poly, _ = get_features_attr(spec)
+    assert list(zip(obj.get_radial_ids(), obj.get_gtinv_ids(), obj.get_tcomb_ids())) == feats
+    assert obj.get_polynomial_ids() == poly
+    r = libmlpcpp.Readgtinv(3, [4, 4], 1)
+    l_comb, lm_seq, lm_coeffs = get_gtinv_attrs(3, [4, 4], 1)
+    assert r.get_l_comb() == [list(x) for x in l_comb]
+    assert len(r.get_lm_seq()) == len(lm_seq) and len(r.get_lm_coeffs()) == len(lm_coeffs)
