@@ -288,3 +288,38 @@ def test_pybind_features_attr_and_readgtinv():
     l_comb, lm_seq, lm_coeffs = get_gtinv_attrs(3, [4, 4], 1)
     assert r.get_l_comb() == [list(x) for x in l_comb]
     assert len(r.get_lm_seq()) == len(lm_seq) and len(r.get_lm_coeffs()) == len(lm_coeffs)
+
+
+def test_vasprun_reader_golden(tmp_path):
+    """The reference's MgO vasprun fixtures (tests/test_core/test_interface_vasp.py:24-66):
+    forces, energies, stresses in eV/cell, volumes; and a polymlp.in with a vasp
+    glob gives the split datasets."""
+    from pypolymlp_amd.interface_vasp import parse_properties_from_vaspruns
+    from tests.helpers import FILES, ref_file
+    files = [ref_file("vasprun-00001-MgO.xml"), ref_file("vasprun-00002-MgO.xml")]
+    sts, (energies, forces, stresses) = parse_properties_from_vaspruns(files)
+    assert forces[0].shape == (3, 64) and forces[1].shape == (3, 64)
+    assert forces[0][0, 3] == pytest.approx(0.00875877)
+    assert forces[1][2, 5] == pytest.approx(0.00594728)
+    np.testing.assert_allclose(energies, [-381.0917812, -381.09122093], atol=1e-6)
+    np.testing.assert_allclose(stresses, [
+        [[0.20440833, -0.00616032, -0.02821958], [-0.00616039, 0.21007008, 0.00932128],
+         [-0.02821948, 0.00932117, 0.25478931]],
+        [[0.31962655, -0.08733854, 0.00825778], [-0.08733852, 0.11339135, 0.0465711],
+         [0.00825789, 0.04657095, 0.23225083]]], atol=1e-6)
+    np.testing.assert_allclose([s.volume for s in sts], [612.13782226, 612.13394808], atol=1e-6)
+    assert sts[0].types[:4] == [0, 0, 0, 0] and sts[0].elements[-1] == "O"
+    from pypolymlp_amd.dataset_io import load_datasets
+    from pypolymlp_amd.params import parse_polymlp_in
+    inp_file = tmp_path / "polymlp.in"
+    inp_file.write_text(open(ref_file("polymlp.in")).read().replace(
+        "data dataset/vasprun-*.xml.polymlp", "train_data vasprun-0000*-MgO.xml\n"
+        "test_data vasprun-00002-MgO.xml True 0.5"))
+    inp = parse_polymlp_in(str(inp_file))
+    train, test = load_datasets(inp, prefix=FILES)
+    assert len(train[0].energies) == 2 and len(test[0].energies) == 1
+    assert test[0].weight == 0.5 and train[0].include_stress
+    # atomic energies (-0.0004, -1.85321219) per Mg / O atom subtracted
+    assert train[0].energies[0] == pytest.approx(-381.0917812 + 32 * 0.0004 + 32 * 1.85321219)
+    assert train[0].stresses[1].tolist() == pytest.approx(
+        [0.31962655, 0.11339135, 0.23225083, -0.08733854, 0.0465711, 0.00825789], abs=1e-6)
