@@ -9,6 +9,7 @@
 // work of _compute_products_single_batch
 // (src/pypolymlp/mlp_dev/core/data_sequential.py:97-155).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "capi_internal.hpp"
@@ -16,11 +17,15 @@
 
 namespace plm {
 
-static constexpr size_t kXBudget = 3ull << 30;   // bytes of X per sub-batch
+static constexpr size_t kXBudget = 3ull << 30;   // bytes of X per sub-batch (PLM_X_BUDGET_MB)
 static constexpr size_t kABudget = 8ull << 30;   // bytes of per-atom work (a, W', x) per sub-batch
 static constexpr size_t kPBudget = 6ull << 30;   // bytes of pair records per sub-batch
 static constexpr size_t kDBudget = 4ull << 30;   // bytes of factor derivatives per sub-batch
 
+// Sub-batches run in order on the library stream: X rows (weights applied,
+// targets in column F), then the Gram of [X | y]. (Running the Gram of one
+// sub-batch on a second stream while the next X is built was measured slower
+// for the mixed form: the two contend for the CUs' registers and LDS.)
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
                        bool fp64) {
     ModelHandle& m = *acc.model;
@@ -34,6 +39,13 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
     const size_t atom_bytes = atom_work_bytes(m, fp64);
     double* xe_sum = reinterpret_cast<double*>(acc.vec.ptr);
     double* xe_sq = xe_sum + acc.F_pad;
+    // PLM_X_BUDGET_MB lowers the X budget of a sub-batch (less device memory,
+    // more sub-batches)
+    size_t x_budget = kXBudget;
+    if (const char* e = std::getenv("PLM_X_BUDGET_MB")) {
+        const long mb = std::atol(e);
+        if (mb > 0) x_budget = std::min(kXBudget, (size_t)mb << 20);
+    }
     int s0 = 0;
     while (s0 < b.n_st) {
         int s1 = s0 + 1;
@@ -41,7 +53,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
             const size_t rows = (size_t)(b.row_off[s_end] - b.row_off[s0]);
             const size_t atoms = (size_t)(b.atom_off[s_end] - b.atom_off[s0]);
             const size_t pairs = (size_t)(b.nb_off[b.atom_off[s_end]] - b.nb_off[b.atom_off[s0]]);
-            return (rows + kstep) * row_bytes <= kXBudget &&
+            return (rows + kstep) * row_bytes <= x_budget &&
                    atoms * atom_bytes <= kABudget && pairs * rec_bytes <= kPBudget &&
                    poly_bytes(m, b, s0, s_end, fp64) <= kDBudget;
         };

@@ -219,3 +219,39 @@ def test_normal_equations_cfg1():
     ev = np.linalg.eigvalsh(G * d[:, None] * d[None, :])
     say(f"  mixed [X|y] Gram, unit diagonal: eigenvalue min {ev.min():.2e} max {ev.max():.2e}")
     assert ev.min() > -1e-10 * ev.max()
+
+
+@pytest.mark.parametrize("prec", [capi.PLM_FP64, capi.PLM_MIXED])
+def test_normal_equations_sub_batches(prec, monkeypatch):
+    """One accumulate call split into several sub-batches (X budget 8 MB,
+    about 7 sub-batches of cfg1): X^T X, X^T y, y^T y and the energy-row sums
+    equal those of a single sub-batch: to
+    rounding in FP64; in the mixed form to 1e-6, because the quantisation
+    exponents are shared per 4096-row chunk of a sub-batch and the chunks move
+    with the sub-batch boundaries (each Gram is exact for its own quantised X)."""
+    ds = datasets.config1_dataset(40)
+    m = capi.Model(datasets.config1_model().as_dict())
+    axes, pos, types = _structs(ds.structures)
+    b = capi.Batch(m, axes, pos, types)
+    rng = np.random.default_rng(8)
+    w = rng.uniform(0.1, 1.0, b.n_rows)
+    y = rng.normal(size=b.n_rows)
+    out = {}
+    for budget in (None, "8"):
+        if budget is None:
+            monkeypatch.delenv("PLM_X_BUDGET_MB", raising=False)
+        else:
+            monkeypatch.setenv("PLM_X_BUDGET_MB", budget)
+        say(f"normal equations prec={prec} x budget={budget}")
+        acc = capi.Normal(m)
+        acc.accumulate(b, w, w * y, prec)
+        acc.accumulate(b, w, w * y, prec)
+        out[budget] = acc.result()
+    a, c = out[None], out["8"]
+    assert a[5] == c[5] == 2 * b.n_rows
+    tol = 1e-12 if prec == capi.PLM_FP64 else 1e-6
+    for k in range(4):
+        err = np.linalg.norm(c[k] - a[k]) / np.linalg.norm(a[k])
+        say(f"  output {k}: relative difference {err:.2e}")
+        assert err <= (1e-12 if k >= 2 else tol)
+    assert c[4] == pytest.approx(a[4], rel=tol)
