@@ -91,17 +91,35 @@ constexpr int kQChunk = 4096;  // rows sharing one exponent per column
 constexpr int kQK = 32;        // rows per K step (one i8 MFMA)
 
 // E[chunk][c]: smallest E with |x| < 2^E over the chunk's rows (0 for a zero column).
-__global__ void colexp_kernel(const float* __restrict__ X, int ld, int rows, int ncols,
-                              int* __restrict__ E) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+// 64 columns per workgroup, four row phases of 64 threads; each thread keeps
+// four loads in flight (the maximum is order independent, so the result does
+// not depend on the split)
+__global__ void __launch_bounds__(256) colexp_kernel(const float* __restrict__ X, int ld, int rows,
+                                                     int ncols, int* __restrict__ E) {
+    __shared__ float part[4][64];
+    const int cx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cx;
     const int chunk = blockIdx.y;
-    if (c >= ncols) return;
     const int r0 = chunk * kQChunk, r1 = min(rows, r0 + kQChunk);
-    float m = 0.f;
-    for (int r = r0; r < r1; ++r) m = fmaxf(m, fabsf(X[(size_t)r * ld + c]));
-    int e = 0;
-    if (m > 0.f) (void)frexpf(m, &e);
-    E[(size_t)chunk * ld + c] = e;
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+    if (c < ncols) {
+        int r = r0 + ty;
+        for (; r + 12 < r1; r += 16) {
+            m0 = fmaxf(m0, fabsf(X[(size_t)r * ld + c]));
+            m1 = fmaxf(m1, fabsf(X[(size_t)(r + 4) * ld + c]));
+            m2 = fmaxf(m2, fabsf(X[(size_t)(r + 8) * ld + c]));
+            m3 = fmaxf(m3, fabsf(X[(size_t)(r + 12) * ld + c]));
+        }
+        for (; r < r1; r += 4) m0 = fmaxf(m0, fabsf(X[(size_t)r * ld + c]));
+    }
+    part[ty][cx] = fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
+    __syncthreads();
+    if (ty == 0 && c < ncols) {
+        const float m = fmaxf(fmaxf(part[0][cx], part[1][cx]), fmaxf(part[2][cx], part[3][cx]));
+        int e = 0;
+        if (m > 0.f) (void)frexpf(m, &e);
+        E[(size_t)chunk * ld + c] = e;
+    }
 }
 
 // Q[s][rb][c][k] (int8): slice s of x[32 rb + k][c], k-contiguous per column so
@@ -360,7 +378,7 @@ void launch_gram_quantize(hipStream_t st, const float* X, int ld, int rows_pad, 
     signed char* Q = reinterpret_cast<signed char*>(quant);
     int* E = reinterpret_cast<int*>(Q + 3 * (size_t)rows_pad * ld);
     const int B = 256;
-    colexp_kernel<<<dim3((F_pad + B - 1) / B, n_chunks), B, 0, st>>>(X, ld, rows_pad, F_pad, E);
+    colexp_kernel<<<dim3((F_pad + 63) / 64, n_chunks), 256, 0, st>>>(X, ld, rows_pad, F_pad, E);
     quantize_kernel<<<dim3((F_pad + B - 1) / B, rows_pad / kQK), B, 0, st>>>(X, ld, rows_pad,
                                                                               F_pad, E, Q);
 }
