@@ -32,7 +32,7 @@
 
 namespace plm {
 
-constexpr int kPdPairs = 16;   // pairs per LDS chunk in pairdata_kernel
+constexpr int kPdPairs = 4;    // pairs per LDS chunk in pairdata_kernel: 8 KB of LDS in FP32, about 20 waves per CU (16: 5 waves, 2.1x slower)
 constexpr int kMaxPairTypes = 64;  // type pairs the grouped record order supports
 // staged record elements per lane (64 lanes): FP32 groups are one record
 // (<= 768 values), FP64 groups two
