@@ -67,6 +67,8 @@ struct PolyTables {
     Store store;
     PolyDev dev{};
     bool enabled = false;
+    // entries of factor f: [ent_off[f], ent_off[f + 1])
+    std::vector<int> ent_off;
 };
 
 struct GroupTables {
@@ -143,7 +145,7 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover);
 struct XWork {
     DevBuf a, w, x, pd;
     DevBuf wg, tpo;            // grouped W', per-atom type-pair ranges of the pair records
-    DevBuf xf, m, d, doff, z;  // polynomial columns: factor values, monomials, D, Z
+    DevBuf xf, m, d, doff, z, msel;  // polynomial columns: factor values, monomials, D, Y, gathered monomials
 };
 
 // X rows of structures [s0, s1) into X (row-major, leading dimension ld),

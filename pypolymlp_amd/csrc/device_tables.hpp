@@ -108,6 +108,11 @@ struct PolyDev {
     const int* col_u;        // [n_cols][3] monomial of the other factors of each slot
     const int* mono_a;       // [U] first factor of each monomial
     const int* mono_b;       // [U] second factor (-1 for degree 1)
+    // entries: the distinct (factor, monomial) pairs of the columns' slots,
+    // sorted by factor, so that the entries of one factor are a contiguous range
+    int E;                   // number of entries
+    const int* ent_u;        // [E] monomial of each entry
+    const int* col_pos;      // [n_cols][3] entry of each slot of a column, -1 if none
 };
 
 // Signature groups of the linear features: features of one atom type whose

@@ -148,6 +148,9 @@ void launch_poly_deriv(hipStream_t st, bool fp64, const ModelDev& md, const Poly
                        const BatchDev& bd, int atom0, int n_atoms, long pair_base,
                        const void* prec, const PairRec& R, const void* wall, int w_stride, int k0,
                        int kc, int rd_max, void* D, int st0, const long* d_off);
+void launch_poly_msel(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,
+                      int s_first, int n_st, int atom_base, int n_max, const void* M, void* Ms,
+                      long ms_stride);
 void launch_poly_gather(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,
                         int s_first, int n_st, int atom_base, const void* Z, long z_stride,
                         const void* XF, void* X, int ld, int row_base, const double* roww,

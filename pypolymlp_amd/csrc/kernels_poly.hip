@@ -6,9 +6,11 @@
 //   energy row:         X[e, c] = sum_i prod_s x_{f_s}(i)
 // where D_f[r, i] = d x_f(i) / d xi_r is the derivative of factor f at atom i
 // (rows r: 6 stress, then 3N forces) and m_{c\s}(i) the product of the other
-// factors at atom i. With Z_s[(f, r), u] = sum_i D[i][f][r] M[i][u] over all
-// factors f and all monomials u of degree 1 and 2 (one GEMM per structure),
-// every column is a sum of nfac entries of Z.
+// factors at atom i. Each (slot s, column) pair is an entry e = (f, u) with
+// f the slot's factor and u the monomial of the others; entries are sorted by
+// slot and factor, and Y[r][e] = sum_i D[i][f][r] M[i][u] is one GEMM per
+// (slot, factor) over that factor's entries (B: the gathered monomials).
+// Every column is then the sum of its nfac entries of Y.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -128,12 +130,27 @@ __global__ void __launch_bounds__(128) poly_deriv_kernel(ModelDev md, PolyDev pd
     for (int e = threadIdx.x; e < kc * RD; e += blockDim.x) out[e] = acc[e];
 }
 
-// X[row][c0 + c] for the structures of one GEMM group: force/stress rows from
-// Z, the energy row from the factor values; rows weighted, raw energy rows
-// into the scale sums.
+// Gathered monomials of one GEMM group: Ms[s][i][e] = M[i][u_e] over the slot
+// entries e, the B operand of the per-factor products.
+template <class T>
+__global__ void poly_msel_kernel(PolyDev pd, BatchDev bd, int s_first, int n_st, int atom_base,
+                                 const T* __restrict__ M, T* __restrict__ Ms, long ms_stride) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    const int s = s_first + blockIdx.z;
+    if (e >= pd.E || blockIdx.z >= n_st) return;
+    const int a0 = bd.st_atom_off[s];
+    if (i >= bd.st_atom_off[s + 1] - a0) return;
+    Ms[(size_t)blockIdx.z * ms_stride + (size_t)i * pd.E + e] =
+        M[(size_t)(a0 + i - atom_base) * pd.U + pd.ent_u[e]];
+}
+
+// X[row][c0 + c] for the structures of one GEMM group: force/stress rows as
+// the sum of the column's slot entries of Y[s][r][e], the energy row from the
+// factor values; rows weighted, raw energy rows into the scale sums.
 template <class T>
 __global__ void poly_gather_kernel(PolyDev pd, BatchDev bd, int s_first, int n_st, int atom_base,
-                                   const T* __restrict__ Z, long z_stride,
+                                   const T* __restrict__ Y, long y_stride,
                                    const T* __restrict__ XF, T* __restrict__ X, int ld,
                                    int row_base, const double* __restrict__ roww,
                                    double* __restrict__ xe_sum, double* __restrict__ xe_sq) {
@@ -145,13 +162,14 @@ __global__ void poly_gather_kernel(PolyDev pd, BatchDev bd, int s_first, int n_s
     const int RD = 6 + 3 * nat;
     const int rb = bd.st_row_off[s] - row_base;
     const int* ph = pd.col_phi + 3 * (size_t)c;
-    const int* uu = pd.col_u + 3 * (size_t)c;
+    const int* pos = pd.col_pos + 3 * (size_t)c;
     const int nf = ph[2] >= 0 ? 3 : 2;
-    const T* Zs = Z + (size_t)(s - s_first) * z_stride;
-    const int U = pd.U;
+    const T* Ys = Y + (size_t)(s - s_first) * y_stride;
+    const int E = pd.E;
     for (int r = blockIdx.y; r < RD; r += gridDim.y) {
-        T v = T(0);
-        for (int k = 0; k < nf; ++k) v += Zs[((size_t)ph[k] * RD + r) * U + uu[k]];
+        const T* yr = Ys + (size_t)r * E;
+        T v = yr[pos[0]] + yr[pos[1]];
+        if (nf == 3) v += yr[pos[2]];
         const double w = roww ? roww[rb + 1 + r] : 1.0;
         X[(size_t)(rb + 1 + r) * ld + pd.c0 + c] = T(w * double(v));
     }
@@ -209,6 +227,21 @@ void launch_poly_deriv(hipStream_t st, bool fp64, const ModelDev& md, const Poly
             reinterpret_cast<const Cpx<float>*>(wall), w_stride, k0, kc,
             reinterpret_cast<float*>(D), st0, d_off);
     }
+}
+
+void launch_poly_msel(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,
+                      int s_first, int n_st, int atom_base, int n_max, const void* M, void* Ms,
+                      long ms_stride) {
+    const int B = 256;
+    dim3 grid((pd.E + B - 1) / B, n_max, n_st);
+    if (fp64)
+        poly_msel_kernel<double><<<grid, B, 0, st>>>(pd, bd, s_first, n_st, atom_base,
+                                                     reinterpret_cast<const double*>(M),
+                                                     reinterpret_cast<double*>(Ms), ms_stride);
+    else
+        poly_msel_kernel<float><<<grid, B, 0, st>>>(pd, bd, s_first, n_st, atom_base,
+                                                    reinterpret_cast<const float*>(M),
+                                                    reinterpret_cast<float*>(Ms), ms_stride);
 }
 
 void launch_poly_gather(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,

@@ -1,11 +1,13 @@
-// Polynomial columns of X by the factor-derivative GEMM: host tables and the
+// Polynomial columns of X by factor-derivative GEMMs: host tables and the
 // per-sub-batch sequence (factor values and monomials, factor derivatives,
-// one rocBLAS strided-batched GEMM per group of equal-size structures,
-// gather into the X rows). The column order and the products follow
+// per group of equal-size structures the gathered monomials of the entries
+// and one rocBLAS strided-batched GEMM per factor over its entries, then the
+// sum of each column's entries into the X rows). The column order and the products follow
 // Model::model_polynomial / model_order{2,3} (src/cxx/src/compute/model.cpp:159-267).
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <stdexcept>
 #include <vector>
@@ -73,7 +75,27 @@ const PolyTables& ModelHandle::poly() {
             if (it != phi.end()) floc[(size_t)t * K + it->second] = (int)k;
         }
     }
+    // entries: the distinct (factor, monomial) pairs of the columns' slots,
+    // sorted by factor then monomial; col_pos maps (column, slot) to its entry
+    std::map<std::pair<int, int>, int> ent_id;
+    for (size_t c = 0; c < cols.size(); ++c)
+        for (int sl = 0; sl < 3; ++sl)
+            if (cphi[3 * c + sl] >= 0) ent_id.emplace(std::make_pair(cphi[3 * c + sl], cu[3 * c + sl]), 0);
+    std::vector<int> ent_u, col_pos(cols.size() * 3, -1);
+    pt->ent_off.assign((size_t)K + 1, 0);
+    for (auto& kv : ent_id) {
+        kv.second = (int)ent_u.size();
+        ent_u.push_back(kv.first.second);
+        ++pt->ent_off[(size_t)kv.first.first + 1];
+    }
+    for (int f = 0; f < K; ++f) pt->ent_off[f + 1] += pt->ent_off[f];
+    for (size_t c = 0; c < cols.size(); ++c)
+        for (int sl = 0; sl < 3; ++sl)
+            if (cphi[3 * c + sl] >= 0) col_pos[3 * c + sl] = ent_id.at({cphi[3 * c + sl], cu[3 * c + sl]});
     PolyDev& d = pt->dev;
+    d.E = (int)ent_u.size();
+    d.ent_u = pt->store.upload(ent_u.data(), ent_u.size() * 4);
+    d.col_pos = pt->store.upload(col_pos.data(), col_pos.size() * 4);
     d.K = K;
     d.U = (int)ma.size();
     d.n_cols = (int)cols.size();
@@ -144,42 +166,58 @@ void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, 
     check_hip(hipGetLastError(), "poly launch");
     rocblas_handle h = blas_handle();
     const size_t z_budget = 2ull << 30;
+    const int E = pd.E, K = pd.K;
     int s = s0;
     while (s < s1) {
         const int N = b.atom_off[s + 1] - b.atom_off[s];
         const long RD = 6 + 3 * N;
-        const long KR = (long)pd.K * RD;
-        const size_t z_one = (size_t)pd.U * KR * elem;
+        const long KR = (long)K * RD;
+        // Y[r][e] and the gathered monomials Ms[i][e] of one structure
+        const size_t y_one = (size_t)RD * E, ms_one = (size_t)N * E;
+        const size_t one = (y_one + ms_one) * elem;
         int g = 1;
         while (s + g < s1 && b.atom_off[s + g + 1] - b.atom_off[s + g] == N &&
-               (size_t)(g + 1) * z_one <= z_budget)
+               (size_t)(g + 1) * one <= z_budget)
             ++g;
-        work.z.alloc(z_one * g);
-        const size_t aoff = (size_t)(b.atom_off[s] - a0) * pd.U;
+        work.z.alloc(y_one * g * elem);
+        work.msel.alloc(ms_one * g * elem);
         const size_t boff = (size_t)doff[s - s0];
         {
             ProfScope ps("poly_gemm", st);
-            rocblas_status rs;
-            if (fp64) {
-                const double one = 1.0, zero = 0.0;
-                rs = rocblas_dgemm_strided_batched(
-                    h, rocblas_operation_none, rocblas_operation_transpose, pd.U, (int)KR, N, &one,
-                    reinterpret_cast<const double*>(work.m.ptr) + aoff, pd.U, (long)N * pd.U,
-                    reinterpret_cast<const double*>(work.d.ptr) + boff, (int)KR, (long)N * KR,
-                    &zero, reinterpret_cast<double*>(work.z.ptr), pd.U, (long)pd.U * KR, g);
-            } else {
-                const float one = 1.f, zero = 0.f;
-                rs = rocblas_sgemm_strided_batched(
-                    h, rocblas_operation_none, rocblas_operation_transpose, pd.U, (int)KR, N, &one,
-                    reinterpret_cast<const float*>(work.m.ptr) + aoff, pd.U, (long)N * pd.U,
-                    reinterpret_cast<const float*>(work.d.ptr) + boff, (int)KR, (long)N * KR,
-                    &zero, reinterpret_cast<float*>(work.z.ptr), pd.U, (long)pd.U * KR, g);
+            launch_poly_msel(st, fp64, pd, bd, s, g, a0, N, work.m.ptr, work.msel.ptr, (long)ms_one);
+            check_hip(hipGetLastError(), "poly monomial gather");
+            // one product per factor: Y[:, e range] = D_f (RD x N) * Ms[:, e range]
+            for (int f = 0; f < K; ++f) {
+                const int e0 = P.ent_off[f];
+                const int m_e = P.ent_off[f + 1] - e0;
+                if (m_e == 0) continue;
+                rocblas_status rs;
+                if (fp64) {
+                    const double one_d = 1.0, zero_d = 0.0;
+                    rs = rocblas_dgemm_strided_batched(
+                        h, rocblas_operation_none, rocblas_operation_transpose, m_e, (int)RD, N,
+                        &one_d, reinterpret_cast<const double*>(work.msel.ptr) + e0, E,
+                        (long)ms_one,
+                        reinterpret_cast<const double*>(work.d.ptr) + boff + (size_t)f * RD,
+                        (int)KR, (long)N * KR, &zero_d,
+                        reinterpret_cast<double*>(work.z.ptr) + e0, E, (long)y_one, g);
+                } else {
+                    const float one_f = 1.f, zero_f = 0.f;
+                    rs = rocblas_sgemm_strided_batched(
+                        h, rocblas_operation_none, rocblas_operation_transpose, m_e, (int)RD, N,
+                        &one_f, reinterpret_cast<const float*>(work.msel.ptr) + e0, E,
+                        (long)ms_one,
+                        reinterpret_cast<const float*>(work.d.ptr) + boff + (size_t)f * RD,
+                        (int)KR, (long)N * KR, &zero_f,
+                        reinterpret_cast<float*>(work.z.ptr) + e0, E, (long)y_one, g);
+                }
+                if (rs != rocblas_status_success)
+                    throw DeviceError("rocBLAS GEMM of the polynomial columns failed");
             }
-            if (rs != rocblas_status_success) throw DeviceError("rocBLAS GEMM of the polynomial columns failed");
         }
         {
             ProfScope ps("poly_gather", st);
-            launch_poly_gather(st, fp64, pd, bd, s, g, a0, work.z.ptr, (long)pd.U * KR,
+            launch_poly_gather(st, fp64, pd, bd, s, g, a0, work.z.ptr, (long)y_one,
                                work.xf.ptr, X, ld, b.row_off[s0], roww, xe_sum, xe_sq, (int)RD);
         }
         check_hip(hipGetLastError(), "poly gather launch");
