@@ -165,7 +165,7 @@ void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, 
     }
     check_hip(hipGetLastError(), "poly launch");
     rocblas_handle h = blas_handle();
-    const size_t z_budget = 2ull << 30;
+    const size_t z_budget = 8ull << 30;  // Y and gathered monomials of one GEMM group
     const int E = pd.E, K = pd.K;
     int s = s0;
     while (s < s1) {
