@@ -615,6 +615,11 @@ static constexpr size_t kLdsBytes = 160 * 1024;
 
 XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover) {
     const int elem = fp64 ? 8 : 4;
+    // the row tile in LDS when some shape holds it, else in X itself
+    // (PLM_XROWS=global: always in X)
+    const char* env = std::getenv("PLM_XROWS");
+    const int g0 = env && std::string(env) == "global" ? 1 : 0;
+    for (int gtile = g0; gtile <= 1; ++gtile)
     for (int CB : {64, 32, 16}) {
         const auto& ct = m.columns(CB, cover);
         const PairRec R(m.md.n_fn, m.md.n_lmh);
@@ -625,7 +630,7 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover) {
             if (NW > xrows_max_waves(fp64)) continue;
             XRowsParams P{};
             P.CB = CB, P.NW = NW, P.need_max = ct.need_max, P.wz_max = ct.wz_max;
-            P.rows_max = rows_max, P.ntp = m.md.n_tp, P.slot = slot;
+            P.rows_max = rows_max, P.ntp = m.md.n_tp, P.slot = slot, P.gtile = gtile;
             const size_t sm = xrows_smem(P, elem);
             if (sm <= kLdsBytes - 1024) {
                 XPlan p{P, &ct, sm};
@@ -633,16 +638,16 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover) {
             }
         }
     }
-    throw std::runtime_error("structure too large for the LDS row tile");
+    throw std::runtime_error("column block too large for LDS");
 }
 
 // Signature-group path for the linear columns: tasks per workgroup (TB) and
 // waves per task (A) for the largest structure, or TB = 0 when its rows do
-// not fit the LDS tile (or the path is switched off: PLM_XROWS=generic).
+// not fit the LDS tile (or the path is switched off: PLM_XROWS=generic or global).
 static void plan_group(ModelHandle& m, int rows_max, bool fp64, int& TB, int& A) {
     TB = 0, A = 0;
     const char* env = std::getenv("PLM_XROWS");
-    if (env && std::string(env) == "generic") return;
+    if (env && (std::string(env) == "generic" || std::string(env) == "global")) return;
     if (!m.groups().enabled) return;
     const int NSW = xgroup_slots(fp64), elem = 8;  // the row tile is FP64 in both precisions
     const size_t per_col = (size_t)(rows_max - 1) * elem;

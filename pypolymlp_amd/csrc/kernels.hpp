@@ -50,6 +50,7 @@ struct XRowsParams {
     int st0, st_end;
     int row_base;  // first X row of the structure range
     int xcd_map;   // 1: all blocks of a structure on one XCD
+    int gtile;     // 1: the row tile is the structure's block of X (rows beyond LDS)
     long pair_base;
 };
 

@@ -295,7 +295,7 @@ struct XRowsLds {
         auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
         size_t o = 0;
         tile = o;
-        o = al(o + (size_t)P.rows_max * P.CB * elem);
+        o = al(o + (P.gtile ? 0 : (size_t)P.rows_max * P.CB * elem));
         wv = o;
         o = al(o + (size_t)P.wz_max * 2 * elem);
         xneed = o;
@@ -372,7 +372,9 @@ __device__ __forceinline__ void q_feature(const T* rec, const PairRec& R,
     q2 = ry * uz + fv * a2;
 }
 
-template <class T>
+// G: the row tile is the structure's own block of X in global memory
+// (structures whose 7 + 3N rows do not fit LDS); its adds are device atomics.
+template <class T, bool G>
 __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) xrows_kernel(ModelDev md, ColumnsDev cd, BatchDev bd,
                                                     BlockWz bw, XRowsParams P,
                                                     const Cpx<T>* __restrict__ wall, int w_stride,
@@ -397,7 +399,6 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) xrows_kernel(Mode
         s = P.st0 + blockIdx.x / cd.n_blocks;
     }
     const XRowsLds Lo(P, sizeof(T));
-    T* tile = reinterpret_cast<T*>(smem_raw + Lo.tile);
     Cpx<T>* Wv = reinterpret_cast<Cpx<T>*>(smem_raw + Lo.wv);
     T* xneed = reinterpret_cast<T*>(smem_raw + Lo.xneed);
     int* nk = reinterpret_cast<int*>(smem_raw + Lo.nk);
@@ -405,6 +406,9 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) xrows_kernel(Mode
 
     const int c0 = cd.blk_c0[blk];
     const int ncol = cd.blk_c0[blk + 1] - c0;
+    const int rbase = bd.st_row_off[s] - P.row_base;
+    T* tile = G ? X + (size_t)rbase * P.ld + c0 : reinterpret_cast<T*>(smem_raw + Lo.tile);
+    const int ts = G ? P.ld : P.CB;  // row stride of the tile
     const int a0 = bd.st_atom_off[s], nat = bd.st_atom_off[s + 1] - a0;
     const int rows = 7 + 3 * nat;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -421,7 +425,9 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) xrows_kernel(Mode
     const bool col_ok = col < ncol;
     const int nfac = col_ok ? cd.nfac[c] : 0;
 
-    for (int e = tid; e < rows * CB; e += blockDim.x) tile[e] = T(0);
+    for (int e = tid; e < rows * CB; e += blockDim.x)
+        if (!G || e % CB < ncol) tile[(size_t)(e / CB) * ts + e % CB] = T(0);
+    if (G) __threadfence();
 
     // this lane's first element of a staged group of records: record p0s, offset o0s
     const int p0s = lane / pre, o0s = lane % pre;
@@ -509,10 +515,10 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) xrows_kernel(Mode
                     q_feature<T>(rec, R, Wv, nk, zr[cn0 * (ntp + 1) + tp],
                                  zr[cn0 * (ntp + 1) + tp + 1], v0, v1, v2);
                     self0 += v0, self1 += v1, self2 += v2;
-                    T* rj = tile + (size_t)(7 + 3 * int(rec[6])) * CB + col;
+                    T* rj = tile + (size_t)(7 + 3 * int(rec[6])) * ts + col;
                     atomicAdd(rj, -v0);
-                    atomicAdd(rj + CB, -v1);
-                    atomicAdd(rj + 2 * CB, -v2);
+                    atomicAdd(rj + ts, -v1);
+                    atomicAdd(rj + 2 * ts, -v2);
                     const T dx = rec[0], dy = rec[1], dz = rec[2];
                     sacc[0] -= v0 * dx, sacc[1] -= v1 * dy, sacc[2] -= v2 * dz;
                     sacc[3] -= v0 * dy, sacc[4] -= v1 * dz, sacc[5] -= v2 * dx;
@@ -539,10 +545,10 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) xrows_kernel(Mode
                         v0 += k2 * q[3 * cn2], v1 += k2 * q[3 * cn2 + 1], v2 += k2 * q[3 * cn2 + 2];
                     }
                     self0 += v0, self1 += v1, self2 += v2;
-                    T* rj = tile + (size_t)(7 + 3 * int(rec[6])) * CB + col;
+                    T* rj = tile + (size_t)(7 + 3 * int(rec[6])) * ts + col;
                     atomicAdd(rj, -v0);
-                    atomicAdd(rj + CB, -v1);
-                    atomicAdd(rj + 2 * CB, -v2);
+                    atomicAdd(rj + ts, -v1);
+                    atomicAdd(rj + 2 * ts, -v2);
                     const T dx = rec[0], dy = rec[1], dz = rec[2];
                     sacc[0] -= v0 * dx, sacc[1] -= v1 * dy, sacc[2] -= v2 * dz;
                     sacc[3] -= v0 * dy, sacc[4] -= v1 * dz, sacc[5] -= v2 * dx;
@@ -551,23 +557,25 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) xrows_kernel(Mode
             wave_sync();
         }
         if (active) {
-            T* ri = tile + (size_t)(7 + 3 * ia) * CB + col;
+            T* ri = tile + (size_t)(7 + 3 * ia) * ts + col;
             atomicAdd(ri, self0);
-            atomicAdd(ri + CB, self1);
-            atomicAdd(ri + 2 * CB, self2);
+            atomicAdd(ri + ts, self1);
+            atomicAdd(ri + 2 * ts, self2);
         }
     }
     if (col_ok) {
-        for (int k = 0; k < 6; ++k) atomicAdd(tile + (size_t)(1 + k) * CB + col, sacc[k]);
+        for (int k = 0; k < 6; ++k) atomicAdd(tile + (size_t)(1 + k) * ts + col, sacc[k]);
         if (wave == 0 && sub == 0) atomicAdd(tile + col, eacc);
     }
+    if (G) __threadfence();
     __syncthreads();
     // write out (weights applied per row); the raw energy row feeds the scale sums
-    const int rbase = bd.st_row_off[s] - P.row_base;
     for (int e = tid; e < rows * CB; e += blockDim.x) {
         const int r = e / CB, cc = e % CB;
         if (cc >= ncol) continue;
-        const T v = tile[e];
+        const T v = G ? __hip_atomic_load(tile + (size_t)r * ts + cc, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT)
+                      : tile[e];
         const double w = roww ? roww[rbase + r] : 1.0;
         X[(size_t)(rbase + r) * P.ld + c0 + cc] = T(w * double(v));
         if (r == 0 && xe_sum) {
@@ -678,11 +686,12 @@ static void launch_xrows_t(hipStream_t st, const ModelDev& md, const ColumnsDev&
                            int atom_base, const void* pd, const PairRec& R, void* X,
                            const double* roww, double* xe_sum, double* xe_sq) {
     const size_t sm = xrows_smem(P, sizeof(T));
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&xrows_kernel<T>),
+    const auto kern = P.gtile ? &xrows_kernel<T, true> : &xrows_kernel<T, false>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     const int n_st = P.st_end - P.st0;
     const long grid = P.xcd_map ? (long)((n_st + 7) / 8) * 8 * cd.n_blocks : (long)n_st * cd.n_blocks;
-    xrows_kernel<T><<<grid, 64 * P.NW, sm, st>>>(
+    kern<<<grid, 64 * P.NW, sm, st>>>(
         md, cd, bd, bw, P, reinterpret_cast<const Cpx<T>*>(w), w_stride,
         reinterpret_cast<const T*>(x), x_stride, atom_base, reinterpret_cast<const T*>(pd), R,
         reinterpret_cast<T*>(X), roww, xe_sum, xe_sq);

@@ -20,6 +20,7 @@ X; the mixed form (exact Gram of the FP32 X quantised to three 7-bit slices)
 within 1e-5 and positive semidefinite to FP64 rounding.
 """
 
+import os
 import sys
 
 import numpy as np
@@ -112,9 +113,11 @@ def test_neighbor_full_and_half_bit_exact():
 
 def _use_path(path, monkeypatch):
     """The linear columns are built by the signature-group kernel unless
-    PLM_XROWS=generic selects the column-block kernel."""
-    if path == "generic":
-        monkeypatch.setenv("PLM_XROWS", "generic")
+    PLM_XROWS=generic selects the column-block kernel; PLM_XROWS=global makes
+    that kernel keep its row tile in X itself (as for structures whose rows do
+    not fit LDS)."""
+    if path in ("generic", "global"):
+        monkeypatch.setenv("PLM_XROWS", path)
     else:
         monkeypatch.delenv("PLM_XROWS", raising=False)
 
@@ -150,7 +153,7 @@ def test_x_rows_mgo(model, path, monkeypatch):
     check_mixed(_features_x(m, b, capi.PLM_MIXED, path), Xr, noise)
 
 
-@pytest.mark.parametrize("path", ["group", "generic"])
+@pytest.mark.parametrize("path", ["group", "generic", "global"])
 @pytest.mark.parametrize("cfg,n_st", [("cfg1", 3), ("cfg2", 2), ("cfg3", 4)])
 def test_x_rows_configs(cfg, n_st, path, tmp_path, monkeypatch):
     _use_path(path, monkeypatch)
@@ -255,3 +258,35 @@ def test_normal_equations_sub_batches(prec, monkeypatch):
         say(f"  output {k}: relative difference {err:.2e}")
         assert err <= (1e-12 if k >= 2 else tol)
     assert c[4] == pytest.approx(a[4], rel=tol)
+
+
+def test_x_rows_largest_cfg3():
+    """The largest structure of the seeded 40-structure ternary set (429 atoms,
+    1294 rows x 25,428 columns) does not fit the LDS row tile, so its block of X
+    is accumulated in place. FP64 against the oracle's seeded projections,
+    energy row and column norms (tests/golden/make_cfg3_large.py); the FP32
+    stream against FP64 column-relative."""
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_cfg3_large as mk
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg3_large.npz"))
+    st = mk.largest_structure()
+    assert st.n_total == int(g["n_atoms"])
+    m = capi.Model(datasets.config3_model().as_dict())
+    say(f"x rows cfg3 largest structure, {st.n_total} atoms")
+    b = capi.Batch(m, [st.axis], [st.positions_cartesian], [st.types])
+    X = capi.features_x(m, b, capi.PLM_FP64)
+    assert X.shape == tuple(g["shape"])
+    p, q = mk.projections(X)
+    for name, a, r in (("u^T X", p, g["ut_x"]), ("X v", q, g["x_v"]), ("energy row", X[0], g["energy_row"])):
+        err = np.linalg.norm(a - r) / np.linalg.norm(r)
+        say(f"  fp64 {name}: relative error {err:.2e}")
+        assert err <= 1e-10
+    cn = np.linalg.norm(X, axis=0)
+    np.testing.assert_allclose(cn, g["col_norm"], rtol=1e-10, atol=1e-10 * g["col_norm"].max())
+    Xm = capi.features_x(m, b, capi.PLM_MIXED)
+    colmax = np.abs(X).max(axis=0)
+    nz = colmax > 0
+    e = np.abs(Xm - X).max(axis=0)[nz] / colmax[nz]
+    say(f"  mixed column-relative: median {np.median(e):.2e} p99 {np.quantile(e, 0.99):.2e} max {e.max():.2e}")
+    assert np.median(e) < 1e-5
+    assert np.quantile(e, 0.99) < 1e-3
