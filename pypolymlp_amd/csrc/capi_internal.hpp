@@ -170,7 +170,7 @@ struct NormalHandle {
     long n_rows = 0;
     DevBuf xtx;   // F_pad x F_pad Gram of [X | y], upper triangle accumulated
     DevBuf vec;   // [xe_sum | xe_sq_sum], each F_pad
-    DevBuf xbuf, wbuf, ybuf, qbuf;
+    DevBuf xbuf, wbuf, ybuf, qbuf, rbuf;
     XWork work;
 };
 

@@ -135,11 +135,16 @@ void launch_set_column(hipStream_t st, bool fp64, void* X, int ld, int col, cons
                        int rows);
 void launch_gram_f64(hipStream_t st, const double* X, int ld, int rows_pad, int F_pad, double* C,
                      int ldc);
-size_t gram_quant_bytes(int rows_pad, int ld);
-void launch_gram_quantize(hipStream_t st, const float* X, int ld, int rows_pad, int F_pad,
-                          void* quant);
-void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int F_pad, const void* quant, double* C,
-                    int ldc);
+// Row q of the quantised matrix is row rmap[q] of X (zero where rmap[q] < 0).
+// Quantisation exponents are shared per column over chunks of rows: the first
+// h0 rows (a multiple of 64, at most gram_chunk_rows()), then
+// gram_chunk_rows() rows each.
+int gram_chunk_rows();
+size_t gram_quant_bytes(int rows_pad, int h0, int ld);
+void launch_gram_quantize(hipStream_t st, const float* X, const int* rmap, int ld, int rows_pad,
+                          int h0, int F_pad, void* quant);
+void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int h0, int F_pad, const void* quant,
+                    double* C, int ldc);
 
 // Polynomial columns (kernels_poly.hip)
 void launch_poly_factor(hipStream_t st, bool fp64, const ModelDev& md, const PolyDev& pd,

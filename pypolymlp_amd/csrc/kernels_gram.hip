@@ -4,8 +4,10 @@
 //
 //   plain FP64 : v_mfma_f64_16x16x4_f64, FP64 accumulation (the reference's
 //                numpy x.T @ x, src/pypolymlp/mlp_dev/core/utils_sequential.py:94-109).
-//   mixed      : X in FP32, quantised per (4096-row chunk, column) to a shared
-//                exponent E and three signed 7-bit integer slices,
+//   mixed      : X in FP32, quantised per (row chunk of at most 4096 rows,
+//                column) to a shared exponent E and three signed 7-bit
+//                integer slices (the energy and stress rows in chunks of their
+//                own, before the force rows),
 //                x~ = 2^(E-7) (d0 + d1 2^-7 + d2 2^-14).
 //                All nine slice products run on v_mfma_i32_32x32x32_i8 with
 //                exact int32 accumulation, one accumulator per level s+t, and
@@ -18,6 +20,8 @@
 // X is row-major with leading dimension ld (>= F + 1, multiple of 128) and
 // rows padded with zeros to a multiple of 64.
 #include <hip/hip_runtime.h>
+
+#include <stdexcept>
 
 #include "kernels.hpp"
 
@@ -87,30 +91,51 @@ __global__ void __launch_bounds__(256) gram_f64_kernel(const double* __restrict_
 // ---------------------------------------------------------------------------
 // Mixed: quantisation
 // ---------------------------------------------------------------------------
-constexpr int kQChunk = 4096;  // rows sharing one exponent per column
+constexpr int kQChunk = 4096;  // rows sharing one exponent per column (at most)
 constexpr int kQK = 32;        // rows per K step (one i8 MFMA)
+
+// Row q of the quantised matrix is row rmap[q] of X (a zero row where
+// rmap[q] < 0). The energy and stress rows of a sub-batch come first, then the
+// force rows: an energy row sums a feature over all N atoms of its structure,
+// so a chunk shared with force rows would set their exponent about N times too
+// high. The Gram does not depend on the row order. Chunk 0 holds the first h0
+// rows (a multiple of 64, at most kQChunk), chunk k > 0 the kQChunk rows from
+// h0 + (k - 1) kQChunk, so the energy and stress rows end on a chunk boundary.
+__host__ __device__ __forceinline__ int chunk_of(int q, int h0) {
+    return q < h0 ? 0 : 1 + (q - h0) / kQChunk;
+}
+__host__ __device__ __forceinline__ int n_chunks_of(int rows, int h0) {
+    return rows <= h0 ? 1 : 1 + (rows - h0 + kQChunk - 1) / kQChunk;
+}
+__device__ __forceinline__ float xrow(const float* __restrict__ X, const int* __restrict__ rmap,
+                                      int q, int ld, int c) {
+    const int r = rmap[q];
+    return r >= 0 ? X[(size_t)r * ld + c] : 0.f;
+}
 
 // E[chunk][c]: smallest E with |x| < 2^E over the chunk's rows (0 for a zero column).
 // 64 columns per workgroup, four row phases of 64 threads; each thread keeps
 // four loads in flight (the maximum is order independent, so the result does
 // not depend on the split)
-__global__ void __launch_bounds__(256) colexp_kernel(const float* __restrict__ X, int ld, int rows,
-                                                     int ncols, int* __restrict__ E) {
+__global__ void __launch_bounds__(256) colexp_kernel(const float* __restrict__ X,
+                                                     const int* __restrict__ rmap, int ld, int rows,
+                                                     int h0, int ncols, int* __restrict__ E) {
     __shared__ float part[4][64];
     const int cx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + cx;
     const int chunk = blockIdx.y;
-    const int r0 = chunk * kQChunk, r1 = min(rows, r0 + kQChunk);
+    const int r0 = chunk == 0 ? 0 : h0 + (chunk - 1) * kQChunk;
+    const int r1 = min(rows, chunk == 0 ? h0 : r0 + kQChunk);
     float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
     if (c < ncols) {
         int r = r0 + ty;
         for (; r + 12 < r1; r += 16) {
-            m0 = fmaxf(m0, fabsf(X[(size_t)r * ld + c]));
-            m1 = fmaxf(m1, fabsf(X[(size_t)(r + 4) * ld + c]));
-            m2 = fmaxf(m2, fabsf(X[(size_t)(r + 8) * ld + c]));
-            m3 = fmaxf(m3, fabsf(X[(size_t)(r + 12) * ld + c]));
+            m0 = fmaxf(m0, fabsf(xrow(X, rmap, r, ld, c)));
+            m1 = fmaxf(m1, fabsf(xrow(X, rmap, r + 4, ld, c)));
+            m2 = fmaxf(m2, fabsf(xrow(X, rmap, r + 8, ld, c)));
+            m3 = fmaxf(m3, fabsf(xrow(X, rmap, r + 12, ld, c)));
         }
-        for (; r < r1; r += 4) m0 = fmaxf(m0, fabsf(X[(size_t)r * ld + c]));
+        for (; r < r1; r += 4) m0 = fmaxf(m0, fabsf(xrow(X, rmap, r, ld, c)));
     }
     part[ty][cx] = fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
     __syncthreads();
@@ -122,15 +147,16 @@ __global__ void __launch_bounds__(256) colexp_kernel(const float* __restrict__ X
     }
 }
 
-// Q[s][rb][c][k] (int8): slice s of x[32 rb + k][c], k-contiguous per column so
-// that 16 bytes are one lane's MFMA operand.
-__global__ void quantize_kernel(const float* __restrict__ X, int ld, int rows_pad, int ncols,
-                                const int* __restrict__ E, signed char* __restrict__ Q) {
+// Q[s][rb][c][k] (int8): slice s of x[rmap[32 rb + k]][c], k-contiguous per
+// column so that 16 bytes are one lane's MFMA operand.
+__global__ void quantize_kernel(const float* __restrict__ X, const int* __restrict__ rmap, int ld,
+                                int rows_pad, int h0, int ncols, const int* __restrict__ E,
+                                signed char* __restrict__ Q) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int rb = blockIdx.y;
     if (c >= ncols) return;
     const int nrb = rows_pad / kQK;
-    const int e = E[(size_t)((rb * kQK) / kQChunk) * ld + c];
+    const int e = E[(size_t)chunk_of(rb * kQK, h0) * ld + c];
     const size_t plane = (size_t)nrb * ld * kQK;
     signed char* q = Q + ((size_t)rb * ld + c) * kQK;
     for (int k0 = 0; k0 < kQK; k0 += 16) {
@@ -139,7 +165,7 @@ __global__ void quantize_kernel(const float* __restrict__ X, int ld, int rows_pa
             v4i v;
         } d0, d1, d2;
         for (int k = 0; k < 16; ++k) {
-            const float x = X[(size_t)(rb * kQK + k0 + k) * ld + c];
+            const float x = xrow(X, rmap, rb * kQK + k0 + k, ld, c);
             const float m = ldexpf(x, 7 - e);  // |m| < 128
             const float t0 = truncf(m);
             const float r1 = (m - t0) * 128.f;  // exact
@@ -202,7 +228,7 @@ static long gram_i8_grid(int nbi) {
 size_t gram_i8_smem() { return 2 * (size_t)(GI_A + GI_B); }
 
 __global__ void __launch_bounds__(GI_NT) gram_i8_kernel(const signed char* __restrict__ Q, int ld,
-                                                        int rows_pad, int nbi,
+                                                        int rows_pad, int h0, int nbi,
                                                         const int* __restrict__ E,
                                                         double* __restrict__ C, int ldc) {
     extern __shared__ __attribute__((aligned(16))) signed char smem_i8[];
@@ -273,14 +299,15 @@ __global__ void __launch_bounds__(GI_NT) gram_i8_kernel(const signed char* __res
         for (int k = 0; k < 5; ++k) acc[k] = v16i{};
     };
     const int st_per_chunk = kQChunk / (kQK * GI_RB);
-    const int nchunk = (nst + st_per_chunk - 1) / st_per_chunk;
+    const int h0s = h0 / (kQK * GI_RB);  // stages of chunk 0
+    const int nchunk = nst <= h0s ? 1 : 1 + (nst - h0s + st_per_chunk - 1) / st_per_chunk;
     load(0);
     store(0);
     load(nst > 1 ? 1 : 0);
     __syncthreads();
     for (int ch = 0; ch < nchunk; ++ch) {
-        const int s1 = min(nst, (ch + 1) * st_per_chunk);
-        for (int st = ch * st_per_chunk; st < s1; ++st) {
+        const int s1 = min(nst, h0s + ch * st_per_chunk);
+        for (int st = ch == 0 ? 0 : h0s + (ch - 1) * st_per_chunk; st < s1; ++st) {
             const signed char* A = smem_i8 + (size_t)(st & 1) * (GI_A + GI_B);
             const signed char* B = A + GI_A;
             // lane -> column (lane % 32) of its block, k half (lane / 32)
@@ -361,9 +388,8 @@ void launch_set_column(hipStream_t st, bool fp64, void* X, int ld, int col, cons
                                                                    col, y, rows);
 }
 
-size_t gram_quant_bytes(int rows_pad, int ld) {
-    const size_t n_chunks = ((size_t)rows_pad + kQChunk - 1) / kQChunk;
-    return 3 * (size_t)rows_pad * ld + n_chunks * ld * sizeof(int);
+size_t gram_quant_bytes(int rows_pad, int h0, int ld) {
+    return 3 * (size_t)rows_pad * ld + (size_t)n_chunks_of(rows_pad, h0) * ld * sizeof(int);
 }
 
 void launch_gram_f64(hipStream_t st, const double* X, int ld, int rows_pad, int F_pad, double* C,
@@ -372,26 +398,31 @@ void launch_gram_f64(hipStream_t st, const double* X, int ld, int rows_pad, int 
     gram_f64_kernel<<<nb * (nb + 1) / 2, 256, 0, st>>>(X, ld, rows_pad, nb, C, ldc);
 }
 
-void launch_gram_quantize(hipStream_t st, const float* X, int ld, int rows_pad, int F_pad,
-                          void* quant) {
-    const int n_chunks = (rows_pad + kQChunk - 1) / kQChunk;
+int gram_chunk_rows() { return kQChunk; }
+
+void launch_gram_quantize(hipStream_t st, const float* X, const int* rmap, int ld, int rows_pad,
+                          int h0, int F_pad, void* quant) {
+    if (h0 < kQK * GI_RB || h0 > kQChunk || h0 % (kQK * GI_RB) || rows_pad % (kQK * GI_RB))
+        throw std::invalid_argument("quantisation chunk rows");
+    const int n_chunks = n_chunks_of(rows_pad, h0);
     signed char* Q = reinterpret_cast<signed char*>(quant);
     int* E = reinterpret_cast<int*>(Q + 3 * (size_t)rows_pad * ld);
     const int B = 256;
-    colexp_kernel<<<dim3((F_pad + 63) / 64, n_chunks), 256, 0, st>>>(X, ld, rows_pad, F_pad, E);
-    quantize_kernel<<<dim3((F_pad + B - 1) / B, rows_pad / kQK), B, 0, st>>>(X, ld, rows_pad,
-                                                                              F_pad, E, Q);
+    colexp_kernel<<<dim3((F_pad + 63) / 64, n_chunks), 256, 0, st>>>(X, rmap, ld, rows_pad, h0,
+                                                                        F_pad, E);
+    quantize_kernel<<<dim3((F_pad + B - 1) / B, rows_pad / kQK), B, 0, st>>>(
+        X, rmap, ld, rows_pad, h0, F_pad, E, Q);
 }
 
-void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int F_pad, const void* quant, double* C,
-                    int ldc) {
+void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int h0, int F_pad, const void* quant,
+                    double* C, int ldc) {
     const signed char* Q = reinterpret_cast<const signed char*>(quant);
     const int* E = reinterpret_cast<const int*>(Q + 3 * (size_t)rows_pad * ld);
     const int nbi = F_pad / GI_TI;
     const size_t sm = gram_i8_smem();
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&gram_i8_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    gram_i8_kernel<<<gram_i8_grid(nbi), GI_NT, sm, st>>>(Q, ld, rows_pad, nbi, E, C, ldc);
+    gram_i8_kernel<<<gram_i8_grid(nbi), GI_NT, sm, st>>>(Q, ld, rows_pad, h0, nbi, E, C, ldc);
 }
 
 }  // namespace plm

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "capi_internal.hpp"
 #include "profile.hpp"
@@ -46,6 +47,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
         const long mb = std::atol(e);
         if (mb > 0) x_budget = std::min(kXBudget, (size_t)mb << 20);
     }
+    std::vector<std::vector<int>> rmaps;  // row maps, alive until the stream is synchronised
     int s0 = 0;
     while (s0 < b.n_st) {
         int s1 = s0 + 1;
@@ -76,16 +78,37 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
             launch_gram_f64(st, reinterpret_cast<const double*>(acc.xbuf.ptr), ld, (int)rows_pad,
                             acc.F_pad, (double*)acc.xtx.ptr, acc.F_pad);
         } else {
-            acc.qbuf.alloc(gram_quant_bytes((int)rows_pad, ld));
+            // quantisation order: the energy and stress rows (7 per structure),
+            // then the force rows; the first exponent chunk is shortened so that
+            // the energy and stress rows end on a chunk boundary. An energy row
+            // sums a feature over the structure's atoms, so sharing a chunk with
+            // it would cost the force rows about log2(N) bits.
+            const long chunk = gram_chunk_rows();
+            const long nes = 7L * (s1 - s0), nes_pad = (nes + kstep - 1) / kstep * kstep;
+            const long h0 = nes_pad % chunk == 0 ? chunk : nes_pad % chunk;
+            const long rows_q = nes_pad + (rows - nes + kstep - 1) / kstep * kstep;
+            rmaps.emplace_back(rows_q, -1);
+            std::vector<int>& rm = rmaps.back();
+            long qe = 0, qf = nes_pad;
+            for (int s = s0; s < s1; ++s) {
+                const long base = b.row_off[s] - r0, n = b.row_off[s + 1] - b.row_off[s];
+                for (long k = 0; k < 7; ++k) rm[qe++] = (int)(base + k);
+                for (long k = 7; k < n; ++k) rm[qf++] = (int)(base + k);
+            }
+            acc.rbuf.alloc((size_t)rows_q * sizeof(int));
+            check_hip(hipMemcpyAsync(acc.rbuf.ptr, rm.data(), (size_t)rows_q * sizeof(int),
+                                     hipMemcpyHostToDevice, st), "copy");
+            acc.qbuf.alloc(gram_quant_bytes((int)rows_q, (int)h0, ld));
             {
                 ProfScope ps("quantize", st);
-                launch_gram_quantize(st, reinterpret_cast<const float*>(acc.xbuf.ptr), ld,
-                                     (int)rows_pad, acc.F_pad, acc.qbuf.ptr);
+                launch_gram_quantize(st, reinterpret_cast<const float*>(acc.xbuf.ptr),
+                                     (const int*)acc.rbuf.ptr, ld, (int)rows_q, (int)h0,
+                                     acc.F_pad, acc.qbuf.ptr);
             }
             check_hip(hipGetLastError(), "quantize launch");
             ProfScope ps("gram", st);
-            launch_gram_i8(st, ld, (int)rows_pad, acc.F_pad, acc.qbuf.ptr, (double*)acc.xtx.ptr,
-                           acc.F_pad);
+            launch_gram_i8(st, ld, (int)rows_q, (int)h0, acc.F_pad, acc.qbuf.ptr,
+                           (double*)acc.xtx.ptr, acc.F_pad);
         }
         check_hip(hipGetLastError(), "gram launch");
         acc.n_rows += rows;
