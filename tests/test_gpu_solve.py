@@ -149,3 +149,32 @@ def test_fit_cfg1_fp64_vs_oracle_and_mixed(tmp_path):
 
 def r64_min_e(ds):
     return mlp_dev.get_min_energy([ds])
+
+
+def test_fit_cfg2_mixed_predictions():
+    """Binary-alloy model (F = 13,020) fitted on 108 structures of the seeded
+    binary fcc alloy set: the mixed fit predicts the 12 test structures within
+    0.1 meV/atom and 1 meV/A of the FP64 fit, at each alpha."""
+    ds = datasets.config2_dataset(120)
+    train, test = ds.split(0.9)
+    m = capi.Model(datasets.config2_model().as_dict())
+    alphas = np.array([1e-3, 1e-1])
+    fits = {}
+    for prec in (capi.PLM_FP64, capi.PLM_MIXED):
+        say(f"cfg2 fit prec={prec}")
+        ne = mlp_dev.calc_xtx_xty(m, [train], precision=prec)
+        fits[prec] = mlp_dev.solver_ridge(ne.xtx, ne.xty, alphas) / ne.scales[:, None]
+    sts = test.structures
+    b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                   [s.types for s in sts])
+    n = np.array([s.n_total for s in sts])
+    for k, a in enumerate(alphas):
+        pred = {}
+        for prec, c in fits.items():
+            m.set_coeffs(c[:, k])
+            E, F, _ = capi.eval_batch(m, b, capi.PLM_FP64)
+            pred[prec] = (E / n, np.concatenate([f.reshape(-1) for f in F]))
+        de = np.abs(pred[capi.PLM_MIXED][0] - pred[capi.PLM_FP64][0]).max()
+        df = np.abs(pred[capi.PLM_MIXED][1] - pred[capi.PLM_FP64][1]).max()
+        say(f"  alpha {a:g}: max |dE| {de * 1e3:.3e} meV/atom, max |dF| {df * 1e3:.3e} meV/A")
+        assert de <= 1e-4 and df <= 1e-3
