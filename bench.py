@@ -106,6 +106,20 @@ def eval_leg(steps: int):
         out[f"eval_atoms_per_s_{name}"] = st.n_total / dt
         out[f"eval_ms_per_step_{name}"] = dt * 1e3
         log(f"eval {name}: {dt * 1e3:.2f} ms/step, {st.n_total / dt:.4g} atoms/s")
+        # the same evaluation as a captured HIP graph over the resident cell and
+        # neighbour lists, replayed (no list rebuild between steps)
+        batch = capi.Batch(model, [ax], [pos], [types])
+        graph = capi.EvalGraph(model, batch, prec)
+        graph.run(1)
+        n_rep = 10 * steps
+        t0 = time.perf_counter()
+        graph.run(n_rep)
+        dt = (time.perf_counter() - t0) / n_rep
+        out[f"eval_graph_atoms_per_s_{name}"] = st.n_total / dt
+        out[f"eval_graph_ms_per_step_{name}"] = dt * 1e3
+        log(f"eval graph {name}: {dt * 1e3:.2f} ms/step over {n_rep} replays, "
+            f"{st.n_total / dt:.4g} atoms/s")
+        del graph, batch
     e64, f64, s64 = res["fp64"]
     em, fm, sm = res["mixed"]
     out["eval_mixed_vs_fp64"] = {
@@ -114,7 +128,9 @@ def eval_leg(steps: int):
         "stress_max_rel": float(np.abs(sm - s64).max() / np.abs(s64).max()),
     }
     out["eval_workload"] = (f"{st.n_total}-atom bcc 20x20x40, a=3.30, sigma=0.1; cfg1 model "
-                            "(rc 6, 11 radial, gtinv order 2, max_l 4, max_p 2) fitted on the GPU")
+                            "(rc 6, 11 radial, gtinv order 2, max_l 4, max_p 2) fitted on the GPU; "
+                            "eval_*: neighbour list + evaluation per step; eval_graph_*: the "
+                            "evaluation kernels captured as a HIP graph and replayed")
     return out
 
 

@@ -34,6 +34,7 @@ extern "C" {
 typedef struct plm_model plm_model;
 typedef struct plm_batch plm_batch;
 typedef struct plm_normal plm_normal;
+typedef struct plm_eval_graph plm_eval_graph;
 
 /* Model description: the contents of the params dict the reference's
  * convert_params_dict_to_feature_params reads
@@ -175,6 +176,19 @@ int plm_eval(plm_model* m, const double* axis, const double* pos_c, const int* t
 /* Evaluation of a device-resident batch (eval_multiple, py_properties_fast.cpp:30-77). */
 int plm_eval_batch(plm_model* m, plm_batch* b, int precision, double* energies, double* forces,
                    double* stresses);
+
+/* Repeated evaluation of one device-resident batch, as in a molecular-dynamics
+ * or relaxation loop that calls PolymlpEval::eval on the same cells
+ * (src/cxx/src/polymlp_eval.cpp, reached from py_properties_fast.cpp:30-77):
+ * the evaluation kernels of plm_eval_batch captured once as a HIP graph, then
+ * replayed n_steps times per run call. The results are those of the last
+ * replay. The graph keeps the model's coefficients and the batch's neighbour
+ * lists as of creation: run fails with PLM_ERR_ARG after plm_model_set_coeffs,
+ * and the model and batch must outlive the graph. */
+int plm_eval_graph_create(plm_model* m, plm_batch* b, int precision, plm_eval_graph** out);
+int plm_eval_graph_run(plm_eval_graph* g, int n_steps, double* energies, double* forces,
+                       double* stresses);
+int plm_eval_graph_destroy(plm_eval_graph* g);
 
 /* Instrumentation (no counterpart in the reference): per-kernel-class timing
  * with HIP events recorded on the library's own stream, for roofline

@@ -27,7 +27,8 @@ EXPORTS = [
     "plm_batch_n_rows", "plm_batch_n_pairs", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
     "plm_solver_cg", "plm_model_feature_attrs",
-    "plm_eval", "plm_eval_batch", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
+    "plm_eval", "plm_eval_batch", "plm_eval_graph_create", "plm_eval_graph_run",
+    "plm_eval_graph_destroy", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
     "plm_surrogate_n_params", "plm_surrogate_train", "plm_surrogate_grad",
     "plm_surrogate_predict",
 ]
@@ -87,6 +88,9 @@ def lib():
         L.plm_ridge.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _ip]
         L.plm_eval.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.plm_eval_batch.argtypes = [vp, vp, ctypes.c_int, _dp, _dp, _dp]
+        L.plm_eval_graph_create.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.plm_eval_graph_run.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp]
+        L.plm_eval_graph_destroy.argtypes = [vp]
         L.plm_profile_enable.argtypes = [ctypes.c_int]
         L.plm_surrogate_n_params.argtypes = [_ip]
         L.plm_surrogate_train.argtypes = [_fp, _fp, _fp, ctypes.c_int, _fp, _fp, ctypes.c_int,
@@ -335,6 +339,35 @@ def eval_batch(model: Model, batch: Batch, precision: int = PLM_FP64):
     _check(lib().plm_eval_batch(model.handle, batch.handle, precision, _pd(e), _pd(f), _pd(s)))
     off = np.concatenate([[0], np.cumsum(batch.n_atoms)])
     return e, [f[off[k]:off[k + 1]] for k in range(n_st)], s
+
+
+class EvalGraph:
+    """Evaluation of one batch captured as a HIP graph (plm_eval_graph_*):
+    run(n) replays it n times and returns the last replay's results in the
+    form of eval_batch. Keep the model and batch alive while it is used; the
+    model's coefficients must not change after capture."""
+
+    def __init__(self, model: Model, batch: Batch, precision: int = PLM_FP64):
+        self._model, self._batch = model, batch
+        h = ctypes.c_void_p()
+        _check(lib().plm_eval_graph_create(model.handle, batch.handle, precision, ctypes.byref(h)))
+        self._h = h
+
+    def run(self, n_steps: int = 1):
+        b = self._batch
+        n_st = len(b.n_atoms)
+        A = int(b.n_atoms.sum())
+        e = np.zeros(n_st)
+        f = np.zeros((A, 3))
+        s = np.zeros((n_st, 6))
+        _check(lib().plm_eval_graph_run(self._h, int(n_steps), _pd(e), _pd(f), _pd(s)))
+        off = np.concatenate([[0], np.cumsum(b.n_atoms)])
+        return e, [f[off[k]:off[k + 1]] for k in range(n_st)], s
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.plm_eval_graph_destroy(self._h)
+            self._h = None
 
 
 def ridge(xtx, xty, alphas) -> np.ndarray:

@@ -1052,6 +1052,30 @@ int plm_eval_batch(plm_model* mp, plm_batch* bp, int precision, double* energies
     });
 }
 
+int plm_eval_graph_create(plm_model* mp, plm_batch* bp, int precision, plm_eval_graph** out) {
+    PLM_TRY({
+        if (!mp || !bp || !out) throw std::invalid_argument("null argument");
+        auto g = std::make_unique<EvalGraph>();
+        g->m = reinterpret_cast<ModelHandle*>(mp);
+        g->b = reinterpret_cast<BatchHandle*>(bp);
+        g->fp64 = precision == PLM_FP64;
+        eval_graph_capture(*g);
+        *out = reinterpret_cast<plm_eval_graph*>(g.release());
+    });
+}
+
+int plm_eval_graph_run(plm_eval_graph* gp, int n_steps, double* energies, double* forces,
+                       double* stresses) {
+    PLM_TRY({
+        if (!gp) throw std::invalid_argument("null graph");
+        eval_graph_run(*reinterpret_cast<EvalGraph*>(gp), n_steps, energies, forces, stresses);
+    });
+}
+
+int plm_eval_graph_destroy(plm_eval_graph* gp) {
+    PLM_TRY(delete reinterpret_cast<EvalGraph*>(gp));
+}
+
 int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
               double* coeffs, int* info) {
     PLM_TRY({ ridge_solve(F, xtx, xty, alphas, n_alpha, coeffs, info); });

@@ -93,6 +93,7 @@ struct ModelHandle {
     // evaluation tables
     Store pot_store;
     bool pot_uploaded = false;
+    unsigned pot_version = 0;  // counts uploads of the potential tables
     EvalDev ed{};
     int max_heads = 0, max_lin = 0;
     explicit ModelHandle(const ModelInput& in);
@@ -178,6 +179,27 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
                        bool fp64);
 
 void upload_potential(ModelHandle& m);
+
+// One evaluation of a device-resident batch (zero fill, order parameters,
+// per-atom contraction, half-pair loop) captured as a HIP graph and replayed.
+// The graph holds the model's potential tables and the batch's neighbour
+// lists as of capture; run() refuses a model whose coefficients changed since.
+struct EvalGraph {
+    ModelHandle* m = nullptr;
+    BatchHandle* b = nullptr;
+    bool fp64 = true;
+    unsigned pot_version = 0;
+    size_t n_out = 0;
+    DevBuf a_buf, pse, psf, out;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    EvalGraph() = default;
+    EvalGraph(const EvalGraph&) = delete;
+    EvalGraph& operator=(const EvalGraph&) = delete;
+    ~EvalGraph();
+};
+void eval_graph_capture(EvalGraph& g);
+void eval_graph_run(EvalGraph& g, int steps, double* energies, double* forces, double* stresses);
 void eval_single(ModelHandle& m, const double* axis, const double* pos_c, const int* types, int n,
                  bool fp64, double* energy, double* forces, double* stress);
 void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, double* forces,

@@ -2,6 +2,8 @@
 // list, order parameters, per-atom prod_sum and the half-pair loop.
 #include <algorithm>
 #include <cstring>
+#include <stdexcept>
+#include <vector>
 
 #include "capi_internal.hpp"
 #include "profile.hpp"
@@ -65,6 +67,7 @@ void upload_potential(ModelHandle& m) {
         E.hd_i = upv(hi), E.hd_j = upv(hj), E.hd_signj = upv(sj), E.hd_m0 = upv(m0);
     }
     m.pot_uploaded = true;
+    ++m.pot_version;
 }
 
 void BatchHandle::ensure_half() {
@@ -74,35 +77,37 @@ void BatchHandle::ensure_half() {
     has_half = true;
 }
 
-void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, double* forces,
-                double* stresses) {
-    if (!m.built) m.build_device();
-    if (!m.pot_uploaded) upload_potential(m);
-    b.ensure_half();
-    Device& dev = device();
-    hipStream_t st = dev.stream;
+// Work buffers of one evaluation; out = [energies | forces (3 per atom) | stresses (6)].
+static void alloc_eval(const ModelHandle& m, const BatchHandle& b, bool fp64, DevBuf& a_buf,
+                       DevBuf& pse, DevBuf& psf, DevBuf& out, size_t& n_out) {
     const int elem = fp64 ? 8 : 4;
-    const int A = b.n_atoms;
-    DevBuf a_buf, pse, psf, out;
-    a_buf.alloc((size_t)A * m.max_nlm * 2 * elem);
-    pse.alloc((size_t)A * std::max(1, m.max_heads) * 2 * elem);
-    psf.alloc((size_t)A * std::max(1, m.max_heads) * 2 * elem);
-    const size_t n_out = (size_t)b.n_st + 3 * (size_t)A + 6 * (size_t)b.n_st;
+    const size_t A = b.n_atoms;
+    a_buf.alloc(A * m.max_nlm * 2 * elem);
+    pse.alloc(A * std::max(1, m.max_heads) * 2 * elem);
+    psf.alloc(A * std::max(1, m.max_heads) * 2 * elem);
+    n_out = (size_t)b.n_st + 3 * A + 6 * (size_t)b.n_st;
     out.alloc(n_out * 8);
-    check_hip(hipMemsetAsync(out.ptr, 0, n_out * 8, st), "memset");
-    double* d_e = (double*)out.ptr;
+}
+
+// The kernels of one evaluation on st: out is cleared by a kernel (no memset
+// node when captured), then filled by the half-pair loop.
+static void enqueue_eval(ModelHandle& m, BatchHandle& b, bool fp64, void* a_buf, void* pse,
+                         void* psf, double* out, size_t n_out, hipStream_t st) {
+    const int A = b.n_atoms;
+    launch_fill_zero(st, out, n_out);
+    double* d_e = out;
     double* d_f = d_e + b.n_st;
     double* d_s = d_f + 3 * (size_t)A;
     BatchDev bd = b.view();
     const int pair_model = m.host.input().gtinv ? 0 : 1;
     {
         ProfScope ps("eval_anlm", st);
-        launch_anlm(st, fp64, m.md, bd, a_buf.ptr, m.max_nlm, 0, A, m.max_noconj, 1);
+        launch_anlm(st, fp64, m.md, bd, a_buf, m.max_nlm, 0, A, m.max_noconj, 1);
     }
     check_hip(hipGetLastError(), "eval anlm launch");
     {
         ProfScope ps("eval_prodsum", st);
-        launch_prodsum(st, fp64, m.md, m.ed, bd, a_buf.ptr, m.max_nlm, pse.ptr, psf.ptr,
+        launch_prodsum(st, fp64, m.md, m.ed, bd, a_buf, m.max_nlm, pse, psf,
                        std::max(1, m.max_heads), m.max_lin, pair_model);
     }
     check_hip(hipGetLastError(), "prodsum launch");
@@ -110,16 +115,79 @@ void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, dou
         ProfScope ps("eval_pair", st);
         launch_pair(st, fp64, m.md, m.ed, bd, (const int*)b.d_atom_st.ptr,
                     (const int*)b.d_hb_off.ptr, (const int*)b.d_hb_j.ptr,
-                    (const double*)b.d_hb_d.ptr, pse.ptr, psf.ptr, std::max(1, m.max_heads), d_e,
-                    d_f, d_s, pair_model);
+                    (const double*)b.d_hb_d.ptr, pse, psf, std::max(1, m.max_heads), d_e, d_f,
+                    d_s, pair_model);
     }
     check_hip(hipGetLastError(), "pair launch");
+}
+
+static void copy_out(const BatchHandle& b, const DevBuf& out, size_t n_out, hipStream_t st,
+                     double* energies, double* forces, double* stresses) {
+    const size_t A = b.n_atoms;
     std::vector<double> h(n_out);
     check_hip(hipMemcpyAsync(h.data(), out.ptr, n_out * 8, hipMemcpyDeviceToHost, st), "copy");
     check_hip(hipStreamSynchronize(st), "sync");
     if (energies) std::memcpy(energies, h.data(), (size_t)b.n_st * 8);
-    if (forces) std::memcpy(forces, h.data() + b.n_st, 3 * (size_t)A * 8);
-    if (stresses) std::memcpy(stresses, h.data() + b.n_st + 3 * (size_t)A, 6 * (size_t)b.n_st * 8);
+    if (forces) std::memcpy(forces, h.data() + b.n_st, 3 * A * 8);
+    if (stresses) std::memcpy(stresses, h.data() + b.n_st + 3 * A, 6 * (size_t)b.n_st * 8);
+}
+
+static void prepare_eval(ModelHandle& m, BatchHandle& b) {
+    if (!m.built) m.build_device();
+    if (!m.pot_uploaded) upload_potential(m);
+    b.ensure_half();
+}
+
+void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, double* forces,
+                double* stresses) {
+    prepare_eval(m, b);
+    hipStream_t st = device().stream;
+    DevBuf a_buf, pse, psf, out;
+    size_t n_out = 0;
+    alloc_eval(m, b, fp64, a_buf, pse, psf, out, n_out);
+    enqueue_eval(m, b, fp64, a_buf.ptr, pse.ptr, psf.ptr, (double*)out.ptr, n_out, device().stream);
+    copy_out(b, out, n_out, st, energies, forces, stresses);
+}
+
+EvalGraph::~EvalGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+}
+
+void eval_graph_capture(EvalGraph& g) {
+    ModelHandle& m = *g.m;
+    BatchHandle& b = *g.b;
+    prepare_eval(m, b);
+    hipStream_t st = device().stream;
+    alloc_eval(m, b, g.fp64, g.a_buf, g.pse, g.psf, g.out, g.n_out);
+    // one eager pass first: kernel attributes are set and every buffer exists
+    enqueue_eval(m, b, g.fp64, g.a_buf.ptr, g.pse.ptr, g.psf.ptr, (double*)g.out.ptr, g.n_out, st);
+    check_hip(hipStreamSynchronize(st), "sync");
+    {
+        ProfPause pause;
+        check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            enqueue_eval(m, b, g.fp64, g.a_buf.ptr, g.pse.ptr, g.psf.ptr, (double*)g.out.ptr,
+                         g.n_out, st);
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(st, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        check_hip(hipStreamEndCapture(st, &g.graph), "end capture");
+    }
+    check_hip(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "graph instantiate");
+    g.pot_version = m.pot_version;
+}
+
+void eval_graph_run(EvalGraph& g, int steps, double* energies, double* forces, double* stresses) {
+    if (steps < 1) throw std::invalid_argument("steps must be positive");
+    if (!g.m->pot_uploaded || g.m->pot_version != g.pot_version)
+        throw std::invalid_argument("model coefficients changed since the graph was captured");
+    hipStream_t st = device().stream;
+    for (int k = 0; k < steps; ++k) check_hip(hipGraphLaunch(g.exec, st), "graph launch");
+    copy_out(*g.b, g.out, g.n_out, st, energies, forces, stresses);
 }
 
 void eval_single(ModelHandle& m, const double* axis, const double* pos_c, const int* types, int n,

@@ -123,6 +123,8 @@ struct EvalDev;
 void launch_prodsum(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,
                     const BatchDev& bd, const void* a_full, int a_stride, void* ps_e, void* ps_f,
                     int ps_stride, int max_lin, int pair_model);
+// x[0, n) = 0 by a kernel (a graph of it replays correctly; a memset node is avoided)
+void launch_fill_zero(hipStream_t st, double* x, size_t n);
 void launch_pair(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,
                  const BatchDev& bd, const int* atom_st, const int* hb_off, const int* hb_j,
                  const double* hb_d, const void* ps_e, const void* ps_f, int ps_stride,

@@ -9,6 +9,8 @@
 //   collect_properties                        polymlp_eval.cpp:293-335
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_math.hpp"
 #include "device_tables.hpp"
 #include "eval_tables.hpp"
@@ -244,6 +246,18 @@ void launch_pair(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& e
     else
         launch_pair_t<float>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, ps_e, ps_f, ps_stride,
                              energy, forces, stress, pair_model);
+}
+
+__global__ void __launch_bounds__(256) fill_zero_kernel(double* __restrict__ x, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x)
+        x[k] = 0.0;
+}
+
+void launch_fill_zero(hipStream_t st, double* x, size_t n) {
+    if (n == 0) return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    fill_zero_kernel<<<(unsigned)blocks, 256, 0, st>>>(x, n);
 }
 
 }  // namespace plm

@@ -40,6 +40,17 @@ Profiler& prof() {
 
 bool profiling_on() { return prof().on; }
 
+ProfPause::ProfPause() {
+    std::lock_guard<std::mutex> g(prof().mu);
+    was_ = prof().on;
+    prof().on = false;
+}
+
+ProfPause::~ProfPause() {
+    std::lock_guard<std::mutex> g(prof().mu);
+    prof().on = was_;
+}
+
 ProfScope::ProfScope(const char* name, hipStream_t st) : name_(name), st_(st) {
     if (!prof().on) return;
     (void)hipEventCreate(&a_);

@@ -8,6 +8,16 @@
 namespace plm {
 
 bool profiling_on();
+// Switches the event pairs off for its lifetime (regions captured into a graph).
+struct ProfPause {
+    ProfPause();
+    ~ProfPause();
+    ProfPause(const ProfPause&) = delete;
+    ProfPause& operator=(const ProfPause&) = delete;
+
+  private:
+    bool was_;
+};
 // Records an event pair around a region on `st` when profiling is enabled.
 struct ProfScope {
     ProfScope(const char* name, hipStream_t st);

@@ -156,3 +156,33 @@ def test_eval_skewed_cells_mgo_golden():
     say(f"eval skewed cells n={len(sts)}")
     E, _, _ = capi.eval_batch(m, b, capi.PLM_FP64)
     np.testing.assert_allclose(E, MGO_HNF_ENERGY, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [capi.PLM_FP64, capi.PLM_MIXED])
+def test_eval_graph_replays(prec):
+    """The evaluation captured as a HIP graph and replayed (1, 2 and 3 times per
+    run, with eager evaluations and synchronisations in between) gives the
+    eager evaluation's energies, forces and stresses on every run, and the
+    golden MgO energy; a coefficient change after capture is refused."""
+    spec, coeffs = load_model("polymlp.yaml.gtinv.MgO")
+    m = capi.Model(spec.as_dict())
+    m.set_coeffs(coeffs)
+    sts = [poscar("POSCAR.RS.MgO"), poscar("POSCAR.RS.idealMgO")]
+    b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                   [s.types for s in sts])
+    say(f"eval graph prec={prec}")
+    e0, f0, s0 = capi.eval_batch(m, b, prec)
+    g = capi.EvalGraph(m, b, prec)
+    for n in (1, 2, 3):
+        e, f, s = g.run(n)
+        say(f"  run({n}) energies {e}")
+        np.testing.assert_allclose(e, e0, rtol=1e-12)
+        for fk, f0k in zip(f, f0):
+            np.testing.assert_allclose(fk, f0k, rtol=0, atol=1e-12 * max(1.0, np.abs(f0k).max()))
+        np.testing.assert_allclose(s, s0, rtol=0, atol=1e-12 * np.abs(s0).max())
+        e1, _, _ = capi.eval_batch(m, b, prec)
+        np.testing.assert_allclose(e1, e0, rtol=1e-12)
+    assert e[0] == pytest.approx(-40.223320043232334, rel=1e-8 if prec == capi.PLM_FP64 else 1e-6)
+    m.set_coeffs(coeffs)
+    with pytest.raises(RuntimeError, match="coefficients changed"):
+        g.run(1)
