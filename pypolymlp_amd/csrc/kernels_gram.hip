@@ -159,13 +159,22 @@ __global__ void quantize_kernel(const float* __restrict__ X, const int* __restri
     const int e = E[(size_t)chunk_of(rb * kQK, h0) * ld + c];
     const size_t plane = (size_t)nrb * ld * kQK;
     signed char* q = Q + ((size_t)rb * ld + c) * kQK;
+    // the block's 32 row indices (uniform) first, then all 32 loads of X in flight
+    int rr[kQK];
+#pragma unroll
+    for (int k = 0; k < kQK; ++k) rr[k] = rmap[rb * kQK + k];
+    float xv[kQK];
+#pragma unroll
+    for (int k = 0; k < kQK; ++k) xv[k] = rr[k] >= 0 ? X[(size_t)rr[k] * ld + c] : 0.f;
+#pragma unroll
     for (int k0 = 0; k0 < kQK; k0 += 16) {
         union {
             signed char b[16];
             v4i v;
         } d0, d1, d2;
+#pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const float x = xrow(X, rmap, rb * kQK + k0 + k, ld, c);
+            const float x = xv[k0 + k];
             const float m = ldexpf(x, 7 - e);  // |m| < 128
             const float t0 = truncf(m);
             const float r1 = (m - t0) * 128.f;  // exact
