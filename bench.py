@@ -134,6 +134,41 @@ def eval_leg(steps: int):
     return out
 
 
+def fit_accuracy_leg(n_st: int):
+    """Mixed against FP64 fit on a seeded subset of the same binary-alloy
+    workload (with Morse targets): both normal-equation forms, the GPU ridge
+    solve at alpha = 1e-3 and 1e-1, and FP64 evaluation of the held-out 10 %.
+    Returns the largest energy (meV/atom) and force (meV/A) differences."""
+    from pypolymlp_amd import capi, datasets, mlp_dev
+
+    ds = datasets.config2_dataset(n_st)
+    train, test = ds.split(0.9)
+    m = capi.Model(datasets.config2_model().as_dict())
+    alphas = np.array([1e-3, 1e-1])
+    fits = {}
+    for prec in (capi.PLM_FP64, capi.PLM_MIXED):
+        ne = mlp_dev.calc_xtx_xty(m, [train], precision=prec)
+        fits[prec] = mlp_dev.solver_ridge(ne.xtx, ne.xty, alphas) / ne.scales[:, None]
+    sts = test.structures
+    b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                   [s.types for s in sts])
+    n = np.array([s.n_total for s in sts])
+    de = df = 0.0
+    for k in range(len(alphas)):
+        pred = {}
+        for prec, c in fits.items():
+            m.set_coeffs(c[:, k])
+            E, F, _ = capi.eval_batch(m, b, capi.PLM_FP64)
+            pred[prec] = (E / n, np.concatenate([f.reshape(-1) for f in F]))
+        de = max(de, float(np.abs(pred[capi.PLM_MIXED][0] - pred[capi.PLM_FP64][0]).max()))
+        df = max(df, float(np.abs(pred[capi.PLM_MIXED][1] - pred[capi.PLM_FP64][1]).max()))
+    log(f"fit accuracy: mixed vs FP64 max |dE| {de * 1e3:.2e} meV/atom, max |dF| {df * 1e3:.2e} meV/A")
+    return {"fit_mixed_vs_fp64": {
+        "energy_max_meV_per_atom": de * 1e3, "force_max_meV_per_A": df * 1e3,
+        "sample": f"{n_st} structures of the workload with Morse targets, 90/10 split, "
+                  "alpha 1e-3 and 1e-1, held-out predictions"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +179,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eval-steps", type=int, default=5)
+    ap.add_argument("--fit-sample", type=int, default=120,
+                    help="structures of the mixed-vs-FP64 fit accuracy check (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three plain 1 GiB device copies (HBM counter calibration)")
     args = ap.parse_args()
@@ -252,6 +289,8 @@ def main():
             roof["traffic_source"] = "profiles/fit_binary_alloy_traffic.json (rocprofv3 --pmc)"
 
     ev = eval_leg(args.eval_steps) if args.eval_steps > 0 else {}
+    if args.fit_sample > 0:
+        ev.update(fit_accuracy_leg(args.fit_sample))
 
     cpu = None
     if not args.no_cpu_baseline:
