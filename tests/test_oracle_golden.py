@@ -242,3 +242,102 @@ def test_oracle_skewed_cells_mgo():
         s2 = supercell(st, h)
         e, _, _ = r.eval(s2.axis, s2.positions_cartesian, s2.types)
         assert e == pytest.approx(MGO_HNF_ENERGY, abs=1e-12), h
+
+
+# --- order-3 polynomial rows (max_p 3), the one part of X that no reference
+# golden model exercises: every golden model has max_p <= 2 ----------------
+def _poly_blocks(spec):
+    """Column ranges [linear | comb2 | comb3] of a model's X."""
+    from pypolymlp_amd import capi
+    radial, _, _, polys, _ = capi.feature_attrs(capi.Model(spec.as_dict()))
+    nl = len(radial)
+    n2 = sum(len(p) == 2 for p in polys)
+    n3 = sum(len(p) == 3 for p in polys)
+    return nl, nl + n2, nl + n2 + n3
+
+
+@pytest.fixture(scope="module")
+def cfg2_order3(tmp_path_factory):
+    """The binary-alloy model (gtinv order 3, max_l (8, 4), max_p 3, model_type 3)
+    written with coefficients c and unit scales, plus two structures: a
+    128-atom cell of the workload and a displaced 16-atom fcc cell."""
+    from pypolymlp_amd import datasets
+    from pypolymlp_amd.datasets import _fcc_supercell
+    spec = datasets.config2_model()
+    nl, e2, e3 = _poly_blocks(spec)
+    rng = np.random.default_rng(31)
+    F = e3
+    c = rng.normal(size=F) / np.sqrt(F)
+    path = str(tmp_path_factory.mktemp("o3") / "cfg2.yaml")
+    from pypolymlp_amd.params import save_mlp_yaml
+    save_mlp_yaml(spec, c, np.ones(F), path)
+    big = datasets.config2_dataset(1, seed=2, targets=False).structures[0]
+    axis, frac = _fcc_supercell(3.9, (2, 2, 1))
+    axis = axis * 1.01
+    small = (axis, axis @ frac + rng.normal(0.0, 0.1, frac.shape),
+             rng.integers(0, 2, frac.shape[1]))
+    return spec, path, c, (nl, e2, e3), [(big.axis, big.positions_cartesian, big.types), small]
+
+
+def test_oracle_order3_rows_match_compiled_eval(cfg2_order3):
+    """X_e.c = E, X_f.c = F and X_s.c = S of the compiled PolymlpEval
+    (polymlp_eval.cpp:24-44, which folds the same coefficients through
+    Potential, polymlp_potential.cpp:25-156, and never uses the restated
+    model.cpp:237-267 rows). Checked with c on every column and with c on
+    the comb2 or comb3 columns only, so that a dropped or misplaced term of
+    the order-3 block cannot hide behind the linear columns."""
+    spec, path, c, (nl, e2, e3), structures = cfg2_order3
+    assert e3 - e2 == 9240 and e2 - nl == 810 and nl == 2970
+    from pypolymlp_amd.params import save_mlp_yaml
+    masks = {"all": np.ones(e3, bool), "comb2": np.zeros(e3, bool), "comb3": np.zeros(e3, bool)}
+    masks["comb2"][nl:e2] = True
+    masks["comb3"][e2:e3] = True
+    rows = None
+    for name, mask in masks.items():
+        cm = np.where(mask, c, 0.0)
+        p = path.replace(".yaml", f".{name}.yaml")
+        save_mlp_yaml(spec, cm, np.ones(e3), p)
+        R = ref.RefModel(p, with_eval=True)
+        if rows is None:
+            rows = [R.structure_rows(*s) for s in structures]
+        for (axis, pos, types), (xe, xs, xf) in zip(structures, rows):
+            e, f, s = R.eval(axis, pos, types)
+            assert abs(xe @ cm - e) <= 1e-10 * max(abs(e), 1e-3), name
+            f = np.asarray(f).reshape(-1)
+            assert np.linalg.norm(xf @ cm - f) <= 1e-10 * np.linalg.norm(f), name
+            assert np.linalg.norm(xs @ cm - s) <= 1e-10 * np.linalg.norm(s), name
+
+
+def test_oracle_order3_rows_finite_differences(cfg2_order3):
+    """Force rows are -d(energy row)/dr and stress rows -d(energy row)/d(strain)
+    (symmetric strain, shear split evenly), by central differences of the
+    oracle's energy-only rows, per column block (linear, comb2, comb3)."""
+    _, path, _, blocks, structures = cfg2_order3
+    R = ref.RefModel(path)
+    axis, pos, types = structures[1]
+    xe, xs, xf = R.structure_rows(axis, pos, types)
+
+    def e_row(ax, ps):
+        return R.x_batch([ax], [ps], [types], force=False)[0]
+
+    h = 1e-4
+    nl, e2, e3 = blocks
+    spans = [(0, nl), (nl, e2), (e2, e3)]
+
+    def check(fd, row):
+        for a, b in spans:
+            sc = np.abs(row[a:b]).max()
+            assert sc > 0
+            assert np.abs(fd[a:b] + row[a:b]).max() <= 1e-6 * sc, (a, b)
+
+    for k, a in [(0, 0), (3, 1), (7, 2), (12, 0)]:
+        pp, pm = pos.copy(), pos.copy()
+        pp[a, k] += h
+        pm[a, k] -= h
+        check((e_row(axis, pp) - e_row(axis, pm)) / (2 * h), xf[3 * k + a])
+    for idx, (i, j) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)]):
+        E = np.zeros((3, 3))
+        E[i, j] += h / 2
+        E[j, i] += h / 2
+        Mp, Mm = np.eye(3) + E, np.eye(3) - E
+        check((e_row(Mp @ axis, Mp @ pos) - e_row(Mm @ axis, Mm @ pos)) / (2 * h), xs[idx])

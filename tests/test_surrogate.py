@@ -48,6 +48,28 @@ def test_oracle_gradient_finite_differences():
     assert oracle_net.n_params() == len(p) == 4674
 
 
+def test_oracle_adam_first_step_closed_form():
+    """With zero moments, the bias-corrected first Adam step is
+    m_hat = g and v_hat = g^2, so every parameter moves by
+    lr * (g / (|g| + eps) + wd * p): about lr * sign(g) wherever |g| >> eps."""
+    p = _init(7)
+    x, y = _data(16, seed=8)
+    _, g = oracle_net.loss_grad(p, x, y)
+    lr, wd, eps = 1e-3, 1e-2, 1e-8
+    p1, m1, v1, _ = oracle_net.adam(p, np.zeros_like(p), np.zeros_like(p), 0, x, y, 1, lr,
+                                    eps=eps, wd=wd)
+    np.testing.assert_allclose(p1, p - lr * (g / (np.abs(g) + eps) + wd * p), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(m1, 0.1 * g, rtol=1e-15)
+    np.testing.assert_allclose(v1, 1e-3 * g * g, rtol=1e-12)
+    big = np.abs(g) > 1e-4
+    np.testing.assert_allclose((p - p1)[big] - lr * wd * p[big], lr * np.sign(g[big]), rtol=1e-4)
+    # the step count continues from t0: with t0 = 9 the first update uses the
+    # bias corrections 1 - 0.9^10 and 1 - 0.999^10
+    p9, _, _, _ = oracle_net.adam(p, np.zeros_like(p), np.zeros_like(p), 9, x, y, 1, lr, eps=eps)
+    mh, vh = 0.1 * g / (1 - 0.9 ** 10), 1e-3 * g * g / (1 - 0.999 ** 10)
+    np.testing.assert_allclose(p9, p - lr * mh / (np.sqrt(vh) + eps), rtol=0, atol=1e-14)
+
+
 @pytest.mark.gpu
 def test_surrogate_forward_and_gradient_vs_oracle():
     assert capi.surrogate_n_params() == oracle_net.n_params()
