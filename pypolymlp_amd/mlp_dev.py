@@ -75,14 +75,31 @@ def row_weights(ds: Dataset, min_e: float, weight_stress: float = 0.1):
         wf = _weight_force(forces) * ds.weight
         w[ne + n_s:] = wf
         y[ne + n_s:] = wf * forces
+    else:
+        # energy-only dataset: the reference builds no force or stress rows for
+        # it (Features, features.py:94-133), so they carry no weight here
+        w[ne:] = 0.0
+        y[ne:] = 0.0
+        return w, y
     if ds.include_stress:
         ws = _weight_stress(stresses, weight_stress * ds.weight)
         w[ne:ne + n_s] = ws
         y[ne:ne + n_s] = ws * stresses
-    elif ds.include_force:
+    else:
         w[ne:ne + n_s] = 0.0
         y[ne:ne + n_s] = 0.0
     return w, y
+
+
+def reference_n_rows(ds: Dataset, n_structures: Optional[int] = None) -> int:
+    """Rows the reference's X holds for the first n_structures of ds: energy
+    rows, plus 6 stress and 3N force rows per structure when the dataset has
+    forces (PyModel::set_index, py_model.cpp:58-106); the RMSE denominator
+    total_n_data of data_sequential.py:155."""
+    n = len(ds.structures) if n_structures is None else n_structures
+    if not ds.include_force:
+        return n
+    return n + 6 * n + 3 * int(ds.total_n_atoms[:n].sum())
 
 
 # --- scales (utils_scales.py) ------------------------------------------------
@@ -130,7 +147,8 @@ def calc_xtx_xty(model: capi.Model, datasets: Sequence[Dataset], precision=capi.
                     batch_size, precision)
     n_data = sum(len(d.energies) for d in datasets)
     return _finish_normal(acc.result(), n_data, scales, datasets[0].include_force,
-                          scale_threshold, min_energy)
+                          scale_threshold, min_energy,
+                          n_rows=sum(reference_n_rows(d) for d in datasets))
 
 
 def _accumulate(model, acc, ds: Dataset, i0: int, i1: int, min_energy, weight_stress,
@@ -146,9 +164,11 @@ def _accumulate(model, acc, ds: Dataset, i0: int, i1: int, min_energy, weight_st
 
 
 def _finish_normal(res, n_data, scales, include_force, scale_threshold, min_energy,
-                   ) -> NormalEquations:
-    """Scale the raw sums: zero-scale columns dropped, xtx / (s s^T), xty / s."""
-    xtx, xty, xe_sum, xe_sq, y_sq, n_rows = res
+                   n_rows: int) -> NormalEquations:
+    """Scale the raw sums: zero-scale columns dropped, xtx / (s s^T), xty / s.
+    n_rows: the reference's row count of the accumulated structures (the
+    device batch also carries force rows of energy-only datasets, at weight 0)."""
+    xtx, xty, xe_sum, xe_sq, y_sq, _ = res
     scales, zero = compute_scales(None if scales is None else np.array(scales, dtype=float),
                                   xe_sum, xe_sq, n_data, include_force=include_force,
                                   threshold=scale_threshold)
@@ -385,13 +405,15 @@ def fit_learning_curve(model: capi.Model, train: List[Dataset], test: List[Datas
         done = n
         prefixes.append((n, acc.result()))
     _accumulate(model, acc, ds, done, n_train, min_e, weight_stress, 4096, precision)
-    full = _finish_normal(acc.result(), n_train, None, ds.include_force, scale_threshold, min_e)
+    full = _finish_normal(acc.result(), n_train, None, ds.include_force, scale_threshold, min_e,
+                          n_rows=reference_n_rows(ds))
     scales = full.scales
     te = calc_xtx_xty(model, test, precision=precision, scales=scales.copy(), min_energy=min_e,
                       weight_stress=weight_stress, scale_threshold=scale_threshold)
     log = []
     for n, res in prefixes:
-        ne = _finish_normal(res, n, scales, ds.include_force, scale_threshold, min_e)
+        ne = _finish_normal(res, n, scales, ds.include_force, scale_threshold, min_e,
+                            n_rows=reference_n_rows(ds, n))
         coefs = solver_ridge(ne.xtx, ne.xty, alphas)
         rmse_train = compute_rmse(coefs, ne)
         rmse_test = compute_rmse(coefs, te)

@@ -180,6 +180,14 @@ def load_mlp_yaml(filename: str):
     yml = yaml.safe_load(open(filename))
     elements = list(yml["elements"])
     n_type = len(elements)
+    # spin-resolved models split each spin-enabled element into two atom types
+    # (PolymlpParams._set_params_spins, data_format.py:291-320) and need spin
+    # labels on the atoms; this build does not implement them, so refuse them
+    # rather than evaluate them with the element types
+    spins = [int(v) for v in yml.get("enable_spins", [0] * n_type) or []]
+    if any(spins):
+        raise ValueError(f"{filename}: spin-resolved models (enable_spins {spins}) are not "
+                         "supported")
     cond = {tuple(tp["atom_type_pair"]): list(tp["pair_params_indices"]) for tp in yml["type_pairs"]}
     ft = yml["feature_type"]
     spec = ModelSpec(

@@ -323,3 +323,17 @@ def test_vasprun_reader_golden(tmp_path):
     assert train[0].energies[0] == pytest.approx(-381.0917812 + 32 * 0.0004 + 32 * 1.85321219)
     assert train[0].stresses[1].tolist() == pytest.approx(
         [0.31962655, 0.11339135, 0.23225083, -0.08733854, 0.0465711, 0.00825789], abs=1e-6)
+
+
+def test_spin_models_are_refused():
+    """A spin-resolved model (enable_spins [1], the reference's Fe fixture
+    tests/test_calc/files/mlps/polymlp.yaml.2.Fe) is refused with an error
+    instead of being evaluated with the element types."""
+    from pypolymlp_amd.params import load_mlp, load_mlp_yaml
+    from tests.helpers import ref_file
+    for loader in (load_mlp_yaml, load_mlp):
+        with pytest.raises(ValueError, match="spin"):
+            loader(ref_file("polymlp.yaml.2.Fe"))
+    # models without spins (enable_spins [0, 0]) still load
+    spec, coeffs = load_mlp_yaml(ref_file("polymlp.yaml.gtinv.MgO"))
+    assert len(coeffs) == 1899
