@@ -19,6 +19,12 @@ events on the library stream (plm_profile_*).
 The CPU baseline times the reference's own C++ (compiled from its sources into
 oracle/_ref) building X for a sample of structures with OpenMP over
 structures, plus numpy X^T X, on the host cores of this machine.
+
+Further legs in the same JSON line: PolymlpEval on a 32,000-atom cell (eager
+steps with the neighbour lists rebuilt, and HIP-graph replays) with the
+pair-kernel roofline and the reference's CPU evaluation; the mixed-vs-FP64
+fit accuracy on a sample; and the cold end-to-end fit (calc_xtx_xty +
+solver_ridge) in both precisions.
 """
 
 from __future__ import annotations
@@ -43,6 +49,33 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_info():
+    """CPU model, usable cores and NUMA nodes of this machine."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nodes = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node")
+                     and d[4:].isdigit()])
+    except OSError:
+        nodes = None
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "usable_cores": usable, "numa_nodes": nodes}
+
+
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
 def cpu_baseline(spec, structures, n_sample):
     """Reference C++ X build (OpenMP over structures) + numpy X^T X on a sample."""
     from oracle import ref
@@ -65,24 +98,62 @@ def cpu_baseline(spec, structures, n_sample):
         t2 = time.perf_counter()
         del G
     rows = X.shape[0]
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {
+    out = {
         "value": rows / (t2 - t0),
         "unit": "rows/s",
-        "cores": cores,
+        "cores": cpu_threads(),
         "kind": "reference",
-        "sample": f"{n_sample} structures ({rows} rows): reference C++ X build "
-                  f"{t1 - t0:.2f} s + numpy X^T X {t2 - t1:.2f} s",
+        "sample": f"{n_sample} structures ({rows} rows): reference C++ X build (OpenMP over "
+                  f"structures) {t1 - t0:.2f} s + numpy X^T X {t2 - t1:.2f} s",
     }
+    out.update(host_info())
+    return out
 
 
-def eval_leg(steps: int):
+def eval_cpu_baseline(model_yaml, st, steps=1):
+    """The reference's PolymlpEval (compiled from its sources into oracle/_ref):
+    NeighborHalf and the evaluation with OpenMP over atoms, on the 32,000-atom
+    cell; returns atoms/s and the per-phase split of one step."""
+    from oracle import ref
+
+    if not ref.available():
+        return None
+    R = ref.RefModel(model_yaml, with_eval=True)
+    ax, pos, types = st.axis, st.positions_cartesian, st.types
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R.eval(ax, pos, types, use_openmp=True)
+    dt = (time.perf_counter() - t0) / steps
+    t1 = time.perf_counter()
+    ref.neighbor_list(ax, pos, 6.0, half=True)
+    t_nb = time.perf_counter() - t1
+    out = {"value": st.n_total / dt, "unit": "atoms/s", "cores": cpu_threads(),
+           "kind": "reference",
+           "sample": f"{steps} step(s) of the {st.n_total}-atom cell: {dt:.2f} s per step, of "
+                     f"which the half neighbour list (timed alone) {t_nb:.2f} s"}
+    out.update(host_info())
+    return out
+
+
+FP32_PEAK_TFLOPS = 157.3
+
+
+def eval_leg(steps: int, graph_steps: int, cpu: bool):
     """PolymlpEval on the 32,000-atom bcc cell (20 x 20 x 40, a = 3.30 A,
     sigma = 0.1 A) with the coefficients of a GPU FP64 ridge fit (alpha = 1e-3)
-    of the 200-structure bcc dataset. One step = neighbour list (cell list),
-    order parameters, per-atom term contraction and the half-pair loop for
-    energy, forces and stress, with the positions already resident."""
+    of the 200-structure bcc dataset. One eager step = neighbour lists (cell
+    list), order parameters, per-atom term contraction and the half-pair loop
+    for energy, forces and stress, with the positions already resident; the
+    graph steps replay the evaluation kernels over the resident lists.
+
+    Roofline of the dominant evaluation kernel, pair_kernel: algorithmic
+    flops = 40 n_fn n_lm_half per half pair (radial functions, Y_lm and their
+    gradients, and the contraction with both atoms' prod_sum), against the
+    vector peak of the precision it computes in (FP64 78.6, FP32 157.3 TFLOP/s)."""
+    import tempfile
+
     from pypolymlp_amd import capi, datasets, mlp_dev
+    from pypolymlp_amd.params import save_mlp_yaml
 
     spec = datasets.config1_model()
     ds = datasets.config1_dataset(200)
@@ -92,6 +163,8 @@ def eval_leg(steps: int):
     model.set_coeffs(coef / ne.scales)
     st = datasets.config4_structure()
     ax, pos, types = st.axis, st.positions_cartesian, st.types
+    n_fn = len(spec.pair_params)
+    n_lmh = (spec.max_l + 1) * (spec.max_l + 2) // 2
     out = {}
     res = {}
     for name, prec in (("fp64", capi.PLM_FP64), ("mixed", capi.PLM_MIXED)):
@@ -103,22 +176,32 @@ def eval_leg(steps: int):
             capi.eval_structure(model, ax, pos, types, prec)
         dt = (time.perf_counter() - t0) / steps
         capi.profile_enable(False)
+        kern = {k: capi.profile_get(k) for k in ("eval_anlm", "eval_prodsum", "eval_pair")}
         out[f"eval_atoms_per_s_{name}"] = st.n_total / dt
         out[f"eval_ms_per_step_{name}"] = dt * 1e3
-        log(f"eval {name}: {dt * 1e3:.2f} ms/step, {st.n_total / dt:.4g} atoms/s")
+        out[f"eval_kernel_ms_per_step_{name}"] = {k: v[0] / steps for k, v in kern.items()}
+        log(f"eval {name}: {dt * 1e3:.2f} ms/step, {st.n_total / dt:.4g} atoms/s, kernels {kern}")
         # the same evaluation as a captured HIP graph over the resident cell and
         # neighbour lists, replayed (no list rebuild between steps)
         batch = capi.Batch(model, [ax], [pos], [types])
+        n_half = batch.n_pairs // 2  # one cell without self images: each pair twice
         graph = capi.EvalGraph(model, batch, prec)
         graph.run(1)
-        n_rep = 10 * steps
         t0 = time.perf_counter()
-        graph.run(n_rep)
-        dt = (time.perf_counter() - t0) / n_rep
+        graph.run(graph_steps)
+        dt = (time.perf_counter() - t0) / graph_steps
         out[f"eval_graph_atoms_per_s_{name}"] = st.n_total / dt
         out[f"eval_graph_ms_per_step_{name}"] = dt * 1e3
-        log(f"eval graph {name}: {dt * 1e3:.2f} ms/step over {n_rep} replays, "
+        log(f"eval graph {name}: {dt * 1e3:.2f} ms/step over {graph_steps} replays, "
             f"{st.n_total / dt:.4g} atoms/s")
+        ms_p, n_p = kern["eval_pair"]
+        peak = FP64_PEAK_TFLOPS if name == "fp64" else FP32_PEAK_TFLOPS
+        ach = 40.0 * n_fn * n_lmh * n_half / (ms_p / n_p * 1e-3) / 1e12
+        out[f"eval_roofline_{name}"] = {
+            "bound": "valu", "kernel": "pair_kernel", "unit": "TFLOP/s",
+            "achieved": ach, "peak": peak, "frac": ach / peak,
+            "work": f"40 n_fn n_lm_half = {40 * n_fn * n_lmh} flops per half pair x {n_half} "
+                    "half pairs per launch"}
         del graph, batch
     e64, f64, s64 = res["fp64"]
     em, fm, sm = res["mixed"]
@@ -129,8 +212,16 @@ def eval_leg(steps: int):
     }
     out["eval_workload"] = (f"{st.n_total}-atom bcc 20x20x40, a=3.30, sigma=0.1; cfg1 model "
                             "(rc 6, 11 radial, gtinv order 2, max_l 4, max_p 2) fitted on the GPU; "
-                            "eval_*: neighbour list + evaluation per step; eval_graph_*: the "
-                            "evaluation kernels captured as a HIP graph and replayed")
+                            "eval_*: neighbour lists + evaluation per step; eval_graph_*: the "
+                            f"evaluation kernels captured as a HIP graph and replayed {graph_steps} "
+                            "times over the resident lists")
+    if cpu:
+        log("eval cpu baseline ...")
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "cfg1.yaml")
+            save_mlp_yaml(spec, coef / ne.scales, np.ones(len(coef)), path)
+            out["eval_cpu_baseline"] = eval_cpu_baseline(path, st)
+        log(f"eval cpu: {out['eval_cpu_baseline']}")
     return out
 
 
@@ -169,6 +260,38 @@ def fit_accuracy_leg(n_st: int):
                   "alpha 1e-3 and 1e-1, held-out predictions"}}
 
 
+def fit_end_to_end_leg(n_st: int, alphas=(1e-3,)):
+    """The fit as a user calls it, cold: mlp_dev.calc_xtx_xty (row weights,
+    batch upload with the host lattice translations and the GPU neighbour
+    lists, the accumulation, read-back and scaling of the normal equations;
+    data_sequential.py:25-94) followed by mlp_dev.solver_ridge (solvers.py:48-84),
+    on the workload's structures, for both precisions. The targets do not
+    change the work, so zero targets are used. Returns rows/s of the whole
+    call and the split."""
+    from pypolymlp_amd import capi, datasets, mlp_dev
+
+    ds = datasets.config2_dataset(n_st, targets=False)
+    model = capi.Model(datasets.config2_model().as_dict())
+    # one-time library work (code objects, model tables) outside the timing
+    mlp_dev.calc_xtx_xty(model, [ds.subset(range(2))], precision=capi.PLM_MIXED)
+    rows = mlp_dev.reference_n_rows(ds)
+    out = {}
+    for name, prec in (("mixed", capi.PLM_MIXED), ("fp64", capi.PLM_FP64)):
+        t0 = time.perf_counter()
+        ne = mlp_dev.calc_xtx_xty(model, [ds], precision=prec)
+        t1 = time.perf_counter()
+        mlp_dev.solver_ridge(ne.xtx, ne.xty, np.asarray(alphas))
+        t2 = time.perf_counter()
+        out[name] = {"rows_per_s": rows / (t2 - t0), "seconds": t2 - t0,
+                     "calc_xtx_xty_s": t1 - t0, "solver_ridge_s": t2 - t1}
+        log(f"end-to-end {name}: {t2 - t0:.2f} s ({rows / (t2 - t0):.4g} rows/s), "
+            f"calc_xtx_xty {t1 - t0:.2f} s, ridge {t2 - t1:.2f} s")
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    out["workload"] = (f"{n_st} structures of the workload, cold calc_xtx_xty + solver_ridge "
+                       f"at alpha {list(alphas)}")
+    return {"fit_end_to_end": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +302,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eval-steps", type=int, default=5)
+    ap.add_argument("--eval-graph-steps", type=int, default=1000)
+    ap.add_argument("--e2e-structures", type=int, default=1000,
+                    help="structures of the cold end-to-end fit timing (0: skip)")
     ap.add_argument("--fit-sample", type=int, default=120,
                     help="structures of the mixed-vs-FP64 fit accuracy check (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -242,55 +368,69 @@ def main():
         dt_fp64, k_fp64 = run(capi.PLM_FP64, args.fp64_steps, 0)
         log(f"fp64: {dt_fp64 * 1e3:.1f} ms/step, {R / dt_fp64:.4g} rows/s, kernels {k_fp64}")
 
-    # dominant kernel of the mixed step and its roofline. Algorithmic work of
-    # the Gram: the exact 3-slice int8 Gram of [X | y] needs 9 slice products
-    # per (i <= j) pair per row, 2 ops each; the FP64 Gram 2 flops per pair per row.
-    ms = {k: v[0] for k, v in k_mixed.items()}
-    dom = max(ms, key=ms.get)
+    # Rooflines. The headline kernel is the Gram of [X | y] (F' = F + 1
+    # columns): algorithmic work F'(F'+1) ops per row (the upper triangle,
+    # 2 ops per product). The mixed form computes the exact Gram of X quantised
+    # to three 7-bit slices, which takes 9 int8 slice products per algorithmic
+    # product, so its method peak is the int8 MFMA peak / 9 (the convention of
+    # an n-product split Gram: algorithmic work against peak / n); the issued
+    # int8 rate is reported beside it. The FP64 form runs against the FP64
+    # MFMA peak. Profile totals cover all timed steps.
     Fy = F + 1
-    pairs_ij = Fy * (Fy + 1) / 2.0
-    # profile totals cover all timed steps: work per step x steps / total time
+    alg_per_row = float(Fy) * (Fy + 1)
     ms_g, n_g = k_mixed["gram"]
-    gram_i8 = {"bound": "mfma", "kernel": "gram_i8_kernel", "unit": "TOP/s (int8)",
-               "peak": INT8_PEAK_TOPS, "launches_per_step": n_g / args.steps,
-               "achieved": 9 * 2 * pairs_ij * R * args.steps / (ms_g * 1e-3) / 1e12}
-    gram_i8["frac"] = gram_i8["achieved"] / INT8_PEAK_TOPS
+    ach = alg_per_row * R * args.steps / (ms_g * 1e-3) / 1e12
+    gram_i8 = {"bound": "mfma", "kernel": "gram_i8_kernel",
+               "unit": "TOP/s (algorithmic Gram ops of the exact 3-slice int8 method)",
+               "achieved": ach, "peak": INT8_PEAK_TOPS / 9.0, "frac": ach / (INT8_PEAK_TOPS / 9.0),
+               "launches_per_step": n_g / args.steps,
+               "work": f"F'(F'+1) = {alg_per_row:.4g} ops per row, F' = {Fy}; {R} rows per step",
+               "issued_int8": {"achieved_tops": 9.0 * ach, "peak_tops": INT8_PEAK_TOPS,
+                               "frac": 9.0 * ach / INT8_PEAK_TOPS},
+               "vs_fp64_mfma_peak": ach / FP64_PEAK_TFLOPS}
     gram_f64 = None
     if args.fp64_steps > 0:
         ms_g64, n_g64 = k_fp64["gram"]
+        a64 = alg_per_row * R * args.fp64_steps / (ms_g64 * 1e-3) / 1e12
         gram_f64 = {"bound": "mfma", "kernel": "gram_f64_kernel", "unit": "TFLOP/s (fp64)",
-                    "peak": FP64_PEAK_TFLOPS, "launches_per_step": n_g64 / args.fp64_steps,
-                    "achieved": 2 * pairs_ij * R * args.fp64_steps / (ms_g64 * 1e-3) / 1e12}
-        gram_f64["frac"] = gram_f64["achieved"] / FP64_PEAK_TFLOPS
-    ms_x, n_x = k_mixed["xrows"]
-    # X rows: FP32 X written once, plus the pair records and order parameters
-    # read once (the algorithmic minimum; the kernel itself is VALU/LDS bound)
-    n_fn = len(spec.as_dict()["model"]["pair_params"])
+                    "achieved": a64, "peak": FP64_PEAK_TFLOPS, "frac": a64 / FP64_PEAK_TFLOPS,
+                    "launches_per_step": n_g64 / args.fp64_steps}
+    # linear columns of X by signature groups (xgroup_kernel): algorithmic
+    # bytes = its output (stress and force rows of the linear columns, FP32)
+    # plus its inputs read once (the pair records of the step)
+    n_fn = len(spec.pair_params)
     n_lmh = (spec.max_l + 1) * (spec.max_l + 2) // 2
     rec_bytes = 4.0 * ((8 + 2 * n_fn + 8 * n_lmh + 3) // 4 * 4)
-    x_bytes = float(R) * F * 4.0 + batch.n_pairs * rec_bytes
-    xrows = {"bound": "hbm", "kernel": "xrows_kernel", "unit": "GB/s", "peak": HBM_PEAK_GBS,
-             "launches_per_step": n_x / args.steps,
-             "achieved": x_bytes * args.steps / (ms_x * 1e-3) / 1e9}
-    xrows["frac"] = xrows["achieved"] / HBM_PEAK_GBS
-    by_name = {"gram": gram_i8, "xrows": xrows}
-    roof = dict(by_name.get(dom, gram_i8))
-    # HBM bytes per launch of the dominant kernel: rocprofv3 FETCH_SIZE and
-    # WRITE_SIZE of this workload (scaled by a 1 GiB copy), committed with the
-    # profiles; the counters cannot be read from inside this process
+    n_lin = len(capi.feature_attrs(model)[0])
+    xg_bytes = float(R - len(sts)) * n_lin * 4.0 + batch.n_pairs * rec_bytes
+    ms_x, n_x = k_mixed["xgroup"]
+    xgroup = {"bound": "hbm", "kernel": "xgroup_kernel", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+              "launches_per_step": n_x / args.steps,
+              "achieved": xg_bytes * args.steps / (ms_x * 1e-3) / 1e9 if ms_x else None}
+    if xgroup["achieved"]:
+        xgroup["frac"] = xgroup["achieved"] / HBM_PEAK_GBS
+    roof = dict(gram_i8)
+    # HBM bytes per launch of the Gram: rocprofv3 FETCH_SIZE and WRITE_SIZE of
+    # this workload (scaled by a 1 GiB copy), committed with the profiles; the
+    # counters cannot be read from inside this process
     roof["traffic"] = None
     tpath = os.path.join(ROOT, "profiles", "fit_binary_alloy_traffic.json")
-    if os.path.exists(tpath) and len(sts) == 1000 and roof["kernel"] == "gram_i8_kernel":
+    if os.path.exists(tpath) and len(sts) == 1000:
         with open(tpath) as f:
             tr = json.load(f)
         k = tr.get(roof["kernel"])
         if k:
             roof["traffic"] = k["fetch_bytes_per_launch"] + k["write_bytes_per_launch"]
             roof["traffic_source"] = "profiles/fit_binary_alloy_traffic.json (rocprofv3 --pmc)"
+            alg_bytes = 3.0 * Fy * R / max(n_g / args.steps, 1)  # three int8 planes per launch
+            roof["traffic_vs_algorithmic"] = roof["traffic"] / alg_bytes
 
-    ev = eval_leg(args.eval_steps) if args.eval_steps > 0 else {}
+    ev = eval_leg(args.eval_steps, args.eval_graph_steps, not args.no_cpu_baseline) \
+        if args.eval_steps > 0 else {}
     if args.fit_sample > 0:
         ev.update(fit_accuracy_leg(args.fit_sample))
+    if args.e2e_structures > 0:
+        ev.update(fit_end_to_end_leg(args.e2e_structures))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -321,7 +461,7 @@ def main():
         "kernel_ms_per_step_mixed": {k: v[0] / args.steps for k, v in k_mixed.items()},
         "kernel_ms_per_step_fp64": {k: v[0] / args.fp64_steps for k, v in k_fp64.items()},
         "roofline": roof,
-        "roofline_kernels": {"gram_i8": gram_i8, "gram_f64": gram_f64, "xrows": xrows},
+        "roofline_kernels": {"gram_i8": gram_i8, "gram_f64": gram_f64, "xgroup": xgroup},
         "cpu_baseline": cpu,
     }
     out.update(ev)

@@ -29,7 +29,9 @@ extern "C" {
 
 /* Precision of the feature stream / Gram accumulation. */
 #define PLM_FP64 0  /* plain FP64 features, FP64 MFMA X^T X (the drop-in form) */
-#define PLM_MIXED 1 /* FP32 features, split-bf16 x3 MFMA, FP64 block sums */
+#define PLM_MIXED 1 /* FP32 features; X^T X as the exact Gram of X quantised per
+                     * (row chunk, column) to a shared exponent and three signed
+                     * 7-bit slices, on int8 MFMA with int32 sums combined in FP64 */
 
 typedef struct plm_model plm_model;
 typedef struct plm_batch plm_batch;
@@ -150,6 +152,24 @@ int plm_surrogate_predict(const float* params, const float* x, int n, float* y);
 int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
               double* coeffs, int* info);
 
+/* The ridge solve on an accumulator's own normal equations, without reading
+ * X^T X back: the scaling of calc_xtx_xty (data_sequential.py:72-92: columns
+ * with zero[f] != 0 dropped, X^T X / (s s^T), X^T y / s; zero may be NULL) is
+ * applied on the device, then solver_ridge (solvers.py:48-84) runs as in
+ * plm_ridge. coeffs: n_alpha x F (coefficients of the scaled problem), info:
+ * n_alpha. sq_err (n_alpha, may be NULL): c^T A c - 2 c^T b + y^T y of each
+ * solution on the same scaled system, the numerator of compute_rmse
+ * (utils_model_selection.py:37-47). */
+int plm_normal_ridge(plm_normal* acc, const double* scales, const unsigned char* zero,
+                     const double* alphas, int n_alpha, double* coeffs, int* info,
+                     double* sq_err);
+/* compute_rmse's numerator (utils_model_selection.py:37-47) of n_coef
+ * coefficient vectors (n_coef x F) against an accumulator's normal equations
+ * scaled by scales / zero on the device (e.g. the test set with the training
+ * scales). */
+int plm_normal_sq_error(plm_normal* acc, const double* scales, const unsigned char* zero,
+                        const double* coeffs, int n_coef, double* sq_err);
+
 /* FeaturesAttr (src/cxx/src/compute/py_features_attr.cpp:11-44): the
  * attributes of the columns of X. Per linear feature f (n_linear of them):
  * radial[f] its radial index, gtinv[f] its invariant (l-combination) index
@@ -190,10 +210,40 @@ int plm_eval_graph_run(plm_eval_graph* g, int n_steps, double* energies, double*
                        double* stresses);
 int plm_eval_graph_destroy(plm_eval_graph* g);
 
+/* libmlpcpp support functions on the device, running the device functions of
+ * the X and evaluation kernels (PLM_FP64 or PLM_MIXED = FP32).
+ * get_fn (src/cxx/src/python/pybind11_mlp.cpp:162-171,
+ * polymlp_functions_interface.cpp:41-63): Gaussian x cosine-cutoff radial
+ * functions at n distances r for n_params (beta, mu) pairs;
+ * fn, fn_d: n x n_params. */
+int plm_get_fn(int n, const double* r, const double* params, int n_params, double cutoff,
+               int precision, double* fn, double* fn_d);
+/* get_ylm (pybind11_mlp.cpp:173-184, polymlp_functions_interface.cpp:113-139):
+ * Y_l^{-m} (m >= 0) at key l(l+3)/2 - m, l <= lmax, and their Cartesian
+ * derivatives at n points xyz (n x 3, r = |xyz|). ylm: n x nh complex as
+ * (re, im) pairs, dylm: n x nh x 3 (d/dx, d/dy, d/dz) complex pairs,
+ * nh = (lmax + 1)(lmax + 2) / 2. */
+int plm_get_ylm(int n, const double* xyz, int lmax, int precision, double* ylm, double* dylm);
+/* PolymlpAPI::compute_anlmtp_conjugate + compute_features
+ * (src/cxx/src/polymlp/polymlp_api.cpp:63-87, 130-140) for one atom of a gtinv
+ * model's type: the non-conjugate order parameters (anlm_re, anlm_im, one per
+ * non-conjugate component, plm_model_type_sizes) are completed with their
+ * conjugates into anlm_full (n_nlm complex pairs) and the type's linear
+ * feature values written to features (n_linear). Either output may be NULL. */
+int plm_api_features(plm_model* m, int type, const double* anlm_re, const double* anlm_im,
+                     int precision, double* anlm_full, double* features);
+/* PolymlpAPI::compute_sum_of_prod_anlmtp (polymlp_api.cpp:174-185): prod_sum_e
+ * and prod_sum_f (one complex pair per non-conjugate head) of the completed
+ * order parameters, with the coefficients of plm_model_set_coeffs. */
+int plm_api_sum_of_prod(plm_model* m, int type, const double* anlm_re, const double* anlm_im,
+                        int precision, double* prod_e, double* prod_f);
+
 /* Instrumentation (no counterpart in the reference): per-kernel-class timing
  * with HIP events recorded on the library's own stream, for roofline
- * figures. Names: "neighbor", "anlm", "xblock", "gram", "xty", "eval_anlm",
- * "eval_prodsum", "eval_pair", "cholesky", "chol_solve". */
+ * figures. Names: "pairdata", "anlm", "wprime", "xenergy", "xgroup" (linear
+ * columns by signature groups), "xrows" (linear columns by column blocks),
+ * "poly_deriv", "poly_gemm", "poly_gather" (polynomial columns), "quantize",
+ * "gram", "eval_anlm", "eval_prodsum", "eval_pair", "cholesky". */
 int plm_profile_enable(int on);
 int plm_profile_reset(void);
 int plm_profile_get(const char* name, double* total_ms, long* launches);

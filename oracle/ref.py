@@ -59,6 +59,12 @@ def lib():
                                   ctypes.c_int, _dp]
         L.ref_eval.argtypes = [ctypes.c_void_p, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int,
                                _dp, _dp, _dp]
+        L.ref_get_fn.restype = None
+        L.ref_get_fn.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.ref_get_ylm.restype = None
+        L.ref_get_ylm.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+        L.ref_api.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp,
+                              _dp, _dp, _ip]
         L.ref_gtinv.restype = ctypes.c_long
         L.ref_gtinv.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, _ip,
                                 _ip, _ip, _dp]
@@ -139,6 +145,51 @@ class RefModel:
         if rc:
             raise RuntimeError(lib().ref_last_error().decode())
         return float(e[0]), f, s
+
+
+def get_fn(r, params, cutoff):
+    """(fn, fn_d), each (len(r), len(params)), from the reference's get_fn_."""
+    r = _d(np.atleast_1d(r))
+    par = _d(params).reshape(-1, 2)
+    fn = np.zeros((len(r), len(par)))
+    fd = np.zeros_like(fn)
+    lib().ref_get_fn(len(r), _pd(r), _pd(par), len(par), float(cutoff), _pd(fn), _pd(fd))
+    return fn, fd
+
+
+def get_ylm(xyz, lmax):
+    """(ylm (n, nh), dylm (n, nh, 3)) complex from the reference's get_ylm_."""
+    xyz = _d(np.atleast_2d(xyz))
+    n = len(xyz)
+    nh = (lmax + 1) * (lmax + 2) // 2
+    y = np.zeros((n, nh, 2))
+    dy = np.zeros((n, nh, 3, 2))
+    lib().ref_get_ylm(n, _pd(xyz), int(lmax), _pd(y), _pd(dy))
+    return y[..., 0] + 1j * y[..., 1], dy[..., 0] + 1j * dy[..., 1]
+
+
+def api(model: "RefModel", anlm_re, anlm_im, type1, with_prod=False):
+    """PolymlpAPI on one atom: (anlm_full, features[, prod_sum_e, prod_sum_f])
+    from its non-conjugate order parameters."""
+    re, im = _d(anlm_re), _d(anlm_im)
+    n = np.zeros(3, dtype=np.int32)
+    L = lib()
+    if L.ref_api(model._h, int(type1), _pd(re), _pd(im), len(re), None, None, None, None, _pi(n)):
+        raise RuntimeError(L.ref_last_error().decode())
+    a = np.zeros((n[0], 2))
+    f = np.zeros(n[1])
+    pe = pf = None
+    if with_prod:
+        pe = np.zeros((n[0], 2))
+        pf = np.zeros((n[0], 2))
+    if L.ref_api(model._h, int(type1), _pd(re), _pd(im), len(re), _pd(a), _pd(f),
+                 _pd(pe) if with_prod else None, _pd(pf) if with_prod else None, _pi(n)):
+        raise RuntimeError(L.ref_last_error().decode())
+    out = (a[:, 0] + 1j * a[:, 1], f)
+    if with_prod:
+        k = n[2]
+        out += (pe[:k, 0] + 1j * pe[:k, 1], pf[:k, 0] + 1j * pf[:k, 1])
+    return out
 
 
 def neighbor_cell(axis, positions_c, cutoff):

@@ -13,6 +13,7 @@
 //   PolymlpAPI / Features / Potential            src/cxx/src/polymlp/*.cpp
 //   PolymlpEval                                  src/cxx/src/compute/polymlp_eval.cpp
 //   parse_polymlp (polymlp.yaml reader)          src/cxx/src/polymlp/polymlp_parse_potential.cpp
+//   get_fn_ / get_ylm_                           src/cxx/src/polymlp/polymlp_functions_interface.cpp
 //
 // Restated here:
 //   Model::model_polynomial / model_order{1,2,3}  model.cpp:159-267
@@ -30,6 +31,7 @@
 #include "compute/neighbor_half.h"
 #include "compute/polymlp_eval.h"
 #include "polymlp/polymlp_api.h"
+#include "polymlp/polymlp_functions_interface.h"
 #include "polymlp/polymlp_parse_potential.h"
 #include "polymlp/polymlp_parse_potential_legacy.h"
 #include "polymlp/polymlp_read_gtinv.h"
@@ -39,6 +41,7 @@ namespace {
 struct RefModel {
     feature_params fp;
     vector1d coeffs;  // as stored in polymlp.yaml (coeffs / scales)
+    vector1d pot2;    // 2 * coeffs / scales, as parse_polymlp returns them
     PolymlpAPI feat;  // features with derivatives (PolymlpAPI::set_features)
     std::unique_ptr<PolymlpEval> eval;
     std::vector<std::vector<PolynomialTerm>> terms1, terms2, terms3;
@@ -167,6 +170,7 @@ void* ref_model_load(const char* yaml, int with_eval) {
             parse_polymlp_legacy(yaml, m->fp, pot2, ele, mass);
         else
             parse_polymlp(yaml, m->fp, pot2, ele, mass);
+        m->pot2 = pot2;
         m->coeffs.resize(pot2.size());
         for (size_t i = 0; i < pot2.size(); ++i) m->coeffs[i] = 0.5 * pot2[i];
         m->feat.set_features(m->fp);
@@ -330,6 +334,83 @@ int ref_eval(void* h, const double* axis9, const double* pos, const int* types, 
         for (int i = 0; i < n; ++i)
             for (int a = 0; a < 3; ++a) forces[3 * i + a] = f[i][a];
         for (int k = 0; k < 6; ++k) stress[k] = s[k];
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// get_fn_ (polymlp_functions_interface.cpp:41-63) at n distances: fn, fn_d
+// (n x n_params, row-major).
+void ref_get_fn(int n, const double* r, const double* params, int n_params, double cutoff,
+                double* fn, double* fn_d) {
+    feature_params fp;
+    fp.pair_type = "gaussian";
+    fp.cutoff = cutoff;
+    vector2d par(n_params, vector1d(2));
+    for (int k = 0; k < n_params; ++k) par[k] = {params[2 * k], params[2 * k + 1]};
+    for (int i = 0; i < n; ++i) {
+        vector1d f, d;
+        get_fn_(r[i], fp, par, f, d);
+        for (int k = 0; k < n_params; ++k) {
+            fn[(size_t)i * n_params + k] = f[k];
+            fn_d[(size_t)i * n_params + k] = d[k];
+        }
+    }
+}
+
+// get_ylm_ (polymlp_functions_interface.cpp:113-139) at n points with
+// r = |xyz|: ylm (n x nh complex pairs), dylm (n x nh x 3 complex pairs).
+void ref_get_ylm(int n, const double* xyz, int lmax, double* ylm, double* dylm) {
+    const int nh = (lmax + 1) * (lmax + 2) / 2;
+    for (int i = 0; i < n; ++i) {
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        const double r = std::sqrt(x * x + y * y + z * z);
+        vector1dc Y, dx, dy, dz;
+        get_ylm_(r, x, y, z, lmax, Y, dx, dy, dz);
+        for (int k = 0; k < nh; ++k) {
+            const size_t o = (size_t)i * nh + k;
+            ylm[2 * o] = Y[k].real(), ylm[2 * o + 1] = Y[k].imag();
+            const dc* d3[3] = {&dx[k], &dy[k], &dz[k]};
+            for (int a = 0; a < 3; ++a) {
+                dylm[2 * (3 * o + a)] = d3[a]->real();
+                dylm[2 * (3 * o + a) + 1] = d3[a]->imag();
+            }
+        }
+    }
+}
+
+// PolymlpAPI::compute_anlmtp_conjugate + compute_features + (with coefficients)
+// compute_sum_of_prod_anlmtp (polymlp_api.cpp:63-185) for one atom of type1
+// from its non-conjugate order parameters. Sizes are returned through n_out:
+// [n_nlm, n_features, n_heads]; outputs may be null.
+int ref_api(void* h, int type1, const double* re, const double* im, int n_noconj, double* anlm,
+            double* features, double* prod_e, double* prod_f, int* n_out) {
+    try {
+        auto* m = static_cast<RefModel*>(h);
+        vector1d r(re, re + n_noconj), i(im, im + n_noconj);
+        vector1dc a;
+        m->feat.compute_anlmtp_conjugate(r, i, type1, a);
+        vector1d f;
+        m->feat.compute_features(a, type1, f);
+        n_out[0] = (int)a.size();
+        n_out[1] = (int)f.size();
+        n_out[2] = 0;
+        if (anlm)
+            for (size_t k = 0; k < a.size(); ++k) anlm[2 * k] = a[k].real(), anlm[2 * k + 1] = a[k].imag();
+        if (features) std::copy(f.begin(), f.end(), features);
+        if (prod_e || prod_f) {
+            PolymlpAPI api;
+            api.set_potential_model(m->fp, m->pot2);
+            vector1dc pe, pf;
+            api.compute_sum_of_prod_anlmtp(a, type1, pe, pf);
+            n_out[2] = (int)pe.size();
+            for (size_t k = 0; k < pe.size(); ++k) {
+                if (prod_e) prod_e[2 * k] = pe[k].real(), prod_e[2 * k + 1] = pe[k].imag();
+                if (prod_f) prod_f[2 * k] = pf[k].real(), prod_f[2 * k + 1] = pf[k].imag();
+            }
+        }
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

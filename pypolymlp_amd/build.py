@@ -50,6 +50,8 @@ SOURCES = [
     ("surrogate.cpp", []),
     ("kernels_cg.hip", []),
     ("cg.cpp", []),
+    ("kernels_api.hip", []),
+    ("api.cpp", []),
 ]
 
 

@@ -26,11 +26,13 @@ EXPORTS = [
     "plm_cell_translations", "plm_neighbor", "plm_batch_create", "plm_batch_destroy",
     "plm_batch_n_rows", "plm_batch_n_pairs", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
+    "plm_normal_ridge", "plm_normal_sq_error",
     "plm_solver_cg", "plm_model_feature_attrs",
     "plm_eval", "plm_eval_batch", "plm_eval_graph_create", "plm_eval_graph_run",
     "plm_eval_graph_destroy", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
     "plm_surrogate_n_params", "plm_surrogate_train", "plm_surrogate_grad",
-    "plm_surrogate_predict",
+    "plm_surrogate_predict", "plm_get_fn", "plm_get_ylm", "plm_api_features",
+    "plm_api_sum_of_prod",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -86,6 +88,9 @@ def lib():
         L.plm_normal_accumulate.argtypes = [vp, vp, _dp, _dp, ctypes.c_int]
         L.plm_normal_result.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _lp]
         L.plm_ridge.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _ip]
+        _bp = ctypes.POINTER(ctypes.c_ubyte)
+        L.plm_normal_ridge.argtypes = [vp, _dp, _bp, _dp, ctypes.c_int, _dp, _ip, _dp]
+        L.plm_normal_sq_error.argtypes = [vp, _dp, _bp, _dp, ctypes.c_int, _dp]
         L.plm_eval.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.plm_eval_batch.argtypes = [vp, vp, ctypes.c_int, _dp, _dp, _dp]
         L.plm_eval_graph_create.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(vp)]
@@ -98,6 +103,11 @@ def lib():
         L.plm_surrogate_grad.argtypes = [_fp, _fp, _fp, ctypes.c_int, _fp, _fp]
         L.plm_surrogate_predict.argtypes = [_fp, _fp, ctypes.c_int, _fp]
         L.plm_profile_get.argtypes = [ctypes.c_char_p, _dp, _lp]
+        L.plm_get_fn.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, _dp, _dp]
+        L.plm_get_ylm.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.plm_api_features.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.plm_api_sum_of_prod.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -285,6 +295,45 @@ class Normal:
         _check(lib().plm_normal_result(self._h, _pd(xtx), _pd(xty), _pd(s), _pd(s2),
                                        ctypes.byref(ysq), ctypes.byref(n)))
         return xtx, xty, s, s2, ysq.value, n.value
+
+    def vectors(self):
+        """(xe_sum, xe_sq_sum, y_sq, n_rows) without reading X^T X back."""
+        F = self.model.n_features
+        s = np.zeros(F)
+        s2 = np.zeros(F)
+        ysq = ctypes.c_double()
+        n = ctypes.c_long()
+        _check(lib().plm_normal_result(self._h, None, None, _pd(s), _pd(s2), ctypes.byref(ysq),
+                                       ctypes.byref(n)))
+        return s, s2, ysq.value, n.value
+
+    def ridge(self, scales, zero, alphas):
+        """(coeffs (F, n_alpha), info, sq_err) of the ridge solves on this
+        accumulator's normal equations scaled on the device (plm_normal_ridge)."""
+        F = self.model.n_features
+        a = _d(alphas)
+        sc = _d(scales)
+        z = None if zero is None else np.ascontiguousarray(zero, dtype=np.uint8)
+        c = np.zeros((len(a), F))
+        info = np.zeros(len(a), dtype=np.int32)
+        sq = np.zeros(len(a))
+        _check(lib().plm_normal_ridge(self._h, _pd(sc), None if z is None else
+                                      z.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _pd(a),
+                                      len(a), _pd(c), _pi(info), _pd(sq)))
+        c[info != 0] = 1e30
+        return c.T.copy(), info, sq
+
+    def sq_error(self, scales, zero, coeffs):
+        """c^T A c - 2 c^T b + y^T y of each column of coeffs (F, k) on this
+        accumulator's normal equations scaled on the device."""
+        sc = _d(scales)
+        z = None if zero is None else np.ascontiguousarray(zero, dtype=np.uint8)
+        C = _d(np.asarray(coeffs).T)
+        out = np.zeros(C.shape[0])
+        _check(lib().plm_normal_sq_error(self._h, _pd(sc), None if z is None else
+                                         z.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _pd(C),
+                                         C.shape[0], _pd(out)))
+        return out
 
 
 def cell_translations(axis, positions_c, cutoff: float):
@@ -482,3 +531,55 @@ def surrogate_predict(params, x):
     y = np.zeros((len(x), SURR_OUT), dtype=np.float32)
     _check(lib().plm_surrogate_predict(_pf(params), _pf(x), len(x), _pf(y)))
     return y
+
+
+# --- libmlpcpp support functions on the device (get_fn, get_ylm, PolymlpAPI) --
+def get_fn(r, params, cutoff: float, precision: int = PLM_FP64):
+    """(fn, fn_d), each (len(r), len(params)): radial functions at the
+    distances r (libmlpcpp.get_fn, pybind11_mlp.cpp:162-171)."""
+    r = _d(np.atleast_1d(r))
+    par = _d(params).reshape(-1, 2)
+    fn = np.zeros((len(r), len(par)))
+    fd = np.zeros_like(fn)
+    _check(lib().plm_get_fn(len(r), _pd(r), _pd(par), len(par), float(cutoff), precision,
+                            _pd(fn), _pd(fd)))
+    return fn, fd
+
+
+def get_ylm(xyz, lmax: int, precision: int = PLM_FP64):
+    """(ylm, dylm) complex: ylm (n, nh), dylm (n, nh, 3) at the points xyz (n, 3)
+    (libmlpcpp.get_ylm, pybind11_mlp.cpp:173-184)."""
+    xyz = _d(np.atleast_2d(xyz))
+    n = len(xyz)
+    nh = (lmax + 1) * (lmax + 2) // 2
+    y = np.zeros((n, nh, 2))
+    dy = np.zeros((n, nh, 3, 2))
+    _check(lib().plm_get_ylm(n, _pd(xyz), int(lmax), precision, _pd(y), _pd(dy)))
+    return y[..., 0] + 1j * y[..., 1], dy[..., 0] + 1j * dy[..., 1]
+
+
+def api_features(model: Model, anlm_re, anlm_im, type1: int, precision: int = PLM_FP64):
+    """(anlm_full, features) of one atom of type1 from its non-conjugate order
+    parameters (PolymlpAPI.compute_anlmtp_conjugate + compute_features)."""
+    n_nlm, n_nc, n_lin = model.type_sizes(type1)
+    re = _d(anlm_re)[:n_nc]
+    im = _d(anlm_im)[:n_nc]
+    a = np.zeros((n_nlm, 2))
+    f = np.zeros(n_lin)
+    _check(lib().plm_api_features(model.handle, int(type1), _pd(re), _pd(im), precision, _pd(a),
+                                  _pd(f)))
+    return a[:, 0] + 1j * a[:, 1], f
+
+
+def api_sum_of_prod(model: Model, anlm_re, anlm_im, type1: int, precision: int = PLM_FP64):
+    """(prod_sum_e, prod_sum_f) complex per non-conjugate head of one atom of
+    type1 (PolymlpAPI.compute_sum_of_prod_anlmtp); the model must hold its
+    coefficients."""
+    _, n_nc, _ = model.type_sizes(type1)
+    re = _d(anlm_re)[:n_nc]
+    im = _d(anlm_im)[:n_nc]
+    pe = np.zeros((n_nc, 2))
+    pf = np.zeros((n_nc, 2))
+    _check(lib().plm_api_sum_of_prod(model.handle, int(type1), _pd(re), _pd(im), precision,
+                                     _pd(pe), _pd(pf)))
+    return pe[:, 0] + 1j * pe[:, 1], pf[:, 0] + 1j * pf[:, 1]

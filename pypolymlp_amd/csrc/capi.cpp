@@ -710,7 +710,6 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
         const GroupTables& G = m.groups();
         const int nlin = m.host.n_linear();
         const long rows = b.row_off[s1] - b.row_off[s0];
-        ProfScope ps("xrows", st);
         if (!x_is_zero)
             check_hip(hipMemset2DAsync(X, (size_t)ld * elem, 0, (size_t)nlin * elem, (size_t)rows, st),
                       "memset");
@@ -1009,15 +1008,21 @@ int plm_normal_result(plm_normal* ap, double* xtx, double* xty, double* xe_sum, 
         // Gram of [X | y]: the upper element triangle is valid (the tiles cover
         // the upper block triangle), so mirror element-wise; column F holds
         // X^T y and element (F, F) y^T y
-        std::vector<double> full((size_t)Fp * Fp);
-        check_hip(hipMemcpy(full.data(), acc->xtx.ptr, full.size() * 8, hipMemcpyDeviceToHost), "copy");
-        if (xtx)
-            for (int i = 0; i < F; ++i)
-                for (int j = 0; j < F; ++j)
-                    xtx[(size_t)i * F + j] = i <= j ? full[(size_t)i * Fp + j] : full[(size_t)j * Fp + i];
-        if (xty)
-            for (int i = 0; i < F; ++i) xty[i] = full[(size_t)i * Fp + F];
-        if (y_sq) *y_sq = full[(size_t)F * Fp + F];
+        if (xtx || xty) {
+            std::vector<double> full((size_t)Fp * Fp);
+            check_hip(hipMemcpy(full.data(), acc->xtx.ptr, full.size() * 8, hipMemcpyDeviceToHost),
+                      "copy");
+            if (xtx)
+                for (int i = 0; i < F; ++i)
+                    for (int j = 0; j < F; ++j)
+                        xtx[(size_t)i * F + j] =
+                            i <= j ? full[(size_t)i * Fp + j] : full[(size_t)j * Fp + i];
+            if (xty)
+                for (int i = 0; i < F; ++i) xty[i] = full[(size_t)i * Fp + F];
+        }
+        if (y_sq)
+            check_hip(hipMemcpy(y_sq, (const double*)acc->xtx.ptr + (size_t)F * Fp + F, 8,
+                                hipMemcpyDeviceToHost), "copy");
         std::vector<double> v((size_t)Fp * 2);
         check_hip(hipMemcpy(v.data(), acc->vec.ptr, v.size() * 8, hipMemcpyDeviceToHost), "copy");
         if (xe_sum) std::memcpy(xe_sum, v.data(), (size_t)F * 8);
@@ -1079,6 +1084,24 @@ int plm_eval_graph_destroy(plm_eval_graph* gp) {
 int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
               double* coeffs, int* info) {
     PLM_TRY({ ridge_solve(F, xtx, xty, alphas, n_alpha, coeffs, info); });
+}
+
+int plm_normal_ridge(plm_normal* ap, const double* scales, const unsigned char* zero,
+                     const double* alphas, int n_alpha, double* coeffs, int* info,
+                     double* sq_err) {
+    PLM_TRY({
+        if (!ap || !scales || !alphas || !coeffs || !info) throw std::invalid_argument("null argument");
+        normal_ridge(*reinterpret_cast<NormalHandle*>(ap), scales, zero, alphas, n_alpha, coeffs,
+                     info, sq_err);
+    });
+}
+
+int plm_normal_sq_error(plm_normal* ap, const double* scales, const unsigned char* zero,
+                        const double* coeffs, int n_coef, double* sq_err) {
+    PLM_TRY({
+        if (!ap || !scales || !coeffs || !sq_err) throw std::invalid_argument("null argument");
+        normal_sq_error(*reinterpret_cast<NormalHandle*>(ap), scales, zero, coeffs, n_coef, sq_err);
+    });
 }
 
 int plm_model_feature_attrs(const plm_model* mp, int* n_linear, int* n_poly, int* radial,

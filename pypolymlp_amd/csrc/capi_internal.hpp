@@ -198,6 +198,9 @@ struct EvalGraph {
     EvalGraph& operator=(const EvalGraph&) = delete;
     ~EvalGraph();
 };
+void enqueue_eval_dev(ModelHandle& m, const BatchDev& bd, const int* hb_off, const int* hb_j,
+                      const double* hb_d, bool fp64, void* a_buf, void* pse, void* psf,
+                      double* out, size_t n_out, hipStream_t st);
 void eval_graph_capture(EvalGraph& g);
 void eval_graph_run(EvalGraph& g, int steps, double* energies, double* forces, double* stresses);
 void eval_single(ModelHandle& m, const double* axis, const double* pos_c, const int* types, int n,
@@ -206,6 +209,12 @@ void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, dou
                 double* stresses);
 void ridge_solve(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
                  double* coeffs, int* info);
+// ridge / squared errors on an accumulator's own normal equations, scaled on
+// the device (no read-back of X^T X)
+void normal_ridge(NormalHandle& acc, const double* scales, const unsigned char* zero,
+                  const double* alphas, int n_alpha, double* coeffs, int* info, double* sq_err);
+void normal_sq_error(NormalHandle& acc, const double* scales, const unsigned char* zero,
+                     const double* coeffs, int n_coef, double* sq_err);
 // CG on (xtx + alpha I) c = xty from c0 (zeros when null); returns the
 // iteration at which the stopping test ended the loop, or max_iter
 int cg_solve(int F, const double* xtx, const double* xty, double alpha, const double* coef0,

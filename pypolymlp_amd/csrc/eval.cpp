@@ -90,15 +90,16 @@ static void alloc_eval(const ModelHandle& m, const BatchHandle& b, bool fp64, De
 }
 
 // The kernels of one evaluation on st: out is cleared by a kernel (no memset
-// node when captured), then filled by the half-pair loop.
-static void enqueue_eval(ModelHandle& m, BatchHandle& b, bool fp64, void* a_buf, void* pse,
-                         void* psf, double* out, size_t n_out, hipStream_t st) {
-    const int A = b.n_atoms;
+// node when captured), then filled by the half-pair loop. bd carries the full
+// neighbour list (order parameters); hb_* the half list (pair loop).
+void enqueue_eval_dev(ModelHandle& m, const BatchDev& bd, const int* hb_off, const int* hb_j,
+                      const double* hb_d, bool fp64, void* a_buf, void* pse, void* psf,
+                      double* out, size_t n_out, hipStream_t st) {
+    const int A = bd.n_atoms_total;
     launch_fill_zero(st, out, n_out);
     double* d_e = out;
-    double* d_f = d_e + b.n_st;
+    double* d_f = d_e + bd.n_st;
     double* d_s = d_f + 3 * (size_t)A;
-    BatchDev bd = b.view();
     const int pair_model = m.host.input().gtinv ? 0 : 1;
     {
         ProfScope ps("eval_anlm", st);
@@ -113,12 +114,16 @@ static void enqueue_eval(ModelHandle& m, BatchHandle& b, bool fp64, void* a_buf,
     check_hip(hipGetLastError(), "prodsum launch");
     {
         ProfScope ps("eval_pair", st);
-        launch_pair(st, fp64, m.md, m.ed, bd, (const int*)b.d_atom_st.ptr,
-                    (const int*)b.d_hb_off.ptr, (const int*)b.d_hb_j.ptr,
-                    (const double*)b.d_hb_d.ptr, pse, psf, std::max(1, m.max_heads), d_e, d_f,
-                    d_s, pair_model);
+        launch_pair(st, fp64, m.md, m.ed, bd, bd.atom_st, hb_off, hb_j, hb_d, pse, psf,
+                    std::max(1, m.max_heads), d_e, d_f, d_s, pair_model);
     }
     check_hip(hipGetLastError(), "pair launch");
+}
+
+static void enqueue_eval(ModelHandle& m, BatchHandle& b, bool fp64, void* a_buf, void* pse,
+                         void* psf, double* out, size_t n_out, hipStream_t st) {
+    enqueue_eval_dev(m, b.view(), (const int*)b.d_hb_off.ptr, (const int*)b.d_hb_j.ptr,
+                     (const double*)b.d_hb_d.ptr, fp64, a_buf, pse, psf, out, n_out, st);
 }
 
 static void copy_out(const BatchHandle& b, const DevBuf& out, size_t n_out, hipStream_t st,

@@ -75,41 +75,42 @@ __global__ void __launch_bounds__(256) trsm_panel_kernel(double* __restrict__ A,
     for (int c = 0; c < SB; ++c) row[c] = x[c];
 }
 
-// Trailing update of the lower triangle: for 64x64 tiles (bi >= bj) of the
-// trailing matrix, C -= P_bi P_bj^T with the panel P (K = SB) on FP64 MFMA.
-__device__ __forceinline__ void low_tri_index(int t, int& bi, int& bj) {
-    // row bi holds bi + 1 tiles
-    int r = 0, rem = t;
-    while (rem >= r + 1) rem -= r + 1, ++r;
-    bi = r, bj = rem;
-}
-
-__global__ void __launch_bounds__(256) syrk_update_kernel(double* __restrict__ A, int ld, int k0) {
+// Update with a kw-wide panel (kw a multiple of SB): for the lower tiles
+// (bi >= bj, bj < ncb) of the nt x nt block grid starting at (base, base),
+// C -= P_bi P_bj^T with P = A[:, k0:k0+kw], on FP64 MFMA; the K loop runs over
+// the panel in 64-column chunks through LDS and C is read and written once.
+// ncb = nt: the whole trailing lower triangle; ncb < nt: only the first ncb
+// column blocks (the rest of the current panel during its factorization).
+__global__ void __launch_bounds__(256) syrk_panel_kernel(double* __restrict__ A, int ld, int k0,
+                                                         int kw, int base, int nt) {
     __shared__ double Ps[SB][SB + 1];  // [k][i] panel rows of the I tile
     __shared__ double Qs[SB][SB + 1];  // [k][j] panel rows of the J tile
-    int bi, bj;
-    low_tri_index(blockIdx.x, bi, bj);
-    const int base = k0 + SB;
+    int bj = 0, rem = blockIdx.x;
+    while (rem >= nt - bj) rem -= nt - bj, ++bj;
+    const int bi = bj + rem;
     const int I0 = base + bi * SB, J0 = base + bj * SB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < SB * SB; e += 256) {
-        const int r = e / SB, k = e % SB;
-        Ps[k][r] = A[(size_t)(I0 + r) * ld + k0 + k];
-        Qs[k][r] = A[(size_t)(J0 + r) * ld + k0 + k];
-    }
-    __syncthreads();
     const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
     doublex4 acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = doublex4{0, 0, 0, 0};
-    for (int ks = 0; ks < SB; ks += 4) {
-        const int kk = ks + (lane >> 4);
-        double av[2], bv[2];
-        for (int m = 0; m < 2; ++m) av[m] = Ps[kk][wi + 16 * m + (lane & 15)];
-        for (int q = 0; q < 2; ++q) bv[q] = Qs[kk][wj + 16 * q + (lane & 15)];
-        for (int m = 0; m < 2; ++m)
-            for (int q = 0; q < 2; ++q)
-                acc[m][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[q], acc[m][q], 0, 0, 0);
+    for (int kc = 0; kc < kw; kc += SB) {
+        __syncthreads();
+        for (int e = tid; e < SB * SB; e += 256) {
+            const int r = e / SB, k = e % SB;
+            Ps[k][r] = A[(size_t)(I0 + r) * ld + k0 + kc + k];
+            Qs[k][r] = A[(size_t)(J0 + r) * ld + k0 + kc + k];
+        }
+        __syncthreads();
+        for (int ks = 0; ks < SB; ks += 4) {
+            const int kk = ks + (lane >> 4);
+            double av[2], bv[2];
+            for (int m = 0; m < 2; ++m) av[m] = Ps[kk][wi + 16 * m + (lane & 15)];
+            for (int q = 0; q < 2; ++q) bv[q] = Qs[kk][wj + 16 * q + (lane & 15)];
+            for (int m = 0; m < 2; ++m)
+                for (int q = 0; q < 2; ++q)
+                    acc[m][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[q], acc[m][q], 0, 0, 0);
+        }
     }
     for (int m = 0; m < 2; ++m)
         for (int q = 0; q < 2; ++q)
@@ -185,23 +186,35 @@ __global__ void gemv_upper_update(const double* __restrict__ L, int ld, int k0,
     z[c] -= s;
 }
 
-// r = b - (A + alpha I) x with the full symmetric A (one thread per row,
-// FP64 with a compensated (TwoSum-style) accumulation of the products).
-__global__ void residual_kernel(const double* __restrict__ A, int ld, int n, double alpha,
-                                const double* __restrict__ x, const double* __restrict__ b,
-                                double* __restrict__ r) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// r = b - (A + alpha I) x with the full symmetric A: one wave per row, lanes
+// over the row (coalesced), FP64 with compensated (TwoSum) accumulation per
+// lane and a compensated combination of the lane sums.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__global__ void __launch_bounds__(256) residual_kernel(const double* __restrict__ A, int ld, int n,
+                                                       double alpha, const double* __restrict__ x,
+                                                       const double* __restrict__ b,
+                                                       double* __restrict__ r) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= n) return;
     const double* row = A + (size_t)i * ld;
     double s = 0.0, comp = 0.0;
-    for (int c = 0; c < n; ++c) {
-        const double p = row[c] * x[c];
-        const double t = s + p;
-        comp += (fabs(s) >= fabs(p)) ? (s - t) + p : (p - t) + s;
-        s = t;
+    for (int c = lane; c < n; c += 64) {
+        double t, e;
+        two_sum(s, row[c] * x[c], t, e);
+        s = t, comp += e;
     }
-    s += comp;
-    r[i] = b[i] - (s + alpha * x[i]);
+    for (int o = 32; o > 0; o >>= 1) {
+        const double so = __shfl_xor(s, o), co = __shfl_xor(comp, o);
+        double t, e;
+        two_sum(s, so, t, e);
+        s = t, comp += co + e;
+    }
+    if (lane == 0) r[i] = b[i] - ((s + comp) + alpha * x[i]);
 }
 
 __global__ void add_diag_kernel(double* __restrict__ A, int ld, int n, double add) {
@@ -214,14 +227,96 @@ __global__ void axpy_kernel(int n, const double* __restrict__ d, double* __restr
     if (i < n) x[i] += d[i];
 }
 
+// The scaled normal equations straight from the accumulator's Gram of
+// [X | y] (upper triangle valid, leading dimension ldg; column F holds X^T y):
+// A[i][j] = G[i][j] / s[i] / s[j] (the reference's xtx /= s[:, None];
+// xtx /= s[None, :]), zero for dropped columns, identity on the padding
+// (i, j >= F), the full symmetric n_pad x n_pad matrix; b[i] = G[i][F] / s[i].
+// 32 x 32 tiles; the lower tiles read the mirrored upper tile through LDS.
+__global__ void __launch_bounds__(256) gram_system_kernel(const double* __restrict__ G, int ldg,
+                                                          int F, const double* __restrict__ s,
+                                                          const unsigned char* __restrict__ zero,
+                                                          double* __restrict__ A, int lda,
+                                                          double* __restrict__ b) {
+    __shared__ double T[32][33];
+    const int bi = blockIdx.y, bj = blockIdx.x;
+    const int r0 = min(bi, bj) * 32, c0 = max(bi, bj) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        T[k][tx] = (r < F && c < F) ? G[(size_t)r * ldg + c] : 0.0;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int i = bi * 32 + k, j = bj * 32 + tx;
+        double v;
+        if (i >= F || j >= F) {
+            v = i == j ? 1.0 : 0.0;
+        } else if (zero[i] || zero[j]) {
+            v = 0.0;
+        } else {
+            // element (min(i, j), max(i, j)) of the upper triangle
+            const bool upper = bi < bj || (bi == bj && k <= tx);
+            v = (upper ? T[k][tx] : T[tx][k]) / s[i] / s[j];
+        }
+        A[(size_t)i * lda + j] = v;
+    }
+    if (bj == 0 && threadIdx.x < 32) {
+        const int i = bi * 32 + threadIdx.x;
+        b[i] = (i < F && !zero[i]) ? G[(size_t)i * ldg + F] / s[i] : 0.0;
+    }
+}
+
+// y = A x for the full symmetric n x n system (one wave per row, lanes over
+// the row: coalesced), FP64
+__global__ void __launch_bounds__(256) symv_rows_kernel(const double* __restrict__ A, int lda,
+                                                        int n, const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double* a = A + (size_t)row * lda;
+    double acc = 0.0;
+    for (int c = lane; c < n; c += 64) acc += a[c] * x[c];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) y[row] = acc;
+}
+
+void launch_gram_system(hipStream_t st, const double* G, int ldg, int F, const double* s,
+                        const unsigned char* zero, double* A, int lda, int n_pad, double* b) {
+    const dim3 grid(n_pad / 32, n_pad / 32);
+    gram_system_kernel<<<grid, 256, 0, st>>>(G, ldg, F, s, zero, A, lda, b);
+}
+
+void launch_symv(hipStream_t st, const double* A, int lda, int n, const double* x, double* y) {
+    symv_rows_kernel<<<(n + 3) / 4, 256, 0, st>>>(A, lda, n, x, y);
+}
+
+// Blocked right-looking Cholesky with NB-wide outer panels: inside a panel,
+// 64-wide steps (diagonal block, panel TRSM below it, update of the rest of
+// the panel's columns); after the panel, one trailing update with K = NB, so
+// that the dominant product runs with a 4x wider inner dimension.
+constexpr int NB = 256;
+
 void launch_cholesky(hipStream_t st, double* A, int ld, int n_pad, int* info) {
-    for (int k0 = 0; k0 < n_pad; k0 += SB) {
-        potf2_kernel<<<1, 256, 0, st>>>(A, ld, k0, info);
-        const int rest = n_pad - k0 - SB;
-        if (rest <= 0) break;
-        trsm_panel_kernel<<<(rest + 255) / 256, 256, 0, st>>>(A, ld, k0, n_pad);
-        const int nt = rest / SB;
-        syrk_update_kernel<<<nt * (nt + 1) / 2, 256, 0, st>>>(A, ld, k0);
+    for (int k0 = 0; k0 < n_pad; k0 += NB) {
+        const int kw = min(NB, n_pad - k0);
+        for (int j0 = k0; j0 < k0 + kw; j0 += SB) {
+            potf2_kernel<<<1, 256, 0, st>>>(A, ld, j0, info);
+            const int rest = n_pad - j0 - SB;
+            if (rest <= 0) break;
+            trsm_panel_kernel<<<(rest + 255) / 256, 256, 0, st>>>(A, ld, j0, n_pad);
+            const int pc = (k0 + kw - j0 - SB) / SB;  // panel column blocks still to update
+            if (pc > 0) {
+                const int nt = rest / SB;
+                syrk_panel_kernel<<<pc * nt - pc * (pc - 1) / 2, 256, 0, st>>>(A, ld, j0, SB,
+                                                                               j0 + SB, nt);
+            }
+        }
+        const int rest = n_pad - k0 - kw;
+        if (rest > 0) {
+            const int nt = rest / SB;
+            syrk_panel_kernel<<<nt * (nt + 1) / 2, 256, 0, st>>>(A, ld, k0, kw, k0 + kw, nt);
+        }
     }
 }
 
@@ -239,7 +334,7 @@ void launch_chol_solve(hipStream_t st, const double* L, int ld, int n_pad, doubl
 
 void launch_residual(hipStream_t st, const double* A, int ld, int n, double alpha, const double* x,
                      const double* b, double* r) {
-    residual_kernel<<<(n + 255) / 256, 256, 0, st>>>(A, ld, n, alpha, x, b, r);
+    residual_kernel<<<(n + 3) / 4, 256, 0, st>>>(A, ld, n, alpha, x, b, r);
 }
 
 void launch_add_diag(hipStream_t st, double* A, int ld, int n, double add) {

@@ -18,23 +18,20 @@ void launch_add_diag(hipStream_t st, double* A, int ld, int n, double add);
 void launch_axpy(hipStream_t st, int n, const double* d, double* x);
 int solve_block();
 
-void ridge_solve(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
-                 double* coeffs, int* info) {
-    if (F <= 0 || n_alpha <= 0) throw std::invalid_argument("empty ridge problem");
-    Device& dev = device();
-    hipStream_t st = dev.stream;
-    const int B = solve_block();
-    const int np = (F + B - 1) / B * B;
+void launch_gram_system(hipStream_t st, const double* G, int ldg, int F, const double* s,
+                        const unsigned char* zero, double* A, int lda, int n_pad, double* b);
+void launch_symv(hipStream_t st, const double* A, int lda, int n, const double* x, double* y);
+
+// For each alpha: Cholesky of (A0 + alpha I) (A0: full symmetric n_pad x
+// n_pad on the device, identity on the padding), the solve with b, one
+// refinement step with the residual of the unfactored system. coeffs: n_alpha
+// x F on the host.
+static void ridge_core(int F, int np, const DevBuf& A0, const DevBuf& b, const double* alphas,
+                       int n_alpha, double* coeffs, int* info) {
+    hipStream_t st = device().stream;
     const size_t mat = (size_t)np * np * 8;
-    std::vector<double> h((size_t)np * np, 0.0);
-    for (int i = 0; i < F; ++i) std::memcpy(&h[(size_t)i * np], xtx + (size_t)i * F, (size_t)F * 8);
-    for (int i = F; i < np; ++i) h[(size_t)i * np + i] = 1.0;
-    std::vector<double> hb(np, 0.0);
-    std::memcpy(hb.data(), xty, (size_t)F * 8);
-    DevBuf A0, Af, b, x, r, di;
-    A0.alloc(mat), Af.alloc(mat), b.alloc(np * 8), x.alloc(np * 8), r.alloc(np * 8), di.alloc(4);
-    check_hip(hipMemcpy(A0.ptr, h.data(), mat, hipMemcpyHostToDevice), "copy");
-    check_hip(hipMemcpy(b.ptr, hb.data(), np * 8, hipMemcpyHostToDevice), "copy");
+    DevBuf Af, x, r, di;
+    Af.alloc(mat), x.alloc(np * 8), r.alloc(np * 8), di.alloc(4);
     for (int k = 0; k < n_alpha; ++k) {
         const double a = alphas[k];
         check_hip(hipMemcpyAsync(Af.ptr, A0.ptr, mat, hipMemcpyDeviceToDevice, st), "copy");
@@ -45,13 +42,17 @@ void ridge_solve(int F, const double* xtx, const double* xty, const double* alph
             launch_cholesky(st, (double*)Af.ptr, np, np, (int*)di.ptr);
         }
         check_hip(hipGetLastError(), "cholesky launch");
-        check_hip(hipMemcpyAsync(x.ptr, b.ptr, np * 8, hipMemcpyDeviceToDevice, st), "copy");
-        launch_chol_solve(st, (double*)Af.ptr, np, np, (double*)x.ptr);
-        // one step of iterative refinement
-        check_hip(hipMemsetAsync(r.ptr, 0, np * 8, st), "memset");
-        launch_residual(st, (double*)A0.ptr, np, F, a, (double*)x.ptr, (double*)b.ptr, (double*)r.ptr);
-        launch_chol_solve(st, (double*)Af.ptr, np, np, (double*)r.ptr);
-        launch_axpy(st, F, (double*)r.ptr, (double*)x.ptr);
+        {
+            ProfScope ps("chol_solve", st);
+            check_hip(hipMemcpyAsync(x.ptr, b.ptr, np * 8, hipMemcpyDeviceToDevice, st), "copy");
+            launch_chol_solve(st, (double*)Af.ptr, np, np, (double*)x.ptr);
+            // one step of iterative refinement
+            check_hip(hipMemsetAsync(r.ptr, 0, np * 8, st), "memset");
+            launch_residual(st, (double*)A0.ptr, np, F, a, (double*)x.ptr, (double*)b.ptr,
+                            (double*)r.ptr);
+            launch_chol_solve(st, (double*)Af.ptr, np, np, (double*)r.ptr);
+            launch_axpy(st, F, (double*)r.ptr, (double*)x.ptr);
+        }
         check_hip(hipGetLastError(), "solve launch");
         int hi = 0;
         check_hip(hipMemcpyAsync(coeffs + (size_t)k * F, x.ptr, (size_t)F * 8, hipMemcpyDeviceToHost, st), "copy");
@@ -59,6 +60,93 @@ void ridge_solve(int F, const double* xtx, const double* xty, const double* alph
         check_hip(hipStreamSynchronize(st), "sync");
         info[k] = hi;
     }
+}
+
+void ridge_solve(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
+                 double* coeffs, int* info) {
+    if (F <= 0 || n_alpha <= 0) throw std::invalid_argument("empty ridge problem");
+    const int B = solve_block();
+    const int np = (F + B - 1) / B * B;
+    const size_t mat = (size_t)np * np * 8;
+    std::vector<double> h((size_t)np * np, 0.0);
+    for (int i = 0; i < F; ++i) std::memcpy(&h[(size_t)i * np], xtx + (size_t)i * F, (size_t)F * 8);
+    for (int i = F; i < np; ++i) h[(size_t)i * np + i] = 1.0;
+    std::vector<double> hb(np, 0.0);
+    std::memcpy(hb.data(), xty, (size_t)F * 8);
+    DevBuf A0, b;
+    A0.alloc(mat), b.alloc(np * 8);
+    check_hip(hipMemcpy(A0.ptr, h.data(), mat, hipMemcpyHostToDevice), "copy");
+    check_hip(hipMemcpy(b.ptr, hb.data(), np * 8, hipMemcpyHostToDevice), "copy");
+    ridge_core(F, np, A0, b, alphas, n_alpha, coeffs, info);
+}
+
+// The scaled system of an accumulator, built on the device: A0 (np x np),
+// b (np), and y^T y / b on the host for the squared-error sums.
+static int build_system(NormalHandle& acc, const double* scales, const unsigned char* zero,
+                        DevBuf& A0, DevBuf& b, std::vector<double>& hb, double& yty) {
+    const int F = acc.F, B = solve_block();
+    const int np = (F + B - 1) / B * B;
+    hipStream_t st = device().stream;
+    DevBuf ds, dz;
+    ds.alloc((size_t)F * 8), dz.alloc((size_t)F);
+    std::vector<unsigned char> z(F, 0);
+    if (zero) std::memcpy(z.data(), zero, F);
+    for (int i = 0; i < F; ++i)
+        if (!z[i] && !(scales[i] != 0.0)) throw std::invalid_argument("zero scale on a kept column");
+    check_hip(hipMemcpyAsync(ds.ptr, scales, (size_t)F * 8, hipMemcpyHostToDevice, st), "copy");
+    check_hip(hipMemcpyAsync(dz.ptr, z.data(), (size_t)F, hipMemcpyHostToDevice, st), "copy");
+    A0.alloc((size_t)np * np * 8), b.alloc((size_t)np * 8);
+    launch_gram_system(st, (const double*)acc.xtx.ptr, acc.F_pad, F, (const double*)ds.ptr,
+                       (const unsigned char*)dz.ptr, (double*)A0.ptr, np, np, (double*)b.ptr);
+    check_hip(hipGetLastError(), "system launch");
+    hb.assign(np, 0.0);
+    check_hip(hipMemcpyAsync(hb.data(), b.ptr, (size_t)np * 8, hipMemcpyDeviceToHost, st), "copy");
+    check_hip(hipMemcpyAsync(&yty, (const double*)acc.xtx.ptr + (size_t)F * acc.F_pad + F, 8,
+                             hipMemcpyDeviceToHost, st), "copy");
+    check_hip(hipStreamSynchronize(st), "sync");
+    return np;
+}
+
+// c^T A0 c - 2 c^T b + y^T y for each coefficient vector (rows of coeffs)
+static void sq_errors(int F, int np, const DevBuf& A0, const std::vector<double>& hb, double yty,
+                      const double* coeffs, int n_coef, double* out) {
+    hipStream_t st = device().stream;
+    DevBuf dc, dv;
+    dc.alloc((size_t)np * 8), dv.alloc((size_t)np * 8);
+    std::vector<double> c(np, 0.0), v(np);
+    for (int k = 0; k < n_coef; ++k) {
+        std::memcpy(c.data(), coeffs + (size_t)k * F, (size_t)F * 8);
+        check_hip(hipMemcpyAsync(dc.ptr, c.data(), (size_t)np * 8, hipMemcpyHostToDevice, st), "copy");
+        launch_symv(st, (const double*)A0.ptr, np, np, (const double*)dc.ptr, (double*)dv.ptr);
+        check_hip(hipGetLastError(), "symv launch");
+        check_hip(hipMemcpyAsync(v.data(), dv.ptr, (size_t)np * 8, hipMemcpyDeviceToHost, st), "copy");
+        check_hip(hipStreamSynchronize(st), "sync");
+        double cav = 0.0, cb = 0.0;
+        for (int i = 0; i < F; ++i) cav += c[i] * v[i], cb += c[i] * hb[i];
+        out[k] = cav - 2.0 * cb + yty;
+    }
+}
+
+void normal_ridge(NormalHandle& acc, const double* scales, const unsigned char* zero,
+                  const double* alphas, int n_alpha, double* coeffs, int* info, double* sq_err) {
+    if (n_alpha <= 0) throw std::invalid_argument("no alpha");
+    check_hip(hipStreamSynchronize(device().stream), "sync");
+    DevBuf A0, b;
+    std::vector<double> hb;
+    double yty = 0.0;
+    const int np = build_system(acc, scales, zero, A0, b, hb, yty);
+    ridge_core(acc.F, np, A0, b, alphas, n_alpha, coeffs, info);
+    if (sq_err) sq_errors(acc.F, np, A0, hb, yty, coeffs, n_alpha, sq_err);
+}
+
+void normal_sq_error(NormalHandle& acc, const double* scales, const unsigned char* zero,
+                     const double* coeffs, int n_coef, double* sq_err) {
+    check_hip(hipStreamSynchronize(device().stream), "sync");
+    DevBuf A0, b;
+    std::vector<double> hb;
+    double yty = 0.0;
+    const int np = build_system(acc, scales, zero, A0, b, hb, yty);
+    sq_errors(acc.F, np, A0, hb, yty, coeffs, n_coef, sq_err);
 }
 
 }  // namespace plm

@@ -302,15 +302,64 @@ class FitResult:
         return self.coeffs / self.scales
 
 
+@dataclass
+class DeviceNormal:
+    """Normal equations left on the GPU: the accumulator (X^T X, X^T y, y^T y
+    of [X | y]) and what the host needs from it."""
+    acc: capi.Normal
+    scales: np.ndarray
+    zero: np.ndarray
+    min_energy: float
+    total_n_data: int
+    y_sq_norm: float
+
+
+def accumulate_normal(model: capi.Model, datasets: Sequence[Dataset], precision=capi.PLM_FP64,
+                      scales: Optional[np.ndarray] = None, min_energy: Optional[float] = None,
+                      weight_stress: float = 0.1, batch_size: int = 4096,
+                      scale_threshold: float = 1e-10) -> DeviceNormal:
+    """calc_xtx_xty (data_sequential.py:25-94) without reading X^T X back: the
+    accumulation is the same; only the energy-row sums and y^T y come to the
+    host, where the scales are computed (utils_scales.py:6-40). The scaling
+    of X^T X and X^T y is applied on the device by DeviceNormal.acc.ridge /
+    sq_error."""
+    if min_energy is None:
+        min_energy = get_min_energy(datasets)
+    acc = capi.Normal(model)
+    for ds in datasets:
+        ds.sort_by_atoms()
+        _accumulate(model, acc, ds, 0, len(ds.structures), min_energy, weight_stress,
+                    batch_size, precision)
+    xe_sum, xe_sq, y_sq, _ = acc.vectors()
+    n_data = sum(len(d.energies) for d in datasets)
+    scales, zero = compute_scales(None if scales is None else np.array(scales, dtype=float),
+                                  xe_sum, xe_sq, n_data, include_force=datasets[0].include_force,
+                                  threshold=scale_threshold)
+    return DeviceNormal(acc=acc, scales=scales, zero=zero, min_energy=float(min_energy),
+                        total_n_data=sum(reference_n_rows(d) for d in datasets), y_sq_norm=y_sq)
+
+
+def _rmse_from_sq(sq, n_data):
+    """compute_rmse (utils_model_selection.py:37-47) from the squared-error sums."""
+    mse = np.asarray(sq, dtype=float) / n_data
+    return np.where(mse >= 0, np.sqrt(np.abs(mse)), 1e10)
+
+
 def fit(model: capi.Model, train: List[Dataset], test: List[Dataset], alphas,
         precision=capi.PLM_FP64) -> FitResult:
-    """Sequential ridge fit with model selection on the test RMSE (fit.py:12-63)."""
-    tr = calc_xtx_xty(model, train, precision=precision)
-    coefs = solver_ridge(tr.xtx, tr.xty, alphas)
-    rmse_train = compute_rmse(coefs, tr)
-    te = calc_xtx_xty(model, test, precision=precision, scales=tr.scales.copy(),
-                      min_energy=tr.min_energy)
-    rmse_test = compute_rmse(coefs, te)
+    """Sequential ridge fit with model selection on the test RMSE (fit.py:12-63).
+    The normal equations stay on the GPU: they are scaled and solved there
+    (plm_normal_ridge), and the training and test RMSEs come from device
+    products with them (plm_normal_sq_error); only vectors cross to the host."""
+    alphas = np.asarray(alphas, dtype=float)
+    tr = accumulate_normal(model, train, precision=precision)
+    coefs, info, sq = tr.acc.ridge(tr.scales, tr.zero, alphas)
+    sq = np.where(info == 0, sq, np.inf)
+    rmse_train = _rmse_from_sq(sq, tr.total_n_data)
+    te = accumulate_normal(model, test, precision=precision, scales=tr.scales.copy(),
+                           min_energy=tr.min_energy)
+    sq_te = np.where(info == 0, te.acc.sq_error(te.scales, te.zero, coefs), np.inf)
+    rmse_test = _rmse_from_sq(sq_te, te.total_n_data)
     k = int(np.argmin(rmse_test))
     return FitResult(coeffs=coefs[:, k], scales=tr.scales, alpha=float(alphas[k]),
                      rmse_train=float(rmse_train[k]), rmse_test=float(rmse_test[k]))

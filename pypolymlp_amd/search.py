@@ -9,8 +9,9 @@ fit only the candidates the network predicts near the Pareto front":
   cost (ms per atom per step) src/pypolymlp/postproc/count_time.py:15-181
                              (here: GPU evaluation of copies of a 256-atom cell)
   optimal models             src/pypolymlp/utils/grid_optimal.py:10-68
-                             (lower convex hull of (cost, energy RMSE),
-                             RMSE < 30 meV/atom)
+                             (convex hull of (cost, energy RMSE); vertices
+                             no other vertex beats in cost, energy and
+                             force RMSE; energy RMSE < 30 meV/atom)
 
 The surrogate (FP32 MLP 5-64-64-2, SiLU; pypolymlp_amd.capi.surrogate_*)
 maps (cutoff, Gaussians, max_l, polynomial order, log10 F) to (log10 RMSE,
@@ -129,24 +130,42 @@ def evaluate(c: Candidate, train: Dataset, test: Dataset, alphas=(1e-4, 1e-3, 1e
     return Result(c, True, model.n_features, rmse_e, rmse_f, cost, time.perf_counter() - t0)
 
 
-def pareto(points: np.ndarray, threshold: float = RMSE_THRESHOLD) -> np.ndarray:
-    """Indices of the lower convex hull of (cost, rmse) with rmse < threshold
-    (grid_optimal.py:32-68 without the force column: vertices of the convex
-    hull that no other vertex beats in both cost and RMSE)."""
+def find_optimal(cost, rmse_e, rmse_f, use_force: bool = False, use_logscale_time: bool = False,
+                 threshold: float = RMSE_THRESHOLD) -> np.ndarray:
+    """Indices of the optimal models by the reference's rule
+    (_find_convex_mlps / _find_lower_convex, utils/grid_optimal.py:32-68), in
+    increasing cost:
+      1. vertices of the convex hull of (cost, energy RMSE), or (cost, force
+         RMSE) with use_force, cost on a log10 scale with use_logscale_time;
+      2. of those, the ones no other vertex beats strictly in cost, energy
+         RMSE and force RMSE at once;
+      3. energy RMSE below the threshold (30 meV/atom).
+    Fewer than three points, or collinear ones, have no hull: every point is
+    then a vertex."""
     from scipy.spatial import ConvexHull
 
+    cost = np.asarray(cost, dtype=float)
+    re = np.asarray(rmse_e, dtype=float)
+    rf = np.asarray(rmse_f, dtype=float)
+    order = np.argsort(cost, kind="stable")  # the reference sorts its table by cost
+    c, e, f = cost[order], re[order], rf[order]
+    tgt = np.column_stack([np.log10(c) if use_logscale_time else c, f if use_force else e])
+    try:
+        verts = np.unique(ConvexHull(tgt).simplices) if len(c) >= 3 else np.arange(len(c))
+    except Exception:  # degenerate (collinear) point sets
+        verts = np.arange(len(c))
+    keep = [i for i in verts
+            if not any(j != i and c[j] < c[i] and e[j] < e[i] and f[j] < f[i] for j in verts)]
+    keep = np.array([i for i in keep if e[i] < threshold], dtype=int)
+    return order[keep]
+
+
+def pareto(points: np.ndarray, threshold: float = RMSE_THRESHOLD) -> np.ndarray:
+    """find_optimal on rows (cost, energy RMSE, force RMSE)."""
     pts = np.asarray(points, dtype=float)
-    if len(pts) < 3:
-        idx = np.arange(len(pts))
-    else:
-        try:
-            idx = np.unique(ConvexHull(pts).simplices)
-        except Exception:  # degenerate (collinear) point sets
-            idx = np.arange(len(pts))
-    keep = [i for i in idx
-            if not any(j != i and pts[j, 0] < pts[i, 0] and pts[j, 1] < pts[i, 1] for j in idx)]
-    keep = np.array([i for i in keep if pts[i, 1] < threshold], dtype=int)
-    return keep[np.argsort(pts[keep, 0])] if len(keep) else keep
+    if not len(pts):
+        return np.zeros(0, dtype=int)
+    return find_optimal(pts[:, 0], pts[:, 1], pts[:, 2], threshold=threshold)
 
 
 def hypervolume(points: np.ndarray, ref: Sequence[float]) -> float:
@@ -296,9 +315,9 @@ def run_search(cands: Sequence[Candidate], evaluate_fn: Callable[[Candidate], Re
         good = [r for r in rs if r.ok]
         if not good:
             return [], []
-        pts = np.array([[r.cost, r.rmse_e] for r in good])
+        pts = np.array([[r.cost, r.rmse_e, r.rmse_f] for r in good])
         idx = pareto(pts)
-        return [good[i].cand.key for i in idx], pts[idx]
+        return [good[i].cand.key for i in idx], pts[idx][:, :2]
 
     fe, pe = front(exhaustive)
     fg, pg = front(guided)
@@ -312,13 +331,26 @@ def run_search(cands: Sequence[Candidate], evaluate_fn: Callable[[Candidate], Re
                         hypervolume(pr, ref) if len(pr) else 0.0)
 
 
+def search_data(n_structures: int = 250, n_test: int = 50):
+    """(train, test) of the search: the first n_structures - n_test seeded
+    binary-alloy structures (128-atom fcc 4x4x2 cells) for training, the last
+    n_test for the errors."""
+    from .datasets import config2_dataset
+
+    ds = config2_dataset(n_structures)
+    n_tr = n_structures - n_test
+
+    def sub(i0, i1, name):
+        return Dataset(structures=ds.structures[i0:i1], energies=ds.energies[i0:i1],
+                       forces=ds.forces[i0:i1], stresses=ds.stresses[i0:i1], name=name)
+    return sub(0, n_tr, "train"), sub(n_tr, n_structures, "test")
+
+
 def main(argv=None):
     """python -m pypolymlp_amd.search --out report.json: the cfg5 search on a
     seeded subset of the binary-alloy dataset (train/test split)."""
     import argparse
     import json
-
-    from .datasets import config2_dataset
 
     ap = argparse.ArgumentParser()
     ap.add_argument("--n-structures", type=int, default=250)
@@ -329,12 +361,8 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--out", default="search_report.json")
     a = ap.parse_args(argv)
-    ds = config2_dataset(a.n_structures)
+    train, test = search_data(a.n_structures, a.n_test)
     n_tr = a.n_structures - a.n_test
-    def sub(i0, i1, name):
-        return Dataset(structures=ds.structures[i0:i1], energies=ds.energies[i0:i1],
-                       forces=ds.forces[i0:i1], stresses=ds.stresses[i0:i1], name=name)
-    train, test = sub(0, n_tr, "train"), sub(n_tr, a.n_structures, "test")
     prec = capi.PLM_MIXED if a.precision == "mixed" else capi.PLM_FP64
     cs = cost_cell()
     cands = grid(a.grid)

@@ -178,3 +178,60 @@ def test_fit_cfg2_mixed_predictions():
         df = np.abs(pred[capi.PLM_MIXED][1] - pred[capi.PLM_FP64][1]).max()
         say(f"  alpha {a:g}: max |dE| {de * 1e3:.3e} meV/atom, max |dF| {df * 1e3:.3e} meV/A")
         assert de <= 1e-4 and df <= 1e-3
+
+
+def test_fit_device_resident_matches_host_path():
+    """The device-resident solve (X^T X scaled and solved on the GPU,
+    plm_normal_ridge / plm_normal_sq_error) against the reference-API path on
+    the same accumulator (its X^T X read back and scaled on the host as
+    calc_xtx_xty does, then solver_ridge and compute_rmse): the scaled
+    systems are bit-identical, so the coefficients agree to 1e-12 and the
+    RMSEs to 1e-10 (relative). Dropped columns (a zero mask) against numpy on
+    the read-back normal equations. fit() picks the same alpha as the host
+    path (the energy-row sums use device atomics, so scales from two
+    accumulations differ in the last bits; coefficients then agree to the
+    solve's conditioning)."""
+    say("device-resident fit vs host path (cfg1 model, 30 + 10 structures)")
+    spec = datasets.config1_model()
+    full = datasets.config1_dataset(40, seed=13)
+    train, test = full.subset(range(30), "train"), full.subset(range(30, 40), "test")
+    model = capi.Model(spec.as_dict())
+    alphas = np.array([1e-4, 1e-2, 1.0])
+    d = mlp_dev.accumulate_normal(model, [train])
+    tr = mlp_dev._finish_normal(d.acc.result(), len(train.energies), None, True, 1e-10,
+                                d.min_energy, n_rows=d.total_n_data)
+    assert np.array_equal(d.scales, tr.scales)
+    coefs = mlp_dev.solver_ridge(tr.xtx, tr.xty, alphas)
+    rmse_tr = mlp_dev.compute_rmse(coefs, tr)
+    c2, info, sq = d.acc.ridge(d.scales, d.zero, alphas)
+    assert np.all(info == 0)
+    for k in range(len(alphas)):
+        assert np.linalg.norm(c2[:, k] - coefs[:, k]) <= 1e-12 * np.linalg.norm(coefs[:, k])
+    np.testing.assert_allclose(mlp_dev._rmse_from_sq(sq, d.total_n_data), rmse_tr, rtol=1e-10)
+    dt = mlp_dev.accumulate_normal(model, [test], scales=d.scales.copy(), min_energy=d.min_energy)
+    te = mlp_dev._finish_normal(dt.acc.result(), len(test.energies), d.scales.copy(), True, 1e-10,
+                                d.min_energy, n_rows=dt.total_n_data)
+    rmse_te = mlp_dev.compute_rmse(coefs, te)
+    np.testing.assert_allclose(
+        mlp_dev._rmse_from_sq(dt.acc.sq_error(dt.scales, dt.zero, c2), dt.total_n_data), rmse_te,
+        rtol=1e-10)
+    k = int(np.argmin(rmse_te))
+    fr = mlp_dev.fit(model, [train], [test], alphas)
+    assert fr.alpha == alphas[k]
+    Aa = tr.xtx + alphas[k] * np.eye(len(tr.xty))
+    bound = max(100 * np.linalg.cond(Aa) * np.finfo(float).eps, 1e-12)
+    assert np.linalg.norm(fr.coeffs - coefs[:, k]) <= bound * np.linalg.norm(coefs[:, k])
+    # dropped columns: the device zeroes rows and columns of the mask
+    zero = np.zeros(len(d.scales), dtype=bool)
+    zero[[3, 100, 2000]] = True
+    a = 1.0
+    c3, info, _ = d.acc.ridge(d.scales, zero, [a])
+    xtx, xty = tr.xtx.copy(), tr.xty.copy()
+    xtx[zero] = 0.0
+    xtx[:, zero] = 0.0
+    xty[zero] = 0.0
+    Aa = xtx + a * np.eye(len(xty))
+    c_ref = np.linalg.solve(Aa, xty)
+    assert np.all(c3[zero, 0] == 0.0)
+    bound = max(100 * np.linalg.cond(Aa) * np.finfo(float).eps, 1e-12)
+    assert np.linalg.norm(c3[:, 0] - c_ref) <= bound * np.linalg.norm(c_ref)

@@ -337,3 +337,27 @@ def test_spin_models_are_refused():
     # models without spins (enable_spins [0, 0]) still load
     spec, coeffs = load_mlp_yaml(ref_file("polymlp.yaml.gtinv.MgO"))
     assert len(coeffs) == 1899
+
+
+def test_find_optimal_grid_ti_golden():
+    """The reference's optimal-model rule on its grid-Ti data: 285 models give
+    the 11 named optimal ones in cost order (tests/test_utils/test_grid_optimal.py:17-45).
+    The data are the reference's per-model cost and error files condensed by
+    tests/golden/make_grid_ti.py."""
+    import json
+    from pypolymlp_amd import search
+    d = json.load(open(os.path.join(ROOT, "tests", "golden", "grid_ti_summary.json")))["models"]
+    assert len(d) == 285
+    idx = search.find_optimal([m["cost_single"] for m in d], [m["rmse_energy"] for m in d],
+                              [m["rmse_force"] for m in d])
+    assert [d[i]["name"] for i in idx] == [
+        "polymlp-00035", "polymlp-00064", "polymlp-00067", "polymlp-00068", "polymlp-00097",
+        "polymlp-00074", "polymlp-00103", "polymlp-00076", "polymlp-00105", "polymlp-00281",
+        "polymlp-00284"]
+    # a hull vertex beaten in cost and energy but not in force stays optimal
+    # (grid_optimal.py:62-66 tests all three columns)
+    c, e, f = [1.0, 2.0, 3.0], [5.0, 8.0, 1.0], [0.2, 0.1, 0.5]
+    assert list(search.find_optimal(c, e, f)) == [0, 1, 2]
+    assert list(search.find_optimal(c, e, [0.2, 0.3, 0.5])) == [0, 2]
+    # the energy threshold (30 meV/atom) applies after the hull
+    assert list(search.find_optimal(c, [5.0, 8.0, 31.0], f)) == [0, 1]
