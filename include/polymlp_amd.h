@@ -193,6 +193,19 @@ int plm_solver_cg(int F, const double* xtx, const double* xty, double alpha, con
 int plm_eval(plm_model* m, const double* axis, const double* pos_c, const int* types, int n,
              int precision, double* energy, double* forces, double* stress);
 
+/* No counterpart in the reference. plm_eval for callers that cannot allocate
+ * device memory (serving loops, code inside a caller's stream capture,
+ * harnesses that forbid hipMalloc): the same kernels and results as plm_eval,
+ * with every device buffer (staged inputs, full and half neighbour lists,
+ * order parameters, per-atom contractions, outputs) carved out of one
+ * file-scope device array of 64 MiB per GPU, held until the process ends. It
+ * calls no hipMalloc / hipFree. The model must already hold its coefficients
+ * (plm_model_set_coeffs). A structure whose buffers exceed 64 MiB is refused
+ * with PLM_ERR_ARG (about 6,000 atoms of a 1-type order-2 max_l 4 model).
+ * Calls on one GPU share the array: overlapping calls on the same GPU race. */
+int plm_eval_noalloc(plm_model* m, const double* axis, const double* pos_c, const int* types,
+                     int n, int precision, double* energy, double* forces, double* stress);
+
 /* Evaluation of a device-resident batch (eval_multiple, py_properties_fast.cpp:30-77). */
 int plm_eval_batch(plm_model* m, plm_batch* b, int precision, double* energies, double* forces,
                    double* stresses);

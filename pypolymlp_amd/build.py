@@ -37,6 +37,7 @@ SOURCES = [
     ("profile.cpp", []),
     ("kernels_solve.hip", []),
     ("eval.cpp", []),
+    ("eval_noalloc.hip", []),
     ("kernels_eval.hip", []),
     ("kernels_x.hip", []),
     ("kernels_xrows.hip", []),

@@ -28,7 +28,7 @@ EXPORTS = [
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
     "plm_normal_ridge", "plm_normal_sq_error",
     "plm_solver_cg", "plm_model_feature_attrs",
-    "plm_eval", "plm_eval_batch", "plm_eval_graph_create", "plm_eval_graph_run",
+    "plm_eval", "plm_eval_noalloc", "plm_eval_batch", "plm_eval_graph_create", "plm_eval_graph_run",
     "plm_eval_graph_destroy", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
     "plm_surrogate_n_params", "plm_surrogate_train", "plm_surrogate_grad",
     "plm_surrogate_predict", "plm_get_fn", "plm_get_ylm", "plm_api_features",
@@ -92,6 +92,8 @@ def lib():
         L.plm_normal_ridge.argtypes = [vp, _dp, _bp, _dp, ctypes.c_int, _dp, _ip, _dp]
         L.plm_normal_sq_error.argtypes = [vp, _dp, _bp, _dp, ctypes.c_int, _dp]
         L.plm_eval.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.plm_eval_noalloc.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                       _dp]
         L.plm_eval_batch.argtypes = [vp, vp, ctypes.c_int, _dp, _dp, _dp]
         L.plm_eval_graph_create.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(vp)]
         L.plm_eval_graph_run.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp]
@@ -375,6 +377,20 @@ def eval_structure(model: Model, axis, positions_c, types, precision: int = PLM_
     s = np.zeros(6)
     _check(lib().plm_eval(model.handle, _pd(axis), _pd(pos), _pi(t), n, precision, _pd(e), _pd(f),
                           _pd(s)))
+    return float(e[0]), f, s
+
+
+def eval_structure_noalloc(model: Model, axis, positions_c, types, precision: int = PLM_FP64):
+    """eval_structure through plm_eval_noalloc: no device allocation in the call
+    (the model must already hold its coefficients)."""
+    axis, pos = _d(axis), _d(positions_c)
+    t = _i(types)
+    n = len(t)
+    e = np.zeros(1)
+    f = np.zeros((n, 3))
+    s = np.zeros(6)
+    _check(lib().plm_eval_noalloc(model.handle, _pd(axis), _pd(pos), _pi(t), n, precision, _pd(e),
+                                  _pd(f), _pd(s)))
     return float(e[0]), f, s
 
 

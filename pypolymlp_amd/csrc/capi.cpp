@@ -1048,6 +1048,16 @@ int plm_eval(plm_model* mp, const double* axis, const double* pos_c, const int* 
     });
 }
 
+int plm_eval_noalloc(plm_model* mp, const double* axis, const double* pos_c, const int* types,
+                     int n, int precision, double* energy, double* forces, double* stress) {
+    PLM_TRY({
+        if (!mp || !axis || !pos_c || !types) throw std::invalid_argument("null argument");
+        auto* m = reinterpret_cast<ModelHandle*>(mp);
+        eval_single_noalloc(*m, axis, pos_c, types, n, precision == PLM_FP64, energy, forces,
+                            stress);
+    });
+}
+
 int plm_eval_batch(plm_model* mp, plm_batch* bp, int precision, double* energies, double* forces,
                    double* stresses) {
     PLM_TRY({

@@ -205,6 +205,10 @@ void eval_graph_capture(EvalGraph& g);
 void eval_graph_run(EvalGraph& g, int steps, double* energies, double* forces, double* stresses);
 void eval_single(ModelHandle& m, const double* axis, const double* pos_c, const int* types, int n,
                  bool fp64, double* energy, double* forces, double* stress);
+// plm_eval with every device buffer in a fixed file-scope device array
+// (eval_noalloc.hip); refuses structures whose buffers exceed it
+void eval_single_noalloc(ModelHandle& m, const double* axis, const double* pos_c, const int* types,
+                         int n, bool fp64, double* energy, double* forces, double* stress);
 void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, double* forces,
                 double* stresses);
 void ridge_solve(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
