@@ -124,13 +124,10 @@ def eval_cpu_baseline(model_yaml, st, steps=1):
     for _ in range(steps):
         R.eval(ax, pos, types, use_openmp=True)
     dt = (time.perf_counter() - t0) / steps
-    t1 = time.perf_counter()
-    ref.neighbor_list(ax, pos, 6.0, half=True)
-    t_nb = time.perf_counter() - t1
     out = {"value": st.n_total / dt, "unit": "atoms/s", "cores": cpu_threads(),
            "kind": "reference",
-           "sample": f"{steps} step(s) of the {st.n_total}-atom cell: {dt:.2f} s per step, of "
-                     f"which the half neighbour list (timed alone) {t_nb:.2f} s"}
+           "sample": f"{steps} step(s) of the {st.n_total}-atom cell (NeighborHalf + "
+                     f"PolymlpEval::eval, OpenMP): {dt:.2f} s per step"}
     out.update(host_info())
     return out
 
@@ -277,18 +274,32 @@ def fit_end_to_end_leg(n_st: int, alphas=(1e-3,)):
     rows = mlp_dev.reference_n_rows(ds)
     out = {}
     for name, prec in (("mixed", capi.PLM_MIXED), ("fp64", capi.PLM_FP64)):
+        # the reference-API path: X^T X read back, scaled on the host, solved
         t0 = time.perf_counter()
         ne = mlp_dev.calc_xtx_xty(model, [ds], precision=prec)
         t1 = time.perf_counter()
         mlp_dev.solver_ridge(ne.xtx, ne.xty, np.asarray(alphas))
         t2 = time.perf_counter()
-        out[name] = {"rows_per_s": rows / (t2 - t0), "seconds": t2 - t0,
-                     "calc_xtx_xty_s": t1 - t0, "solver_ridge_s": t2 - t1}
-        log(f"end-to-end {name}: {t2 - t0:.2f} s ({rows / (t2 - t0):.4g} rows/s), "
-            f"calc_xtx_xty {t1 - t0:.2f} s, ridge {t2 - t1:.2f} s")
+        del ne
+        # the path of mlp_dev.fit: the normal equations stay on the device
+        t3 = time.perf_counter()
+        dn = mlp_dev.accumulate_normal(model, [ds], precision=prec)
+        t4 = time.perf_counter()
+        dn.acc.ridge(dn.scales, dn.zero, np.asarray(alphas))
+        t5 = time.perf_counter()
+        del dn
+        out[name] = {"rows_per_s": rows / (t5 - t3), "seconds": t5 - t3,
+                     "accumulate_s": t4 - t3, "ridge_s": t5 - t4,
+                     "read_back_path": {"rows_per_s": rows / (t2 - t0), "seconds": t2 - t0,
+                                        "calc_xtx_xty_s": t1 - t0, "solver_ridge_s": t2 - t1}}
+        log(f"end-to-end {name}: device-resident {t5 - t3:.2f} s ({rows / (t5 - t3):.4g} rows/s;"
+            f" accumulate {t4 - t3:.2f} s, ridge {t5 - t4:.2f} s); read-back path "
+            f"{t2 - t0:.2f} s (calc_xtx_xty {t1 - t0:.2f} s, ridge {t2 - t1:.2f} s)")
     out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
-    out["workload"] = (f"{n_st} structures of the workload, cold calc_xtx_xty + solver_ridge "
-                       f"at alpha {list(alphas)}")
+    out["workload"] = (f"{n_st} structures of the workload, cold: row weights, batch upload and "
+                       "neighbour lists, accumulation, scaling and the ridge solve at alpha "
+                       f"{list(alphas)} (mlp_dev.accumulate_normal + plm_normal_ridge, the path "
+                       "of mlp_dev.fit); read_back_path: calc_xtx_xty + solver_ridge")
     return {"fit_end_to_end": out}
 
 

@@ -264,7 +264,9 @@ long ref_neighbor(const double* axis9, const double* pos, int n, double cutoff, 
         return total;
     };
     if (half) {
-        NeighborHalf nb(axis, p, cutoff, false);
+        // OpenMP over atoms: the per-atom lists are built independently and
+        // concatenated in atom order (neighbor_half.cpp:30-94), as serially
+        NeighborHalf nb(axis, p, cutoff, true);
         return fill(nb);
     }
     NeighborFull nb(axis, p, cutoff);

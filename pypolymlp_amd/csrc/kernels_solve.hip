@@ -16,44 +16,38 @@ constexpr int SB = 64;  // block size
 
 typedef __attribute__((__vector_size__(4 * sizeof(double)))) double doublex4;
 
-// Unblocked Cholesky of the diagonal block at (k0, k0) in LDS.
-__global__ void __launch_bounds__(256) potf2_kernel(double* __restrict__ A, int ld, int k0,
-                                                    int* __restrict__ info) {
+// Unblocked Cholesky of the diagonal block at (k0, k0) in LDS, one wave:
+// lane i owns row i; column step j scales column j below the diagonal, then
+// each lane updates its own row (a[i][c] -= l_ij l_cj, c in (j, i]).
+__global__ void __launch_bounds__(SB) potf2_kernel(double* __restrict__ A, int ld, int k0,
+                                                   int* __restrict__ info) {
     __shared__ double a[SB][SB + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < SB * SB; e += 256) {
-        const int i = e / SB, j = e % SB;
-        a[i][j] = (j <= i) ? A[(size_t)(k0 + i) * ld + k0 + j] : 0.0;
-    }
+    const int i = threadIdx.x;
+    for (int c = 0; c < SB; ++c) a[i][c] = (c <= i) ? A[(size_t)(k0 + i) * ld + k0 + c] : 0.0;
     __syncthreads();
     for (int j = 0; j < SB; ++j) {
-        if (tid == 0) {
-            double d = a[j][j];
-            if (!(d > 0.0)) {
-                if (*info == 0) *info = k0 + j + 1;
-                d = 1.0;  // keep the factor finite; the caller discards this solve
-            }
-            a[j][j] = sqrt(d);
+        double d = a[j][j];
+        if (!(d > 0.0)) {
+            if (i == j && *info == 0) *info = k0 + j + 1;
+            d = 1.0;  // keep the factor finite; the caller discards this solve
         }
+        const double djj = sqrt(d);
         __syncthreads();
-        const double djj = a[j][j];
-        for (int i = j + 1 + tid; i < SB; i += 256) a[i][j] /= djj;
+        if (i == j) a[j][j] = djj;
+        if (i > j) a[i][j] /= djj;
         __syncthreads();
-        const int m = SB - j - 1;
-        for (int e = tid; e < m * m; e += 256) {
-            const int i = j + 1 + e / m, c = j + 1 + e % m;
-            if (c <= i) a[i][c] -= a[i][j] * a[c][j];
+        if (i > j) {
+            const double lij = a[i][j];
+            for (int c = j + 1; c <= i; ++c) a[i][c] -= lij * a[c][j];
         }
         __syncthreads();
     }
-    for (int e = tid; e < SB * SB; e += 256) {
-        const int i = e / SB, j = e % SB;
-        if (j <= i) A[(size_t)(k0 + i) * ld + k0 + j] = a[i][j];
-    }
+    for (int c = 0; c <= i; ++c) A[(size_t)(k0 + i) * ld + k0 + c] = a[i][c];
 }
 
-// Panel: rows i in [k0+SB, n): A[i, k0:k0+SB] <- A[i, k0:k0+SB] L_kk^{-T}.
-// One thread per row, the diagonal block in LDS.
+// Panel: rows i in [k0+SB, n): A[i, k0:k0+SB] <- A[i, k0:k0+SB] L_kk^{-T} by
+// forward substitution, one thread per row, the diagonal block in LDS; the
+// loops are unrolled so that the row stays in registers.
 __global__ void __launch_bounds__(256) trsm_panel_kernel(double* __restrict__ A, int ld, int k0,
                                                          int n) {
     __shared__ double L[SB][SB + 1];
@@ -66,12 +60,16 @@ __global__ void __launch_bounds__(256) trsm_panel_kernel(double* __restrict__ A,
     if (i >= n) return;
     double x[SB];
     double* row = A + (size_t)i * ld + k0;
+#pragma unroll
     for (int c = 0; c < SB; ++c) x[c] = row[c];
+#pragma unroll
     for (int c = 0; c < SB; ++c) {
         double v = x[c];
+#pragma unroll
         for (int cp = 0; cp < c; ++cp) v -= x[cp] * L[c][cp];
         x[c] = v / L[c][c];
     }
+#pragma unroll
     for (int c = 0; c < SB; ++c) row[c] = x[c];
 }
 
@@ -121,48 +119,42 @@ __global__ void __launch_bounds__(256) syrk_panel_kernel(double* __restrict__ A,
             }
 }
 
-// Forward substitution on one diagonal block: z_k = L_kk^{-1} z_k.
-__global__ void __launch_bounds__(64) trsv_lower_block(const double* __restrict__ L, int ld, int k0,
-                                                       double* __restrict__ z) {
+// Forward substitution on one diagonal block, z_k = L_kk^{-1} z_k, one wave:
+// step c finishes z_c, then every lane below subtracts its term (each z_i sees
+// the same operations in the same order as row-wise substitution).
+__global__ void __launch_bounds__(SB) trsv_lower_block(const double* __restrict__ L, int ld,
+                                                       int k0, double* __restrict__ z) {
     __shared__ double Lb[SB][SB + 1];
     __shared__ double zb[SB];
-    for (int e = threadIdx.x; e < SB * SB; e += 64) {
-        const int i = e / SB, j = e % SB;
-        Lb[i][j] = (j <= i) ? L[(size_t)(k0 + i) * ld + k0 + j] : 0.0;
-    }
-    zb[threadIdx.x] = z[k0 + threadIdx.x];
+    const int t = threadIdx.x;
+    for (int j = 0; j <= t; ++j) Lb[t][j] = L[(size_t)(k0 + t) * ld + k0 + j];
+    zb[t] = z[k0 + t];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int c = 0; c < SB; ++c) {
-            double v = zb[c];
-            for (int cp = 0; cp < c; ++cp) v -= Lb[c][cp] * zb[cp];
-            zb[c] = v / Lb[c][c];
-        }
+    for (int c = 0; c < SB; ++c) {
+        if (t == c) zb[c] /= Lb[c][c];
+        __syncthreads();
+        if (t > c) zb[t] -= Lb[t][c] * zb[c];
+        __syncthreads();
     }
-    __syncthreads();
-    z[k0 + threadIdx.x] = zb[threadIdx.x];
+    z[k0 + t] = zb[t];
 }
 
-// Backward substitution on one diagonal block: x_k = L_kk^{-T} x_k.
-__global__ void __launch_bounds__(64) trsv_upper_block(const double* __restrict__ L, int ld, int k0,
-                                                       double* __restrict__ z) {
+// Backward substitution on one diagonal block, x_k = L_kk^{-T} x_k, one wave.
+__global__ void __launch_bounds__(SB) trsv_upper_block(const double* __restrict__ L, int ld,
+                                                       int k0, double* __restrict__ z) {
     __shared__ double Lb[SB][SB + 1];
     __shared__ double zb[SB];
-    for (int e = threadIdx.x; e < SB * SB; e += 64) {
-        const int i = e / SB, j = e % SB;
-        Lb[i][j] = (j <= i) ? L[(size_t)(k0 + i) * ld + k0 + j] : 0.0;
-    }
-    zb[threadIdx.x] = z[k0 + threadIdx.x];
+    const int t = threadIdx.x;
+    for (int j = 0; j <= t; ++j) Lb[t][j] = L[(size_t)(k0 + t) * ld + k0 + j];
+    zb[t] = z[k0 + t];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int c = SB - 1; c >= 0; --c) {
-            double v = zb[c];
-            for (int cp = c + 1; cp < SB; ++cp) v -= Lb[cp][c] * zb[cp];
-            zb[c] = v / Lb[c][c];
-        }
+    for (int c = SB - 1; c >= 0; --c) {
+        if (t == c) zb[c] /= Lb[c][c];
+        __syncthreads();
+        if (t < c) zb[t] -= Lb[c][t] * zb[c];
+        __syncthreads();
     }
-    __syncthreads();
-    z[k0 + threadIdx.x] = zb[threadIdx.x];
+    z[k0 + t] = zb[t];
 }
 
 // z[i] -= sum_c L[i][k0+c] z[k0+c] for rows i >= k0+SB (after a forward block).
@@ -301,7 +293,7 @@ void launch_cholesky(hipStream_t st, double* A, int ld, int n_pad, int* info) {
     for (int k0 = 0; k0 < n_pad; k0 += NB) {
         const int kw = min(NB, n_pad - k0);
         for (int j0 = k0; j0 < k0 + kw; j0 += SB) {
-            potf2_kernel<<<1, 256, 0, st>>>(A, ld, j0, info);
+            potf2_kernel<<<1, SB, 0, st>>>(A, ld, j0, info);
             const int rest = n_pad - j0 - SB;
             if (rest <= 0) break;
             trsm_panel_kernel<<<(rest + 255) / 256, 256, 0, st>>>(A, ld, j0, n_pad);

@@ -106,6 +106,38 @@ def test_large_cell_binned_lists_and_eval(tmp_path):
     assert np.linalg.norm(s - sr) <= 1e-10 * np.linalg.norm(sr)
 
 
+@pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+def test_eval_32000_atom_cell_vs_oracle(tmp_path):
+    """The 32,000-atom bcc cell of the evaluation workload (20 x 20 x 40,
+    a = 3.30 A, sigma = 0.1 A) with the 1-type order-2 max_l 4 model: the
+    half neighbour list bit-exact (offsets, neighbours, differences) and FP64
+    energy, forces and stress to 1e-10 (relative, normwise) against the
+    reference's NeighborHalf and PolymlpEval; the mixed evaluation within
+    1 meV/A and 0.1 meV/atom of FP64."""
+    st = datasets.config4_structure()
+    ax, pos = st.axis, st.positions_cartesian
+    say(f"32000-atom half list n={st.n_total}")
+    off, nb, tid, d = capi.neighbor(ax, pos, 6.0, half=True)
+    ro, rn, rd = ref.neighbor_list(ax, pos, 6.0, half=True)
+    assert np.array_equal(off, ro)
+    assert np.array_equal(nb, rn)
+    assert np.array_equal(d, rd)
+    spec = datasets.config1_model()
+    m = capi.Model(spec.as_dict())
+    coeffs = np.random.default_rng(4).normal(size=m.n_features) * 1e-3
+    m.set_coeffs(coeffs)
+    say("32000-atom eval FP64 and mixed")
+    e, f, s = capi.eval_structure(m, ax, pos, st.types)
+    em, fm, sm = capi.eval_structure(m, ax, pos, st.types, capi.PLM_MIXED)
+    path = write_model_yaml(spec, str(tmp_path / "m.yaml"), coeffs)
+    er, fr, sr = ref.RefModel(path, with_eval=True).eval(ax, pos, st.types)
+    assert e == pytest.approx(er, rel=1e-10)
+    assert np.linalg.norm(f - fr) <= 1e-10 * np.linalg.norm(fr)
+    assert np.linalg.norm(s - sr) <= 1e-10 * np.linalg.norm(sr)
+    assert abs(em - e) / st.n_total < 1e-4
+    assert np.abs(fm - f).max() < 1e-3
+
+
 # --- legacy-format models, hybrid "flexible" models, skewed cells ------------
 from tests.test_oracle_golden import MGO_HNF, MGO_HNF_ENERGY, SRTIO3_LEGACY  # noqa: E402
 

@@ -207,14 +207,24 @@ def test_fit_device_resident_matches_host_path():
     assert np.all(info == 0)
     for k in range(len(alphas)):
         assert np.linalg.norm(c2[:, k] - coefs[:, k]) <= 1e-12 * np.linalg.norm(coefs[:, k])
-    np.testing.assert_allclose(mlp_dev._rmse_from_sq(sq, d.total_n_data), rmse_tr, rtol=1e-10)
+    # the squared-error sum c^T A c - 2 c^T b + y^T y cancels: its rounding is
+    # bounded by the size of its terms, in any summation order
+    def sq_close(sq_dev, ne, C):
+        for k in range(C.shape[1]):
+            c = C[:, k]
+            cac, cb = c @ (ne.xtx @ c), c @ ne.xty
+            host = cac - 2 * cb + ne.y_sq_norm
+            assert abs(sq_dev[k] - host) <= 1e-12 * (abs(cac) + 2 * abs(cb) + ne.y_sq_norm)
+
+    sq_close(sq, tr, coefs)
+    np.testing.assert_allclose(mlp_dev._rmse_from_sq(sq, d.total_n_data), rmse_tr, rtol=1e-5)
     dt = mlp_dev.accumulate_normal(model, [test], scales=d.scales.copy(), min_energy=d.min_energy)
     te = mlp_dev._finish_normal(dt.acc.result(), len(test.energies), d.scales.copy(), True, 1e-10,
                                 d.min_energy, n_rows=dt.total_n_data)
     rmse_te = mlp_dev.compute_rmse(coefs, te)
-    np.testing.assert_allclose(
-        mlp_dev._rmse_from_sq(dt.acc.sq_error(dt.scales, dt.zero, c2), dt.total_n_data), rmse_te,
-        rtol=1e-10)
+    sq_te = dt.acc.sq_error(dt.scales, dt.zero, c2)
+    sq_close(sq_te, te, coefs)
+    np.testing.assert_allclose(mlp_dev._rmse_from_sq(sq_te, dt.total_n_data), rmse_te, rtol=1e-5)
     k = int(np.argmin(rmse_te))
     fr = mlp_dev.fit(model, [train], [test], alphas)
     assert fr.alpha == alphas[k]
