@@ -32,11 +32,14 @@ extern "C" {
 #define PLM_MIXED 1 /* FP32 features; X^T X as the exact Gram of X quantised per
                      * (row chunk, column) to a shared exponent and three signed
                      * 7-bit slices, on int8 MFMA with int32 sums combined in FP64 */
+#define PLM_MIXED64 2 /* FP64 features with the quantised int8 Gram of PLM_MIXED
+                       * (X^T X accumulation; elsewhere the same as PLM_FP64) */
 
 typedef struct plm_model plm_model;
 typedef struct plm_batch plm_batch;
 typedef struct plm_normal plm_normal;
 typedef struct plm_eval_graph plm_eval_graph;
+typedef struct plm_normal_graph plm_normal_graph;
 
 /* Model description: the contents of the params dict the reference's
  * convert_params_dict_to_feature_params reads
@@ -124,6 +127,19 @@ int plm_normal_accumulate(plm_normal* acc, plm_batch* b, const double* row_w, co
  * Any output may be NULL. */
 int plm_normal_result(plm_normal* acc, double* xtx, double* xty, double* xe_sum,
                       double* xe_sq_sum, double* y_sq, long* n_rows);
+/* The fit step captured as a HIP graph: one plm_normal_accumulate pass of a
+ * device-resident batch with fixed row weights and targets
+ * (_compute_products_single_batch, data_sequential.py:97-155, over the
+ * batch's sub-batches: X rows, quantisation and Gram) recorded once and
+ * replayed. Each replay adds the batch's contribution to the accumulator,
+ * exactly as another plm_normal_accumulate call would. Creation runs one
+ * eager pass into scratch sums (the accumulator is unchanged). The graph
+ * holds its own work buffers; the accumulator and batch must outlive it. */
+int plm_normal_graph_create(plm_normal* acc, plm_batch* b, const double* row_w, const double* y_w,
+                            int precision, plm_normal_graph** out);
+int plm_normal_graph_run(plm_normal_graph* g, int n_steps);
+int plm_normal_graph_destroy(plm_normal_graph* g);
+
 /* Row permutation: for each row of the batch in the reference layout, the
  * batch-internal row index (batch rows are [e, 6 s, 3N f] per structure). */
 int plm_batch_row_order(const plm_batch* b, long* order);

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libpolymlp_amd.so")
 
 PLM_FP64 = 0
 PLM_MIXED = 1
+PLM_MIXED64 = 2  # FP64 features with the quantised int8 Gram
 
 EXPORTS = [
     "plm_last_error", "plm_device_count", "plm_model_create", "plm_model_destroy",
@@ -26,7 +27,8 @@ EXPORTS = [
     "plm_cell_translations", "plm_neighbor", "plm_batch_create", "plm_batch_destroy",
     "plm_batch_n_rows", "plm_batch_n_pairs", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
-    "plm_normal_ridge", "plm_normal_sq_error",
+    "plm_normal_ridge", "plm_normal_sq_error", "plm_normal_graph_create", "plm_normal_graph_run",
+    "plm_normal_graph_destroy",
     "plm_solver_cg", "plm_model_feature_attrs",
     "plm_eval", "plm_eval_noalloc", "plm_eval_batch", "plm_eval_graph_create", "plm_eval_graph_run",
     "plm_eval_graph_destroy", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
@@ -91,6 +93,9 @@ def lib():
         _bp = ctypes.POINTER(ctypes.c_ubyte)
         L.plm_normal_ridge.argtypes = [vp, _dp, _bp, _dp, ctypes.c_int, _dp, _ip, _dp]
         L.plm_normal_sq_error.argtypes = [vp, _dp, _bp, _dp, ctypes.c_int, _dp]
+        L.plm_normal_graph_create.argtypes = [vp, vp, _dp, _dp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.plm_normal_graph_run.argtypes = [vp, ctypes.c_int]
+        L.plm_normal_graph_destroy.argtypes = [vp]
         L.plm_eval.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.plm_eval_noalloc.argtypes = [vp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                        _dp]
@@ -336,6 +341,31 @@ class Normal:
                                          z.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _pd(C),
                                          C.shape[0], _pd(out)))
         return out
+
+
+class NormalGraph:
+    """One accumulation pass of a batch captured as a HIP graph
+    (plm_normal_graph_*): each run(n) adds n passes to the accumulator."""
+
+    def __init__(self, acc: Normal, batch: Batch, w_ref, y_ref, precision: int = PLM_FP64):
+        order = batch.row_order()
+        w = np.zeros(batch.n_rows)
+        y = np.zeros(batch.n_rows)
+        w[order] = w_ref
+        y[order] = y_ref
+        self._keep = (acc, batch)
+        h = ctypes.c_void_p()
+        _check(lib().plm_normal_graph_create(acc._h, batch.handle, _pd(w), _pd(y), precision,
+                                             ctypes.byref(h)))
+        self._h = h
+
+    def run(self, n_steps: int = 1):
+        _check(lib().plm_normal_graph_run(self._h, int(n_steps)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.plm_normal_graph_destroy(self._h)
+            self._h = None
 
 
 def cell_translations(axis, positions_c, cutoff: float):

@@ -123,7 +123,7 @@ int plm_get_fn(int n, const double* r, const double* params, int n_params, doubl
         upload(dp, params, (size_t)n_params * 16, st);
         df.alloc((size_t)n * n_params * 8);
         dd.alloc((size_t)n * n_params * 8);
-        launch_get_fn(st, precision == PLM_FP64, n, (const double*)dr.ptr, (const double*)dp.ptr,
+        launch_get_fn(st, precision != PLM_MIXED, n, (const double*)dr.ptr, (const double*)dp.ptr,
                       n_params, cutoff, (double*)df.ptr, (double*)dd.ptr);
         check_hip(hipGetLastError(), "get_fn launch");
         download(fn, df, (size_t)n * n_params * 8, st);
@@ -142,7 +142,7 @@ int plm_get_ylm(int n, const double* xyz, int lmax, int precision, double* ylm, 
         upload(dx, xyz, (size_t)n * 24, st);
         dy.alloc((size_t)n * nh * 16);
         dd.alloc((size_t)n * nh * 48);
-        launch_get_ylm(st, precision == PLM_FP64, n, (const double*)dx.ptr, lmax,
+        launch_get_ylm(st, precision != PLM_MIXED, n, (const double*)dx.ptr, lmax,
                        (double*)dy.ptr, (double*)dd.ptr);
         check_hip(hipGetLastError(), "get_ylm launch");
         download(ylm, dy, (size_t)n * nh * 16, st);
@@ -156,7 +156,7 @@ int plm_api_features(plm_model* mp, int type, const double* anlm_re, const doubl
     PLM_API_TRY({
         if (!mp) throw std::invalid_argument("null model");
         auto& m = *reinterpret_cast<ModelHandle*>(mp);
-        const bool fp64 = precision == PLM_FP64;
+        const bool fp64 = precision != PLM_MIXED;
         hipStream_t st = device().stream;
         OneAtom one(m, type, anlm_re, anlm_im, fp64, st);
         const int elem = fp64 ? 8 : 4;
@@ -186,7 +186,7 @@ int plm_api_sum_of_prod(plm_model* mp, int type, const double* anlm_re, const do
         if (!mp) throw std::invalid_argument("null model");
         auto& m = *reinterpret_cast<ModelHandle*>(mp);
         if (!m.pot_uploaded) throw std::invalid_argument("coefficients not set");
-        const bool fp64 = precision == PLM_FP64;
+        const bool fp64 = precision != PLM_MIXED;
         hipStream_t st = device().stream;
         OneAtom one(m, type, anlm_re, anlm_im, fp64, st);
         const int elem = fp64 ? 8 : 4;

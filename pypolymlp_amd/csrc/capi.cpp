@@ -661,7 +661,8 @@ static void plan_group(ModelHandle& m, int rows_max, bool fp64, int& TB, int& A)
 }
 
 void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
-                  const double* roww, double* xe_sum, double* xe_sq, XWork& work, bool x_is_zero) {
+                  const double* roww, double* xe_sum, double* xe_sq, XWork& work, bool x_is_zero,
+                  const long* poly_doff) {
     Device& dev = device();
     hipStream_t st = dev.stream;
     const int a0 = b.atom_off[s0], a1 = b.atom_off[s1];
@@ -748,7 +749,7 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     check_hip(hipGetLastError(), "xrows launch");
     if (use_poly)
         build_poly_rows(m, b, s0, s1, fp64, X, ld, roww, xe_sum, xe_sq, work, pb, R, w_stride,
-                        x_stride);
+                        x_stride, poly_doff);
 }
 
 size_t atom_work_bytes(ModelHandle& m, bool fp64) {
@@ -784,6 +785,14 @@ using namespace plm;
         g_last_error = e.what();                             \
         return PLM_ERR_MODEL;                                \
     }
+
+// precision -> (FP64 X build, quantised int8 Gram)
+static bool fp64_features(int precision) {
+    if (precision != PLM_FP64 && precision != PLM_MIXED && precision != PLM_MIXED64)
+        throw std::invalid_argument("unknown precision");
+    return precision != PLM_MIXED;
+}
+static bool quant_gram(int precision) { return precision != PLM_FP64; }
 
 extern "C" {
 
@@ -928,7 +937,7 @@ int plm_features_x(plm_model* mp, plm_batch* bp, int precision, double* x_out) {
         auto* m = reinterpret_cast<ModelHandle*>(mp);
         auto* b = reinterpret_cast<BatchHandle*>(bp);
         if (!m->built) m->build_device();
-        const bool fp64 = precision == PLM_FP64;
+        const bool fp64 = fp64_features(precision);
         const int F = m->host.n_features();
         const long R = b->row_off.back();
         const int elem = fp64 ? 8 : 4;
@@ -993,7 +1002,7 @@ int plm_normal_accumulate(plm_normal* ap, plm_batch* bp, const double* row_w, co
     PLM_TRY({
         auto* acc = reinterpret_cast<NormalHandle*>(ap);
         auto* b = reinterpret_cast<BatchHandle*>(bp);
-        normal_accumulate(*acc, *b, row_w, y_w, precision == PLM_FP64);
+        normal_accumulate(*acc, *b, row_w, y_w, fp64_features(precision), quant_gram(precision));
     });
 }
 
@@ -1044,7 +1053,7 @@ int plm_eval(plm_model* mp, const double* axis, const double* pos_c, const int* 
              int precision, double* energy, double* forces, double* stress) {
     PLM_TRY({
         auto* m = reinterpret_cast<ModelHandle*>(mp);
-        eval_single(*m, axis, pos_c, types, n, precision == PLM_FP64, energy, forces, stress);
+        eval_single(*m, axis, pos_c, types, n, fp64_features(precision), energy, forces, stress);
     });
 }
 
@@ -1053,7 +1062,7 @@ int plm_eval_noalloc(plm_model* mp, const double* axis, const double* pos_c, con
     PLM_TRY({
         if (!mp || !axis || !pos_c || !types) throw std::invalid_argument("null argument");
         auto* m = reinterpret_cast<ModelHandle*>(mp);
-        eval_single_noalloc(*m, axis, pos_c, types, n, precision == PLM_FP64, energy, forces,
+        eval_single_noalloc(*m, axis, pos_c, types, n, fp64_features(precision), energy, forces,
                             stress);
     });
 }
@@ -1063,7 +1072,7 @@ int plm_eval_batch(plm_model* mp, plm_batch* bp, int precision, double* energies
     PLM_TRY({
         auto* m = reinterpret_cast<ModelHandle*>(mp);
         auto* b = reinterpret_cast<BatchHandle*>(bp);
-        eval_batch(*m, *b, precision == PLM_FP64, energies, forces, stresses);
+        eval_batch(*m, *b, fp64_features(precision), energies, forces, stresses);
     });
 }
 
@@ -1073,7 +1082,7 @@ int plm_eval_graph_create(plm_model* mp, plm_batch* bp, int precision, plm_eval_
         auto g = std::make_unique<EvalGraph>();
         g->m = reinterpret_cast<ModelHandle*>(mp);
         g->b = reinterpret_cast<BatchHandle*>(bp);
-        g->fp64 = precision == PLM_FP64;
+        g->fp64 = fp64_features(precision);
         eval_graph_capture(*g);
         *out = reinterpret_cast<plm_eval_graph*>(g.release());
     });
@@ -1094,6 +1103,29 @@ int plm_eval_graph_destroy(plm_eval_graph* gp) {
 int plm_ridge(int F, const double* xtx, const double* xty, const double* alphas, int n_alpha,
               double* coeffs, int* info) {
     PLM_TRY({ ridge_solve(F, xtx, xty, alphas, n_alpha, coeffs, info); });
+}
+
+int plm_normal_graph_create(plm_normal* ap, plm_batch* bp, const double* row_w, const double* y_w,
+                            int precision, plm_normal_graph** out) {
+    PLM_TRY({
+        if (!ap || !bp || !row_w || !y_w || !out) throw std::invalid_argument("null argument");
+        auto g = std::make_unique<NormalGraph>();
+        g->acc = reinterpret_cast<NormalHandle*>(ap);
+        g->b = reinterpret_cast<BatchHandle*>(bp);
+        normal_graph_capture(*g, row_w, y_w, fp64_features(precision), quant_gram(precision));
+        *out = reinterpret_cast<plm_normal_graph*>(g.release());
+    });
+}
+
+int plm_normal_graph_run(plm_normal_graph* gp, int n_steps) {
+    PLM_TRY({
+        if (!gp) throw std::invalid_argument("null graph");
+        normal_graph_run(*reinterpret_cast<NormalGraph*>(gp), n_steps);
+    });
+}
+
+int plm_normal_graph_destroy(plm_normal_graph* gp) {
+    PLM_TRY(delete reinterpret_cast<NormalGraph*>(gp));
 }
 
 int plm_normal_ridge(plm_normal* ap, const double* scales, const unsigned char* zero,

@@ -152,18 +152,29 @@ struct XWork {
 // X rows of structures [s0, s1) into X (row-major, leading dimension ld),
 // weighted by roww when given.
 // x_is_zero: X's rows of the range are already zero.
+// poly_doff: device D offsets of the sub-batch (poly_d_offsets) or null
 void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
                   const double* roww, double* xe_sum, double* xe_sq, XWork& work,
-                  bool x_is_zero = false);
+                  bool x_is_zero = false, const long* poly_doff = nullptr);
 // device bytes per atom of the X build's per-atom work buffers
 size_t atom_work_bytes(ModelHandle& m, bool fp64);
 // polynomial columns of structures [s0, s1) (after the linear columns of the
 // same sub-batch have been built, which leaves the W', x and pair buffers)
 void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
                      const double* roww, double* xe_sum, double* xe_sq, XWork& work,
-                     long pair_base, const PairRec& R, int w_stride, int x_stride);
+                     long pair_base, const PairRec& R, int w_stride, int x_stride,
+                     const long* doff_dev = nullptr);
+// offsets of the factor derivatives D of structures [s0, s1), from 0
+std::vector<long> poly_d_offsets(ModelHandle& m, const BatchHandle& b, int s0, int s1);
 size_t poly_bytes(ModelHandle& m, const BatchHandle& b, int s0, int s1, bool fp64);
 size_t pair_record_bytes(const ModelHandle& m, bool fp64);
+
+// Work buffers of an accumulation: X of a sub-batch, its quantised slices,
+// and the X build's per-atom and per-pair buffers (grown as needed).
+struct AccumWork {
+    DevBuf xbuf, qbuf;
+    XWork work;
+};
 
 struct NormalHandle {
     ModelHandle* model = nullptr;
@@ -171,12 +182,50 @@ struct NormalHandle {
     long n_rows = 0;
     DevBuf xtx;   // F_pad x F_pad Gram of [X | y], upper triangle accumulated
     DevBuf vec;   // [xe_sum | xe_sq_sum], each F_pad
-    DevBuf xbuf, wbuf, ybuf, qbuf, rbuf;
-    XWork work;
+    AccumWork ws;
 };
 
+// fp64: the X build in FP64 (else FP32); quant: the exact three-slice int8
+// Gram (else the FP64 Gram, which needs fp64)
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
-                       bool fp64);
+                       bool fp64, bool quant);
+
+// The accumulation of one batch as a host-side plan (sub-batches, row maps,
+// D offsets, weights and targets uploaded once) and a launch-only sequence,
+// so that it can run eagerly or be captured as a HIP graph.
+struct SubBatch {
+    int s0, s1;
+    long rows, rows_pad, nes_pad, h0, rows_q;  // rows_q / h0: quantised rows, first chunk
+    long rmap_off, doff_off;                    // into AccumPlan::rmaps / doffs
+};
+struct AccumPlan {
+    bool fp64 = true, quant = false;
+    long n_rows = 0;
+    std::vector<SubBatch> sb;
+    DevBuf rmaps, doffs, w, y;
+};
+void plan_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
+                     bool fp64, bool quant, AccumPlan& plan);
+// launches only (no allocation beyond growing ws, no copy, no synchronisation)
+void enqueue_accumulate(NormalHandle& acc, BatchHandle& b, const AccumPlan& plan, AccumWork& ws,
+                        double* gram, double* vec, hipStream_t st);
+
+// The fit step (one accumulation pass over a batch) captured as a HIP graph.
+struct NormalGraph {
+    NormalHandle* acc = nullptr;
+    BatchHandle* b = nullptr;
+    AccumPlan plan;
+    AccumWork ws;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    NormalGraph() = default;
+    NormalGraph(const NormalGraph&) = delete;
+    NormalGraph& operator=(const NormalGraph&) = delete;
+    ~NormalGraph();
+};
+void normal_graph_capture(NormalGraph& g, const double* row_w, const double* y_w, bool fp64,
+                          bool quant);
+void normal_graph_run(NormalGraph& g, int n_steps);
 
 void upload_potential(ModelHandle& m);
 

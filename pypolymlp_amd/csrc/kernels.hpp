@@ -143,8 +143,8 @@ void launch_gram_f64(hipStream_t st, const double* X, int ld, int rows_pad, int 
 // gram_chunk_rows() rows each.
 int gram_chunk_rows();
 size_t gram_quant_bytes(int rows_pad, int h0, int ld);
-void launch_gram_quantize(hipStream_t st, const float* X, const int* rmap, int ld, int rows_pad,
-                          int h0, int F_pad, void* quant);
+void launch_gram_quantize(hipStream_t st, bool x_fp64, const void* X, const int* rmap, int ld,
+                          int rows_pad, int h0, int F_pad, void* quant);
 void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int h0, int F_pad, const void* quant,
                     double* C, int ldc);
 

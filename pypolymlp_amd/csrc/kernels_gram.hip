@@ -107,49 +107,60 @@ __host__ __device__ __forceinline__ int chunk_of(int q, int h0) {
 __host__ __device__ __forceinline__ int n_chunks_of(int rows, int h0) {
     return rows <= h0 ? 1 : 1 + (rows - h0 + kQChunk - 1) / kQChunk;
 }
-__device__ __forceinline__ float xrow(const float* __restrict__ X, const int* __restrict__ rmap,
-                                      int q, int ld, int c) {
+template <class T>
+__device__ __forceinline__ T xrow(const T* __restrict__ X, const int* __restrict__ rmap, int q,
+                                  int ld, int c) {
     const int r = rmap[q];
-    return r >= 0 ? X[(size_t)r * ld + c] : 0.f;
+    return r >= 0 ? X[(size_t)r * ld + c] : T(0);
+}
+__device__ __forceinline__ int exponent_of(float m) {
+    int e = 0;
+    if (m > 0.f) (void)frexpf(m, &e);
+    return e;
+}
+__device__ __forceinline__ int exponent_of(double m) {
+    int e = 0;
+    if (m > 0.0) (void)frexp(m, &e);
+    return e;
 }
 
 // E[chunk][c]: smallest E with |x| < 2^E over the chunk's rows (0 for a zero column).
 // 64 columns per workgroup, four row phases of 64 threads; each thread keeps
 // four loads in flight (the maximum is order independent, so the result does
 // not depend on the split)
-__global__ void __launch_bounds__(256) colexp_kernel(const float* __restrict__ X,
+template <class T>
+__global__ void __launch_bounds__(256) colexp_kernel(const T* __restrict__ X,
                                                      const int* __restrict__ rmap, int ld, int rows,
                                                      int h0, int ncols, int* __restrict__ E) {
-    __shared__ float part[4][64];
+    __shared__ T part[4][64];
     const int cx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + cx;
     const int chunk = blockIdx.y;
     const int r0 = chunk == 0 ? 0 : h0 + (chunk - 1) * kQChunk;
     const int r1 = min(rows, chunk == 0 ? h0 : r0 + kQChunk);
-    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+    T m0 = T(0), m1 = T(0), m2 = T(0), m3 = T(0);
     if (c < ncols) {
         int r = r0 + ty;
         for (; r + 12 < r1; r += 16) {
-            m0 = fmaxf(m0, fabsf(xrow(X, rmap, r, ld, c)));
-            m1 = fmaxf(m1, fabsf(xrow(X, rmap, r + 4, ld, c)));
-            m2 = fmaxf(m2, fabsf(xrow(X, rmap, r + 8, ld, c)));
-            m3 = fmaxf(m3, fabsf(xrow(X, rmap, r + 12, ld, c)));
+            m0 = fmax(m0, fabs(xrow(X, rmap, r, ld, c)));
+            m1 = fmax(m1, fabs(xrow(X, rmap, r + 4, ld, c)));
+            m2 = fmax(m2, fabs(xrow(X, rmap, r + 8, ld, c)));
+            m3 = fmax(m3, fabs(xrow(X, rmap, r + 12, ld, c)));
         }
-        for (; r < r1; r += 4) m0 = fmaxf(m0, fabsf(xrow(X, rmap, r, ld, c)));
+        for (; r < r1; r += 4) m0 = fmax(m0, fabs(xrow(X, rmap, r, ld, c)));
     }
-    part[ty][cx] = fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
+    part[ty][cx] = fmax(fmax(m0, m1), fmax(m2, m3));
     __syncthreads();
     if (ty == 0 && c < ncols) {
-        const float m = fmaxf(fmaxf(part[0][cx], part[1][cx]), fmaxf(part[2][cx], part[3][cx]));
-        int e = 0;
-        if (m > 0.f) (void)frexpf(m, &e);
-        E[(size_t)chunk * ld + c] = e;
+        const T m = fmax(fmax(part[0][cx], part[1][cx]), fmax(part[2][cx], part[3][cx]));
+        E[(size_t)chunk * ld + c] = exponent_of(m);
     }
 }
 
 // Q[s][rb][c][k] (int8): slice s of x[rmap[32 rb + k]][c], k-contiguous per
 // column so that 16 bytes are one lane's MFMA operand.
-__global__ void quantize_kernel(const float* __restrict__ X, const int* __restrict__ rmap, int ld,
+template <class T>
+__global__ void quantize_kernel(const T* __restrict__ X, const int* __restrict__ rmap, int ld,
                                 int rows_pad, int h0, int ncols, const int* __restrict__ E,
                                 signed char* __restrict__ Q) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,9 +174,9 @@ __global__ void quantize_kernel(const float* __restrict__ X, const int* __restri
     int rr[kQK];
 #pragma unroll
     for (int k = 0; k < kQK; ++k) rr[k] = rmap[rb * kQK + k];
-    float xv[kQK];
+    T xv[kQK];
 #pragma unroll
-    for (int k = 0; k < kQK; ++k) xv[k] = rr[k] >= 0 ? X[(size_t)rr[k] * ld + c] : 0.f;
+    for (int k = 0; k < kQK; ++k) xv[k] = rr[k] >= 0 ? X[(size_t)rr[k] * ld + c] : T(0);
 #pragma unroll
     for (int k0 = 0; k0 < kQK; k0 += 16) {
         union {
@@ -174,13 +185,13 @@ __global__ void quantize_kernel(const float* __restrict__ X, const int* __restri
         } d0, d1, d2;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const float x = xv[k0 + k];
-            const float m = ldexpf(x, 7 - e);  // |m| < 128
-            const float t0 = truncf(m);
-            const float r1 = (m - t0) * 128.f;  // exact
-            const float t1 = truncf(r1);
-            const float r2 = (r1 - t1) * 128.f;  // exact
-            const float t2 = fminf(fmaxf(rintf(r2), -127.f), 127.f);
+            const T x = xv[k0 + k];
+            const T m = ldexp(x, 7 - e);  // |m| < 128
+            const T t0 = trunc(m);
+            const T r1 = (m - t0) * T(128);  // exact
+            const T t1 = trunc(r1);
+            const T r2 = (r1 - t1) * T(128);  // exact
+            const T t2 = fmin(fmax(rint(r2), T(-127)), T(127));
             d0.b[k] = (signed char)t0, d1.b[k] = (signed char)t1, d2.b[k] = (signed char)t2;
         }
         *reinterpret_cast<v4i*>(q + k0) = d0.v;
@@ -409,18 +420,24 @@ void launch_gram_f64(hipStream_t st, const double* X, int ld, int rows_pad, int 
 
 int gram_chunk_rows() { return kQChunk; }
 
-void launch_gram_quantize(hipStream_t st, const float* X, const int* rmap, int ld, int rows_pad,
-                          int h0, int F_pad, void* quant) {
+void launch_gram_quantize(hipStream_t st, bool x_fp64, const void* X, const int* rmap, int ld,
+                          int rows_pad, int h0, int F_pad, void* quant) {
     if (h0 < kQK * GI_RB || h0 > kQChunk || h0 % (kQK * GI_RB) || rows_pad % (kQK * GI_RB))
         throw std::invalid_argument("quantisation chunk rows");
     const int n_chunks = n_chunks_of(rows_pad, h0);
     signed char* Q = reinterpret_cast<signed char*>(quant);
     int* E = reinterpret_cast<int*>(Q + 3 * (size_t)rows_pad * ld);
     const int B = 256;
-    colexp_kernel<<<dim3((F_pad + 63) / 64, n_chunks), 256, 0, st>>>(X, rmap, ld, rows_pad, h0,
-                                                                        F_pad, E);
-    quantize_kernel<<<dim3((F_pad + B - 1) / B, rows_pad / kQK), B, 0, st>>>(
-        X, rmap, ld, rows_pad, h0, F_pad, E, Q);
+    const dim3 ge((F_pad + 63) / 64, n_chunks), gq((F_pad + B - 1) / B, rows_pad / kQK);
+    if (x_fp64) {
+        const double* Xd = static_cast<const double*>(X);
+        colexp_kernel<double><<<ge, 256, 0, st>>>(Xd, rmap, ld, rows_pad, h0, F_pad, E);
+        quantize_kernel<double><<<gq, B, 0, st>>>(Xd, rmap, ld, rows_pad, h0, F_pad, E, Q);
+    } else {
+        const float* Xf = static_cast<const float*>(X);
+        colexp_kernel<float><<<ge, 256, 0, st>>>(Xf, rmap, ld, rows_pad, h0, F_pad, E);
+        quantize_kernel<float><<<gq, B, 0, st>>>(Xf, rmap, ld, rows_pad, h0, F_pad, E, Q);
+    }
 }
 
 void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int h0, int F_pad, const void* quant,

@@ -16,16 +16,28 @@ constexpr int SB = 64;  // block size
 
 typedef __attribute__((__vector_size__(4 * sizeof(double)))) double doublex4;
 
-// Unblocked Cholesky of the diagonal block at (k0, k0) in LDS, one wave:
-// lane i owns row i; column step j scales column j below the diagonal, then
-// each lane updates its own row (a[i][c] -= l_ij l_cj, c in (j, i]).
+// Unblocked Cholesky of the diagonal block at (k0, k0) by one wave, in LDS
+// (coalesced load and store). Column j, left-looking: lane i >= j forms
+// a_ij - sum_{k<j} l_ik l_jk (k ascending: the same operations in the same
+// order as the right-looking rank-1 updates), then divides by the pivot.
 __global__ void __launch_bounds__(SB) potf2_kernel(double* __restrict__ A, int ld, int k0,
                                                    int* __restrict__ info) {
     __shared__ double a[SB][SB + 1];
     const int i = threadIdx.x;
-    for (int c = 0; c < SB; ++c) a[i][c] = (c <= i) ? A[(size_t)(k0 + i) * ld + k0 + c] : 0.0;
+    for (int e = i; e < SB * SB; e += SB) {
+        const int r = e / SB, c = e % SB;
+        a[r][c] = (c <= r) ? A[(size_t)(k0 + r) * ld + k0 + c] : 0.0;
+    }
     __syncthreads();
     for (int j = 0; j < SB; ++j) {
+        double v = 0.0;
+        if (i >= j) {
+            v = a[i][j];
+#pragma unroll 8
+            for (int k = 0; k < j; ++k) v -= a[i][k] * a[j][k];
+            if (i == j) a[j][j] = v;
+        }
+        __syncthreads();
         double d = a[j][j];
         if (!(d > 0.0)) {
             if (i == j && *info == 0) *info = k0 + j + 1;
@@ -34,15 +46,13 @@ __global__ void __launch_bounds__(SB) potf2_kernel(double* __restrict__ A, int l
         const double djj = sqrt(d);
         __syncthreads();
         if (i == j) a[j][j] = djj;
-        if (i > j) a[i][j] /= djj;
-        __syncthreads();
-        if (i > j) {
-            const double lij = a[i][j];
-            for (int c = j + 1; c <= i; ++c) a[i][c] -= lij * a[c][j];
-        }
+        if (i > j) a[i][j] = v / djj;
         __syncthreads();
     }
-    for (int c = 0; c <= i; ++c) A[(size_t)(k0 + i) * ld + k0 + c] = a[i][c];
+    for (int e = i; e < SB * SB; e += SB) {
+        const int r = e / SB, c = e % SB;
+        if (c <= r) A[(size_t)(k0 + r) * ld + k0 + c] = a[r][c];
+    }
 }
 
 // Panel: rows i in [k0+SB, n): A[i, k0:k0+SB] <- A[i, k0:k0+SB] L_kk^{-T} by
@@ -286,8 +296,9 @@ void launch_symv(hipStream_t st, const double* A, int lda, int n, const double* 
 // Blocked right-looking Cholesky with NB-wide outer panels: inside a panel,
 // 64-wide steps (diagonal block, panel TRSM below it, update of the rest of
 // the panel's columns); after the panel, one trailing update with K = NB, so
-// that the dominant product runs with a 4x wider inner dimension.
-constexpr int NB = 256;
+// that the dominant product runs with an 8x wider inner dimension and the
+// trailing matrix is read and written once per 512 columns.
+constexpr int NB = 512;
 
 void launch_cholesky(hipStream_t st, double* A, int ld, int n_pad, int* info) {
     for (int k0 = 0; k0 < n_pad; k0 += NB) {
@@ -337,6 +348,7 @@ void launch_axpy(hipStream_t st, int n, const double* d, double* x) {
     axpy_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, d, x);
 }
 
+// padding granularity of the solve's matrices: whole 64-blocks
 int solve_block() { return SB; }
 
 }  // namespace plm

@@ -1,13 +1,18 @@
 // Fused X -> X^T X accumulation over a device-resident batch.
 //
 // Structures are processed in sub-batches whose X rows (bounded to a few GiB)
-// are built by the column-block kernel straight into a row-major buffer with
-// the row weights applied. The weighted targets are placed in column F of the
+// are built by the column kernels straight into a row-major buffer with the
+// row weights applied. The weighted targets are placed in column F of the
 // same buffer, and the Gram kernel folds the whole [X | y] block into the
 // FP64 accumulator, so X^T y and y^T y come from exactly the X that X^T X
 // sees. X for the whole dataset is never materialised. Mirrors the per-batch
 // work of _compute_products_single_batch
 // (src/pypolymlp/mlp_dev/core/data_sequential.py:97-155).
+//
+// The work is split into a host-side plan (sub-batch bounds, quantisation row
+// maps, factor-derivative offsets, the row weights and targets, all uploaded
+// once) and a launch-only sequence over the plan, so that one accumulation
+// pass can run eagerly or be captured once as a HIP graph and replayed.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -22,24 +27,17 @@ static constexpr size_t kXBudget = 3ull << 30;   // bytes of X per sub-batch (PL
 static constexpr size_t kABudget = 8ull << 30;   // bytes of per-atom work (a, W', x) per sub-batch
 static constexpr size_t kPBudget = 6ull << 30;   // bytes of pair records per sub-batch
 static constexpr size_t kDBudget = 4ull << 30;   // bytes of factor derivatives per sub-batch
+static constexpr long kStep = 64;                // rows_pad multiple of the Gram's 64-row stage
 
-// Sub-batches run in order on the library stream: X rows (weights applied,
-// targets in column F), then the Gram of [X | y]. (Running the Gram of one
-// sub-batch on a second stream while the next X is built was measured slower
-// for the mixed form: the two contend for the CUs' registers and LDS.)
-void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
-                       bool fp64) {
+void plan_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
+                     bool fp64, bool quant, AccumPlan& plan) {
+    if (!fp64 && !quant) throw std::invalid_argument("the FP64 Gram needs the FP64 X build");
     ModelHandle& m = *acc.model;
-    Device& dev = device();
-    hipStream_t st = dev.stream;
+    hipStream_t st = device().stream;
     const int elem = fp64 ? 8 : 4;
-    const int F = acc.F, ld = acc.F_pad;
-    const int kstep = 64;  // rows_pad multiple of the Gram's 64-row stage
-    const size_t row_bytes = (size_t)ld * elem;
+    const size_t row_bytes = (size_t)acc.F_pad * elem;
     const size_t rec_bytes = pair_record_bytes(m, fp64);
     const size_t atom_bytes = atom_work_bytes(m, fp64);
-    double* xe_sum = reinterpret_cast<double*>(acc.vec.ptr);
-    double* xe_sq = xe_sum + acc.F_pad;
     // PLM_X_BUDGET_MB lowers the X budget of a sub-batch (less device memory,
     // more sub-batches)
     size_t x_budget = kXBudget;
@@ -47,7 +45,12 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
         const long mb = std::atol(e);
         if (mb > 0) x_budget = std::min(kXBudget, (size_t)mb << 20);
     }
-    std::vector<std::vector<int>> rmaps;  // row maps, alive until the stream is synchronised
+    plan.fp64 = fp64;
+    plan.quant = quant;
+    plan.sb.clear();
+    std::vector<int> rmaps;
+    std::vector<long> doffs;
+    const bool use_poly = m.poly().enabled;
     int s0 = 0;
     while (s0 < b.n_st) {
         int s1 = s0 + 1;
@@ -55,66 +58,151 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
             const size_t rows = (size_t)(b.row_off[s_end] - b.row_off[s0]);
             const size_t atoms = (size_t)(b.atom_off[s_end] - b.atom_off[s0]);
             const size_t pairs = (size_t)(b.nb_off[b.atom_off[s_end]] - b.nb_off[b.atom_off[s0]]);
-            return (rows + kstep) * row_bytes <= x_budget &&
+            return (rows + kStep) * row_bytes <= x_budget &&
                    atoms * atom_bytes <= kABudget && pairs * rec_bytes <= kPBudget &&
                    poly_bytes(m, b, s0, s_end, fp64) <= kDBudget;
         };
         while (s1 < b.n_st && fits(s1 + 1)) ++s1;
-        const long r0 = b.row_off[s0], r1 = b.row_off[s1];
-        const long rows = r1 - r0;
-        const long rows_pad = (rows + kstep - 1) / kstep * kstep;
-        acc.xbuf.alloc((size_t)rows_pad * row_bytes);
-        acc.wbuf.alloc((size_t)rows * 8);
-        acc.ybuf.alloc((size_t)rows * 8);
-        check_hip(hipMemsetAsync(acc.xbuf.ptr, 0, (size_t)rows_pad * row_bytes, st), "memset");
-        check_hip(hipMemcpyAsync(acc.wbuf.ptr, row_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
-        check_hip(hipMemcpyAsync(acc.ybuf.ptr, y_w + r0, (size_t)rows * 8, hipMemcpyHostToDevice, st), "copy");
-        build_x_rows(m, b, s0, s1, fp64, acc.xbuf.ptr, ld, (const double*)acc.wbuf.ptr, xe_sum, xe_sq,
-                     acc.work, true);
-        launch_set_column(st, fp64, acc.xbuf.ptr, ld, F, (const double*)acc.ybuf.ptr, (int)rows);
-        check_hip(hipGetLastError(), "target column");
-        if (fp64) {
-            ProfScope ps("gram", st);
-            launch_gram_f64(st, reinterpret_cast<const double*>(acc.xbuf.ptr), ld, (int)rows_pad,
-                            acc.F_pad, (double*)acc.xtx.ptr, acc.F_pad);
-        } else {
+        SubBatch sb{};
+        sb.s0 = s0, sb.s1 = s1;
+        sb.rows = b.row_off[s1] - b.row_off[s0];
+        sb.rows_pad = (sb.rows + kStep - 1) / kStep * kStep;
+        sb.doff_off = (long)doffs.size();
+        if (use_poly) {
+            const std::vector<long> d = poly_d_offsets(m, b, s0, s1);
+            doffs.insert(doffs.end(), d.begin(), d.end());
+        }
+        sb.rmap_off = (long)rmaps.size();
+        if (quant) {
             // quantisation order: the energy and stress rows (7 per structure),
             // then the force rows; the first exponent chunk is shortened so that
             // the energy and stress rows end on a chunk boundary. An energy row
             // sums a feature over the structure's atoms, so sharing a chunk with
             // it would cost the force rows about log2(N) bits.
             const long chunk = gram_chunk_rows();
-            const long nes = 7L * (s1 - s0), nes_pad = (nes + kstep - 1) / kstep * kstep;
-            const long h0 = nes_pad % chunk == 0 ? chunk : nes_pad % chunk;
-            const long rows_q = nes_pad + (rows - nes + kstep - 1) / kstep * kstep;
-            rmaps.emplace_back(rows_q, -1);
-            std::vector<int>& rm = rmaps.back();
-            long qe = 0, qf = nes_pad;
+            const long r0 = b.row_off[s0];
+            const long nes = 7L * (s1 - s0);
+            sb.nes_pad = (nes + kStep - 1) / kStep * kStep;
+            sb.h0 = sb.nes_pad % chunk == 0 ? chunk : sb.nes_pad % chunk;
+            sb.rows_q = sb.nes_pad + (sb.rows - nes + kStep - 1) / kStep * kStep;
+            const size_t base_q = rmaps.size();
+            rmaps.resize(base_q + sb.rows_q, -1);
+            long qe = 0, qf = sb.nes_pad;
             for (int s = s0; s < s1; ++s) {
                 const long base = b.row_off[s] - r0, n = b.row_off[s + 1] - b.row_off[s];
-                for (long k = 0; k < 7; ++k) rm[qe++] = (int)(base + k);
-                for (long k = 7; k < n; ++k) rm[qf++] = (int)(base + k);
+                for (long k = 0; k < 7; ++k) rmaps[base_q + qe++] = (int)(base + k);
+                for (long k = 7; k < n; ++k) rmaps[base_q + qf++] = (int)(base + k);
             }
-            acc.rbuf.alloc((size_t)rows_q * sizeof(int));
-            check_hip(hipMemcpyAsync(acc.rbuf.ptr, rm.data(), (size_t)rows_q * sizeof(int),
-                                     hipMemcpyHostToDevice, st), "copy");
-            acc.qbuf.alloc(gram_quant_bytes((int)rows_q, (int)h0, ld));
+        }
+        plan.sb.push_back(sb);
+        s0 = s1;
+    }
+    plan.n_rows = b.row_off[b.n_st];
+    auto up = [&](DevBuf& d, const void* h, size_t bytes) {
+        d.alloc(std::max<size_t>(bytes, 16));
+        if (bytes) check_hip(hipMemcpyAsync(d.ptr, h, bytes, hipMemcpyHostToDevice, st), "copy");
+    };
+    up(plan.rmaps, rmaps.data(), rmaps.size() * sizeof(int));
+    up(plan.doffs, doffs.data(), doffs.size() * sizeof(long));
+    up(plan.w, row_w, (size_t)plan.n_rows * 8);
+    up(plan.y, y_w, (size_t)plan.n_rows * 8);
+    check_hip(hipStreamSynchronize(st), "sync");  // the host tables go out of scope
+}
+
+// Sub-batches run in order on one stream: X rows (weights applied, targets
+// in column F), then the Gram of [X | y]. (Running the Gram of one sub-batch
+// on a second stream while the next X is built was measured slower for the
+// mixed form: the two contend for the CUs' registers and LDS.)
+void enqueue_accumulate(NormalHandle& acc, BatchHandle& b, const AccumPlan& plan, AccumWork& ws,
+                        double* gram, double* vec, hipStream_t st) {
+    ModelHandle& m = *acc.model;
+    const bool fp64 = plan.fp64;
+    const int elem = fp64 ? 8 : 4;
+    const int F = acc.F, ld = acc.F_pad;
+    const size_t row_bytes = (size_t)ld * elem;
+    double* xe_sum = vec;
+    double* xe_sq = vec + acc.F_pad;
+    const double* w = (const double*)plan.w.ptr;
+    const double* y = (const double*)plan.y.ptr;
+    for (const SubBatch& sb : plan.sb) {
+        const long r0 = b.row_off[sb.s0];
+        ws.xbuf.alloc((size_t)sb.rows_pad * row_bytes);
+        // cleared by a kernel (a memset node would not replay correctly in a graph)
+        launch_fill_zero(st, (double*)ws.xbuf.ptr, (size_t)sb.rows_pad * row_bytes / 8);
+        build_x_rows(m, b, sb.s0, sb.s1, fp64, ws.xbuf.ptr, ld, w + r0, xe_sum, xe_sq, ws.work,
+                     true, (const long*)plan.doffs.ptr + sb.doff_off);
+        launch_set_column(st, fp64, ws.xbuf.ptr, ld, F, y + r0, (int)sb.rows);
+        check_hip(hipGetLastError(), "target column");
+        if (!plan.quant) {
+            ProfScope ps("gram", st);
+            launch_gram_f64(st, reinterpret_cast<const double*>(ws.xbuf.ptr), ld, (int)sb.rows_pad,
+                            acc.F_pad, gram, acc.F_pad);
+        } else {
+            ws.qbuf.alloc(gram_quant_bytes((int)sb.rows_q, (int)sb.h0, ld));
             {
                 ProfScope ps("quantize", st);
-                launch_gram_quantize(st, reinterpret_cast<const float*>(acc.xbuf.ptr),
-                                     (const int*)acc.rbuf.ptr, ld, (int)rows_q, (int)h0,
-                                     acc.F_pad, acc.qbuf.ptr);
+                launch_gram_quantize(st, fp64, ws.xbuf.ptr,
+                                     (const int*)plan.rmaps.ptr + sb.rmap_off, ld, (int)sb.rows_q,
+                                     (int)sb.h0, acc.F_pad, ws.qbuf.ptr);
             }
             check_hip(hipGetLastError(), "quantize launch");
             ProfScope ps("gram", st);
-            launch_gram_i8(st, ld, (int)rows_q, (int)h0, acc.F_pad, acc.qbuf.ptr,
-                           (double*)acc.xtx.ptr, acc.F_pad);
+            launch_gram_i8(st, ld, (int)sb.rows_q, (int)sb.h0, acc.F_pad, ws.qbuf.ptr, gram,
+                           acc.F_pad);
         }
         check_hip(hipGetLastError(), "gram launch");
-        acc.n_rows += rows;
-        s0 = s1;
     }
+}
+
+void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
+                       bool fp64, bool quant) {
+    hipStream_t st = device().stream;
+    AccumPlan plan;
+    plan_accumulate(acc, b, row_w, y_w, fp64, quant, plan);
+    enqueue_accumulate(acc, b, plan, acc.ws, (double*)acc.xtx.ptr, (double*)acc.vec.ptr, st);
     check_hip(hipStreamSynchronize(st), "sync");
+    acc.n_rows += plan.n_rows;
+}
+
+NormalGraph::~NormalGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+}
+
+// One eager pass into scratch sums first (every work buffer reaches its size,
+// code objects and model tables are loaded, library handles initialised), then
+// the same launches captured with the accumulator's own sums as targets.
+void normal_graph_capture(NormalGraph& g, const double* row_w, const double* y_w, bool fp64,
+                          bool quant) {
+    NormalHandle& acc = *g.acc;
+    hipStream_t st = device().stream;
+    plan_accumulate(acc, *g.b, row_w, y_w, fp64, quant, g.plan);
+    {
+        DevBuf gram, vec;
+        gram.alloc(acc.xtx.bytes), vec.alloc(acc.vec.bytes);
+        enqueue_accumulate(acc, *g.b, g.plan, g.ws, (double*)gram.ptr, (double*)vec.ptr, st);
+        check_hip(hipStreamSynchronize(st), "sync");
+    }
+    ProfPause pause;
+    check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+    try {
+        enqueue_accumulate(acc, *g.b, g.plan, g.ws, (double*)acc.xtx.ptr, (double*)acc.vec.ptr, st);
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(st, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    check_hip(hipStreamEndCapture(st, &g.graph), "end capture");
+    check_hip(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "graph instantiate");
+}
+
+void normal_graph_run(NormalGraph& g, int n_steps) {
+    if (n_steps < 1) throw std::invalid_argument("steps must be positive");
+    hipStream_t st = device().stream;
+    for (int k = 0; k < n_steps; ++k) check_hip(hipGraphLaunch(g.exec, st), "graph launch");
+    check_hip(hipStreamSynchronize(st), "sync");
+    g.acc->n_rows += (long)n_steps * g.plan.n_rows;
 }
 
 }  // namespace plm

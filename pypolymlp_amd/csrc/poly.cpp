@@ -125,9 +125,20 @@ size_t poly_bytes(ModelHandle& m, const BatchHandle& b, int s0, int s1, bool fp6
 // factors mean more passes over the pairs but more workgroups per CU
 constexpr size_t kDerivLds = 64 * 1024;  // 32 KB: 16 % slower (more passes)
 
+std::vector<long> poly_d_offsets(ModelHandle& m, const BatchHandle& b, int s0, int s1) {
+    const PolyDev& pd = m.poly().dev;
+    std::vector<long> doff(s1 - s0 + 1, 0);
+    for (int s = s0; s < s1; ++s) {
+        const long N = b.atom_off[s + 1] - b.atom_off[s];
+        doff[s - s0 + 1] = doff[s - s0] + N * pd.K * (6 + 3 * N);
+    }
+    return doff;
+}
+
 void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
                      const double* roww, double* xe_sum, double* xe_sq, XWork& work,
-                     long pair_base, const PairRec& R, int w_stride, int x_stride) {
+                     long pair_base, const PairRec& R, int w_stride, int x_stride,
+                     const long* doff_dev) {
     const PolyTables& P = m.poly();
     const PolyDev& pd = P.dev;
     Device& dev = device();
@@ -137,19 +148,20 @@ void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, 
     const BatchDev bd = b.view();
     work.xf.alloc((size_t)na * pd.K * elem);
     work.m.alloc((size_t)na * pd.U * elem);
-    // D offsets per structure of the sub-batch
-    std::vector<long> doff(s1 - s0 + 1, 0);
+    // D offsets per structure of the sub-batch (uploaded here unless the caller
+    // holds them on the device already, as a captured sequence must)
+    const std::vector<long> doff = poly_d_offsets(m, b, s0, s1);
     int nmax = 0;
-    for (int s = s0; s < s1; ++s) {
-        const long N = b.atom_off[s + 1] - b.atom_off[s];
-        nmax = std::max<int>(nmax, (int)N);
-        doff[s - s0 + 1] = doff[s - s0] + N * pd.K * (6 + 3 * N);
-    }
+    for (int s = s0; s < s1; ++s) nmax = std::max(nmax, b.atom_off[s + 1] - b.atom_off[s]);
     const int rd_max = 6 + 3 * nmax;
     work.d.alloc((size_t)doff.back() * elem);
-    work.doff.alloc(doff.size() * sizeof(long));
-    check_hip(hipMemcpyAsync(work.doff.ptr, doff.data(), doff.size() * sizeof(long),
-                             hipMemcpyHostToDevice, st), "copy");
+    if (!doff_dev) {
+        work.doff.alloc(doff.size() * sizeof(long));
+        // a pageable-memory copy returns once the source has been read
+        check_hip(hipMemcpyAsync(work.doff.ptr, doff.data(), doff.size() * sizeof(long),
+                                 hipMemcpyHostToDevice, st), "copy");
+        doff_dev = (const long*)work.doff.ptr;
+    }
     {
         ProfScope ps("poly_deriv", st);
         launch_poly_factor(st, fp64, m.md, pd, bd, a0, na, work.x.ptr, x_stride, work.xf.ptr,
@@ -161,7 +173,7 @@ void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, 
         for (int k0 = 0; k0 < pd.K; k0 += kc)
             launch_poly_deriv(st, fp64, m.md, pd, bd, a0, na, pair_base, work.pd.ptr, R,
                               work.w.ptr, w_stride, k0, std::min(kc, pd.K - k0), rd_max,
-                              work.d.ptr, s0, (const long*)work.doff.ptr);
+                              work.d.ptr, s0, doff_dev);
     }
     check_hip(hipGetLastError(), "poly launch");
     rocblas_handle h = blas_handle();
