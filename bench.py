@@ -23,8 +23,10 @@ structures, plus numpy X^T X, on the host cores of this machine.
 Further legs in the same JSON line: PolymlpEval on a 32,000-atom cell (eager
 steps with the neighbour lists rebuilt, and HIP-graph replays) with the
 pair-kernel roofline and the reference's CPU evaluation; the mixed-vs-FP64
-fit accuracy on a sample; and the cold end-to-end fit (calc_xtx_xty +
-solver_ridge) in both precisions.
+fit accuracy on a sample (the mixed fit with and without its refinement
+step); and the cold end-to-end fit in both precisions (device-resident
+accumulation, ridge and, for the mixed form, the refinement step; and the
+read-back path calc_xtx_xty + solver_ridge).
 """
 
 from __future__ import annotations
@@ -224,9 +226,11 @@ def eval_leg(steps: int, graph_steps: int, cpu: bool):
 
 def fit_accuracy_leg(n_st: int):
     """Mixed against FP64 fit on a seeded subset of the same binary-alloy
-    workload (with Morse targets): both normal-equation forms, the GPU ridge
-    solve at alpha = 1e-3 and 1e-1, and FP64 evaluation of the held-out 10 %.
-    Returns the largest energy (meV/atom) and force (meV/A) differences."""
+    workload (with Morse targets): the device-resident normal equations of
+    both forms, the GPU ridge solve at alpha = 1e-3 and 1e-1, for the mixed
+    form with and without the refinement step that mlp_dev.fit applies, and
+    FP64 evaluation of the held-out 10 %. Returns the largest energy
+    (meV/atom) and force (meV/A) differences."""
     from pypolymlp_amd import capi, datasets, mlp_dev
 
     ds = datasets.config2_dataset(n_st)
@@ -234,27 +238,39 @@ def fit_accuracy_leg(n_st: int):
     m = capi.Model(datasets.config2_model().as_dict())
     alphas = np.array([1e-3, 1e-1])
     fits = {}
-    for prec in (capi.PLM_FP64, capi.PLM_MIXED):
-        ne = mlp_dev.calc_xtx_xty(m, [train], precision=prec)
-        fits[prec] = mlp_dev.solver_ridge(ne.xtx, ne.xty, alphas) / ne.scales[:, None]
+    for name, prec in (("fp64", capi.PLM_FP64), ("mixed", capi.PLM_MIXED)):
+        dn = mlp_dev.accumulate_normal(m, [train], precision=prec, keep_parts=name == "mixed")
+        c, info, _ = dn.acc.ridge(dn.scales, dn.zero, alphas)
+        fits[name] = c / dn.scales[:, None]
+        if name == "mixed":
+            c2, _, _ = mlp_dev.refine(dn, c, alphas, prec)
+            fits["mixed_refined"] = c2 / dn.scales[:, None]
+        del dn
     sts = test.structures
     b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
                    [s.types for s in sts])
     n = np.array([s.n_total for s in sts])
-    de = df = 0.0
+    err = {"mixed": [0.0, 0.0], "mixed_refined": [0.0, 0.0]}
     for k in range(len(alphas)):
         pred = {}
-        for prec, c in fits.items():
+        for name, c in fits.items():
             m.set_coeffs(c[:, k])
             E, F, _ = capi.eval_batch(m, b, capi.PLM_FP64)
-            pred[prec] = (E / n, np.concatenate([f.reshape(-1) for f in F]))
-        de = max(de, float(np.abs(pred[capi.PLM_MIXED][0] - pred[capi.PLM_FP64][0]).max()))
-        df = max(df, float(np.abs(pred[capi.PLM_MIXED][1] - pred[capi.PLM_FP64][1]).max()))
-    log(f"fit accuracy: mixed vs FP64 max |dE| {de * 1e3:.2e} meV/atom, max |dF| {df * 1e3:.2e} meV/A")
+            pred[name] = (E / n, np.concatenate([f.reshape(-1) for f in F]))
+        for name in err:
+            err[name][0] = max(err[name][0], float(np.abs(pred[name][0] - pred["fp64"][0]).max()))
+            err[name][1] = max(err[name][1], float(np.abs(pred[name][1] - pred["fp64"][1]).max()))
+    de, df = err["mixed_refined"]
+    log(f"fit accuracy: mixed (refined) vs FP64 max |dE| {de * 1e3:.2e} meV/atom, max |dF| "
+        f"{df * 1e3:.2e} meV/A; without the refinement step {err['mixed'][0] * 1e3:.2e} / "
+        f"{err['mixed'][1] * 1e3:.2e}")
     return {"fit_mixed_vs_fp64": {
         "energy_max_meV_per_atom": de * 1e3, "force_max_meV_per_A": df * 1e3,
+        "unrefined": {"energy_max_meV_per_atom": err["mixed"][0] * 1e3,
+                      "force_max_meV_per_A": err["mixed"][1] * 1e3},
         "sample": f"{n_st} structures of the workload with Morse targets, 90/10 split, "
-                  "alpha 1e-3 and 1e-1, held-out predictions"}}
+                  "alpha 1e-3 and 1e-1, held-out predictions; the mixed fit refined once "
+                  "(mlp_dev.refine, as mlp_dev.fit does)"}}
 
 
 def fit_end_to_end_leg(n_st: int, alphas=(1e-3,)):
@@ -281,25 +297,31 @@ def fit_end_to_end_leg(n_st: int, alphas=(1e-3,)):
         mlp_dev.solver_ridge(ne.xtx, ne.xty, np.asarray(alphas))
         t2 = time.perf_counter()
         del ne
-        # the path of mlp_dev.fit: the normal equations stay on the device
+        # the path of mlp_dev.fit: the normal equations stay on the device;
+        # the quantised form adds its refinement step (a residual pass over
+        # the rows and the correction solve)
         t3 = time.perf_counter()
-        dn = mlp_dev.accumulate_normal(model, [ds], precision=prec)
+        dn = mlp_dev.accumulate_normal(model, [ds], precision=prec, keep_parts=name == "mixed")
         t4 = time.perf_counter()
-        dn.acc.ridge(dn.scales, dn.zero, np.asarray(alphas))
+        c, _, _ = dn.acc.ridge(dn.scales, dn.zero, np.asarray(alphas))
         t5 = time.perf_counter()
+        if name == "mixed":
+            mlp_dev.refine(dn, c, np.asarray(alphas), prec)
+        t6 = time.perf_counter()
         del dn
-        out[name] = {"rows_per_s": rows / (t5 - t3), "seconds": t5 - t3,
-                     "accumulate_s": t4 - t3, "ridge_s": t5 - t4,
+        out[name] = {"rows_per_s": rows / (t6 - t3), "seconds": t6 - t3,
+                     "accumulate_s": t4 - t3, "ridge_s": t5 - t4, "refine_s": t6 - t5,
                      "read_back_path": {"rows_per_s": rows / (t2 - t0), "seconds": t2 - t0,
                                         "calc_xtx_xty_s": t1 - t0, "solver_ridge_s": t2 - t1}}
-        log(f"end-to-end {name}: device-resident {t5 - t3:.2f} s ({rows / (t5 - t3):.4g} rows/s;"
-            f" accumulate {t4 - t3:.2f} s, ridge {t5 - t4:.2f} s); read-back path "
-            f"{t2 - t0:.2f} s (calc_xtx_xty {t1 - t0:.2f} s, ridge {t2 - t1:.2f} s)")
+        log(f"end-to-end {name}: device-resident {t6 - t3:.2f} s ({rows / (t6 - t3):.4g} rows/s;"
+            f" accumulate {t4 - t3:.2f} s, ridge {t5 - t4:.2f} s, refine {t6 - t5:.2f} s); "
+            f"read-back path {t2 - t0:.2f} s (calc_xtx_xty {t1 - t0:.2f} s, ridge {t2 - t1:.2f} s)")
     out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
     out["workload"] = (f"{n_st} structures of the workload, cold: row weights, batch upload and "
                        "neighbour lists, accumulation, scaling and the ridge solve at alpha "
                        f"{list(alphas)} (mlp_dev.accumulate_normal + plm_normal_ridge, the path "
-                       "of mlp_dev.fit); read_back_path: calc_xtx_xty + solver_ridge")
+                       "of mlp_dev.fit), plus for the mixed form its refinement step "
+                       "(mlp_dev.refine); read_back_path: calc_xtx_xty + solver_ridge")
     return {"fit_end_to_end": out}
 
 

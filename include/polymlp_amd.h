@@ -186,6 +186,33 @@ int plm_normal_ridge(plm_normal* acc, const double* scales, const unsigned char*
 int plm_normal_sq_error(plm_normal* acc, const double* scales, const unsigned char* zero,
                         const double* coeffs, int n_coef, double* sq_err);
 
+/* Residual pass over a batch for n_coef (1..8) unscaled coefficient vectors
+ * (coeffs: n_coef x F): xtr[k] += X^T (y - X c_k) (n_coef x F) and
+ * sq[k] += |y - X c_k|^2, from the same weighted rows [X | y] that
+ * plm_normal_accumulate folds into the Gram (row_w / y_w in batch order; X
+ * built in FP64 for PLM_FP64 and PLM_MIXED64, FP32 for PLM_MIXED). No Gram is
+ * formed and the accumulator's sums are not changed; it supplies the model
+ * and the work buffers. sq is compute_rmse's numerator
+ * (utils_model_selection.py:37-47) taken directly from the rows; xtr is the
+ * residual of the normal equations of X itself, for the refinement step of
+ * plm_normal_ridge_correct. The reference solves in FP64 from an FP64 Gram
+ * and needs no such step; this serves its fit driver (fit.py:12-63) when the
+ * Gram is the quantised one. */
+int plm_normal_residual(plm_normal* acc, plm_batch* b, const double* row_w, const double* y_w,
+                        int precision, const double* coeffs, int n_coef, double* xtr, double* sq);
+/* Correction step of iterative refinement on an accumulator's scaled system
+ * (scaling as in plm_normal_ridge): delta_k = (A + alpha_k I)^{-1} rhs_k
+ * (Cholesky of the accumulated Gram's system in FP64, with one inner
+ * refinement step), rhs / delta: n_alpha x F, info as in plm_normal_ridge.
+ * With rhs_k = xtr_k / s - alpha_k c_k from plm_normal_residual, c_k + delta_k
+ * solves the normal equations of X to the accuracy of the X build instead of
+ * that of the quantised Gram. dad (n_alpha, may be NULL): delta_k^T A delta_k,
+ * for the squared error after the step. No counterpart in the reference (its
+ * Gram is FP64); serves its fit driver (fit.py:12-63). */
+int plm_normal_ridge_correct(plm_normal* acc, const double* scales, const unsigned char* zero,
+                             const double* alphas, int n_alpha, const double* rhs, double* delta,
+                             int* info, double* dad);
+
 /* FeaturesAttr (src/cxx/src/compute/py_features_attr.cpp:11-44): the
  * attributes of the columns of X. Per linear feature f (n_linear of them):
  * radial[f] its radial index, gtinv[f] its invariant (l-combination) index

@@ -36,6 +36,7 @@ SOURCES = [
     ("ridge.cpp", []),
     ("profile.cpp", []),
     ("kernels_solve.hip", []),
+    ("kernels_resid.hip", []),
     ("eval.cpp", []),
     ("eval_noalloc.hip", []),
     ("kernels_eval.hip", []),

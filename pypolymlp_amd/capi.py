@@ -28,7 +28,7 @@ EXPORTS = [
     "plm_batch_n_rows", "plm_batch_n_pairs", "plm_features_x", "plm_normal_create", "plm_normal_destroy",
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
     "plm_normal_ridge", "plm_normal_sq_error", "plm_normal_graph_create", "plm_normal_graph_run",
-    "plm_normal_graph_destroy",
+    "plm_normal_graph_destroy", "plm_normal_residual", "plm_normal_ridge_correct",
     "plm_solver_cg", "plm_model_feature_attrs",
     "plm_eval", "plm_eval_noalloc", "plm_eval_batch", "plm_eval_graph_create", "plm_eval_graph_run",
     "plm_eval_graph_destroy", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
@@ -341,6 +341,47 @@ class Normal:
                                          z.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _pd(C),
                                          C.shape[0], _pd(out)))
         return out
+
+    def residual(self, batch: Batch, w_ref, y_ref, coeffs_unscaled, precision: int = PLM_FP64):
+        """(X^T (y - X c) (F, k), |y - X c|^2 (k)) for each column c of
+        coeffs_unscaled (F, k), from the weighted rows of batch built as the
+        accumulation builds them (plm_normal_residual); up to 8 columns per
+        call, more are split."""
+        order = batch.row_order()
+        w = np.zeros(batch.n_rows)
+        y = np.zeros(batch.n_rows)
+        w[order] = w_ref
+        y[order] = y_ref
+        C = _d(np.asarray(coeffs_unscaled, dtype=float).reshape(self.model.n_features, -1).T)
+        xtr = np.zeros_like(C)
+        sq = np.zeros(C.shape[0])
+        for k0 in range(0, C.shape[0], 8):
+            k1 = min(C.shape[0], k0 + 8)
+            Ck = _d(C[k0:k1])
+            part = np.zeros((k1 - k0, C.shape[1]))
+            s = np.zeros(k1 - k0)
+            _check(lib().plm_normal_residual(self._h, batch.handle, _pd(w), _pd(y), precision,
+                                             _pd(Ck), k1 - k0, _pd(part), _pd(s)))
+            xtr[k0:k1] = part
+            sq[k0:k1] = s
+        return xtr.T.copy(), sq
+
+    def ridge_correct(self, scales, zero, alphas, rhs):
+        """(delta (F, n_alpha), info, delta^T A delta) of the refinement
+        correction (A + alpha_k I) delta_k = rhs_k on this accumulator's scaled
+        system (plm_normal_ridge_correct); rhs: (F, n_alpha)."""
+        F = self.model.n_features
+        a = _d(alphas)
+        sc = _d(scales)
+        z = None if zero is None else np.ascontiguousarray(zero, dtype=np.uint8)
+        R = _d(np.asarray(rhs, dtype=float).reshape(F, len(a)).T)
+        d = np.zeros((len(a), F))
+        info = np.zeros(len(a), dtype=np.int32)
+        dad = np.zeros(len(a))
+        _check(lib().plm_normal_ridge_correct(self._h, _pd(sc), None if z is None else
+                                              z.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                                              _pd(a), len(a), _pd(R), _pd(d), _pi(info), _pd(dad)))
+        return d.T.copy(), info, dad
 
 
 class NormalGraph:

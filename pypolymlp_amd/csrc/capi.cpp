@@ -1146,6 +1146,27 @@ int plm_normal_sq_error(plm_normal* ap, const double* scales, const unsigned cha
     });
 }
 
+int plm_normal_residual(plm_normal* ap, plm_batch* bp, const double* row_w, const double* y_w,
+                        int precision, const double* coeffs, int n_coef, double* xtr, double* sq) {
+    PLM_TRY({
+        if (!ap || !bp || !row_w || !y_w || !coeffs || !xtr || !sq)
+            throw std::invalid_argument("null argument");
+        normal_residual(*reinterpret_cast<NormalHandle*>(ap), *reinterpret_cast<BatchHandle*>(bp),
+                        row_w, y_w, fp64_features(precision), coeffs, n_coef, xtr, sq);
+    });
+}
+
+int plm_normal_ridge_correct(plm_normal* ap, const double* scales, const unsigned char* zero,
+                             const double* alphas, int n_alpha, const double* rhs, double* delta,
+                             int* info, double* dad) {
+    PLM_TRY({
+        if (!ap || !scales || !alphas || !rhs || !delta || !info)
+            throw std::invalid_argument("null argument");
+        normal_ridge_correct(*reinterpret_cast<NormalHandle*>(ap), scales, zero, alphas, n_alpha,
+                             rhs, delta, info, dad);
+    });
+}
+
 int plm_model_feature_attrs(const plm_model* mp, int* n_linear, int* n_poly, int* radial,
                             int* gtinv, int* tcomb, int* poly, int* type_pairs) {
     PLM_TRY({

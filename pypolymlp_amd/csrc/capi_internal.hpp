@@ -204,6 +204,7 @@ struct AccumPlan {
     std::vector<SubBatch> sb;
     DevBuf rmaps, doffs, w, y;
 };
+// quant: also the quantisation row maps (the int8 Gram's rows)
 void plan_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
                      bool fp64, bool quant, AccumPlan& plan);
 // launches only (no allocation beyond growing ws, no copy, no synchronisation)
@@ -226,6 +227,13 @@ struct NormalGraph {
 void normal_graph_capture(NormalGraph& g, const double* row_w, const double* y_w, bool fp64,
                           bool quant);
 void normal_graph_run(NormalGraph& g, int n_steps);
+
+// Residual pass over a batch's weighted rows for n_coef (1..8) unscaled
+// coefficient vectors (rows of coeffs, n_coef x F): xtr[k] += X^T (y - X c_k)
+// and sq[k] += |y - X c_k|^2, with X built in FP64 or FP32. The accumulator
+// supplies the model and its work buffers; its sums are not touched.
+void normal_residual(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
+                     bool fp64, const double* coeffs, int n_coef, double* xtr, double* sq);
 
 void upload_potential(ModelHandle& m);
 
@@ -268,6 +276,9 @@ void normal_ridge(NormalHandle& acc, const double* scales, const unsigned char* 
                   const double* alphas, int n_alpha, double* coeffs, int* info, double* sq_err);
 void normal_sq_error(NormalHandle& acc, const double* scales, const unsigned char* zero,
                      const double* coeffs, int n_coef, double* sq_err);
+void normal_ridge_correct(NormalHandle& acc, const double* scales, const unsigned char* zero,
+                          const double* alphas, int n_alpha, const double* rhs, double* delta,
+                          int* info, double* dad);
 // CG on (xtx + alpha I) c = xty from c0 (zeros when null); returns the
 // iteration at which the stopping test ended the loop, or max_iter
 int cg_solve(int F, const double* xtx, const double* xty, double alpha, const double* coef0,

@@ -148,6 +148,12 @@ void launch_gram_quantize(hipStream_t st, bool x_fp64, const void* X, const int*
 void launch_gram_i8(hipStream_t st, int ld, int rows_pad, int h0, int F_pad, const void* quant,
                     double* C, int ldc);
 
+// Residual pass (kernels_resid.hip): U[r][k] = X[r][F] - X[r][:ld] C[k][:ld]
+// (C rows zero from column F on), R[k][:ld] += X^T U[:, k], S[k] += U[:, k]^T U[:, k],
+// for n (1 to 8) coefficient rows C[k] of length ld; U holds rows x n.
+void launch_residual_pass(hipStream_t st, bool fp64, const void* X, int ld, int rows, int F,
+                          const double* C, int n, double* U, double* R, double* S);
+
 // Polynomial columns (kernels_poly.hip)
 void launch_poly_factor(hipStream_t st, bool fp64, const ModelDev& md, const PolyDev& pd,
                         const BatchDev& bd, int atom0, int n_atoms, const void* xall, int x_stride,

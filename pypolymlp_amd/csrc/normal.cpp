@@ -31,7 +31,6 @@ static constexpr long kStep = 64;                // rows_pad multiple of the Gra
 
 void plan_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
                      bool fp64, bool quant, AccumPlan& plan) {
-    if (!fp64 && !quant) throw std::invalid_argument("the FP64 Gram needs the FP64 X build");
     ModelHandle& m = *acc.model;
     hipStream_t st = device().stream;
     const int elem = fp64 ? 8 : 4;
@@ -156,6 +155,7 @@ void enqueue_accumulate(NormalHandle& acc, BatchHandle& b, const AccumPlan& plan
 
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
                        bool fp64, bool quant) {
+    if (!fp64 && !quant) throw std::invalid_argument("the FP64 Gram needs the FP64 X build");
     hipStream_t st = device().stream;
     AccumPlan plan;
     plan_accumulate(acc, b, row_w, y_w, fp64, quant, plan);
@@ -174,6 +174,7 @@ NormalGraph::~NormalGraph() {
 // the same launches captured with the accumulator's own sums as targets.
 void normal_graph_capture(NormalGraph& g, const double* row_w, const double* y_w, bool fp64,
                           bool quant) {
+    if (!fp64 && !quant) throw std::invalid_argument("the FP64 Gram needs the FP64 X build");
     NormalHandle& acc = *g.acc;
     hipStream_t st = device().stream;
     plan_accumulate(acc, *g.b, row_w, y_w, fp64, quant, g.plan);
@@ -203,6 +204,57 @@ void normal_graph_run(NormalGraph& g, int n_steps) {
     for (int k = 0; k < n_steps; ++k) check_hip(hipGraphLaunch(g.exec, st), "graph launch");
     check_hip(hipStreamSynchronize(st), "sync");
     g.acc->n_rows += (long)n_steps * g.plan.n_rows;
+}
+
+// The same sub-batches and X build as the accumulation, then the residual
+// kernels in place of the Gram. Used for one step of iterative refinement of
+// a solution from the quantised Gram (the residual of the normal equations
+// of X itself, X^T y - X^T X c = X^T (y - X c)) and for exact squared errors.
+void normal_residual(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
+                     bool fp64, const double* coeffs, int n_coef, double* xtr, double* sq) {
+    if (n_coef < 1 || n_coef > 8) throw std::invalid_argument("residual pass: 1 to 8 coefficient vectors");
+    ModelHandle& m = *acc.model;
+    hipStream_t st = device().stream;
+    const int F = acc.F, ld = acc.F_pad;
+    const int elem = fp64 ? 8 : 4;
+    AccumPlan plan;
+    plan_accumulate(acc, b, row_w, y_w, fp64, false, plan);
+    std::vector<double> hc((size_t)n_coef * ld, 0.0);
+    for (int k = 0; k < n_coef; ++k)
+        std::memcpy(&hc[(size_t)k * ld], coeffs + (size_t)k * F, (size_t)F * 8);
+    DevBuf C, R, S, U;
+    C.alloc(hc.size() * 8), R.alloc(hc.size() * 8), S.alloc((size_t)n_coef * 8);
+    check_hip(hipMemcpyAsync(C.ptr, hc.data(), hc.size() * 8, hipMemcpyHostToDevice, st), "copy");
+    check_hip(hipMemsetAsync(R.ptr, 0, hc.size() * 8, st), "memset");
+    check_hip(hipMemsetAsync(S.ptr, 0, (size_t)n_coef * 8, st), "memset");
+    long u_rows = 1;
+    for (const SubBatch& sb : plan.sb) u_rows = std::max(u_rows, sb.rows);
+    U.alloc((size_t)u_rows * n_coef * 8);
+    AccumWork& ws = acc.ws;
+    const double* w = (const double*)plan.w.ptr;
+    const double* y = (const double*)plan.y.ptr;
+    for (const SubBatch& sb : plan.sb) {
+        if (sb.rows == 0) continue;
+        const long r0 = b.row_off[sb.s0];
+        ws.xbuf.alloc((size_t)sb.rows_pad * ld * elem);
+        launch_fill_zero(st, (double*)ws.xbuf.ptr, (size_t)sb.rows_pad * ld * elem / 8);
+        build_x_rows(m, b, sb.s0, sb.s1, fp64, ws.xbuf.ptr, ld, w + r0, nullptr, nullptr, ws.work,
+                     true, (const long*)plan.doffs.ptr + sb.doff_off);
+        launch_set_column(st, fp64, ws.xbuf.ptr, ld, F, y + r0, (int)sb.rows);
+        check_hip(hipGetLastError(), "target column");
+        ProfScope ps("residual", st);
+        launch_residual_pass(st, fp64, ws.xbuf.ptr, ld, (int)sb.rows, F, (const double*)C.ptr,
+                             n_coef, (double*)U.ptr, (double*)R.ptr, (double*)S.ptr);
+        check_hip(hipGetLastError(), "residual launch");
+    }
+    std::vector<double> hr(hc.size()), hs(n_coef);
+    check_hip(hipMemcpyAsync(hr.data(), R.ptr, hr.size() * 8, hipMemcpyDeviceToHost, st), "copy");
+    check_hip(hipMemcpyAsync(hs.data(), S.ptr, hs.size() * 8, hipMemcpyDeviceToHost, st), "copy");
+    check_hip(hipStreamSynchronize(st), "sync");
+    for (int k = 0; k < n_coef; ++k) {
+        for (int f = 0; f < F; ++f) xtr[(size_t)k * F + f] += hr[(size_t)k * ld + f];
+        sq[k] += hs[k];
+    }
 }
 
 }  // namespace plm

@@ -25,9 +25,10 @@ void launch_symv(hipStream_t st, const double* A, int lda, int n, const double* 
 // For each alpha: Cholesky of (A0 + alpha I) (A0: full symmetric n_pad x
 // n_pad on the device, identity on the padding), the solve with b, one
 // refinement step with the residual of the unfactored system. coeffs: n_alpha
-// x F on the host.
+// x F on the host. b: one right-hand side of np entries on the device, or,
+// with b_stride np, one per alpha.
 static void ridge_core(int F, int np, const DevBuf& A0, const DevBuf& b, const double* alphas,
-                       int n_alpha, double* coeffs, int* info) {
+                       int n_alpha, double* coeffs, int* info, size_t b_stride = 0) {
     hipStream_t st = device().stream;
     const size_t mat = (size_t)np * np * 8;
     DevBuf Af, x, r, di;
@@ -44,12 +45,12 @@ static void ridge_core(int F, int np, const DevBuf& A0, const DevBuf& b, const d
         check_hip(hipGetLastError(), "cholesky launch");
         {
             ProfScope ps("chol_solve", st);
-            check_hip(hipMemcpyAsync(x.ptr, b.ptr, np * 8, hipMemcpyDeviceToDevice, st), "copy");
+            const double* bk = (const double*)b.ptr + k * b_stride;
+            check_hip(hipMemcpyAsync(x.ptr, bk, np * 8, hipMemcpyDeviceToDevice, st), "copy");
             launch_chol_solve(st, (double*)Af.ptr, np, np, (double*)x.ptr);
             // one step of iterative refinement
             check_hip(hipMemsetAsync(r.ptr, 0, np * 8, st), "memset");
-            launch_residual(st, (double*)A0.ptr, np, F, a, (double*)x.ptr, (double*)b.ptr,
-                            (double*)r.ptr);
+            launch_residual(st, (double*)A0.ptr, np, F, a, (double*)x.ptr, bk, (double*)r.ptr);
             launch_chol_solve(st, (double*)Af.ptr, np, np, (double*)r.ptr);
             launch_axpy(st, F, (double*)r.ptr, (double*)x.ptr);
         }
@@ -147,6 +148,39 @@ void normal_sq_error(NormalHandle& acc, const double* scales, const unsigned cha
     double yty = 0.0;
     const int np = build_system(acc, scales, zero, A0, b, hb, yty);
     sq_errors(acc.F, np, A0, hb, yty, coeffs, n_coef, sq_err);
+}
+
+// The correction step of iterative refinement for a solution c_k of the
+// scaled system of an accumulator: delta_k = (A0 + alpha_k I)^{-1} rhs_k, with
+// rhs_k = D^{-1} X^T (y - X D^{-1} c_k) - alpha_k c_k from a residual pass over
+// the data (the residual of the normal equations of X, not of the
+// accumulated Gram). rhs / delta: n_alpha x F on the host (dropped columns
+// zero); dad[k] = delta_k^T A0 delta_k when dad is not null.
+void normal_ridge_correct(NormalHandle& acc, const double* scales, const unsigned char* zero,
+                          const double* alphas, int n_alpha, const double* rhs, double* delta,
+                          int* info, double* dad) {
+    if (n_alpha <= 0) throw std::invalid_argument("no alpha");
+    hipStream_t st = device().stream;
+    check_hip(hipStreamSynchronize(st), "sync");
+    DevBuf A0, b;
+    std::vector<double> hb;
+    double yty = 0.0;
+    const int F = acc.F;
+    const int np = build_system(acc, scales, zero, A0, b, hb, yty);
+    std::vector<unsigned char> z(F, 0);
+    if (zero) std::memcpy(z.data(), zero, F);
+    std::vector<double> hr((size_t)n_alpha * np, 0.0);
+    for (int k = 0; k < n_alpha; ++k)
+        for (int i = 0; i < F; ++i)
+            if (!z[i]) hr[(size_t)k * np + i] = rhs[(size_t)k * F + i];
+    DevBuf R;
+    R.alloc(hr.size() * 8);
+    check_hip(hipMemcpyAsync(R.ptr, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, st), "copy");
+    ridge_core(F, np, A0, R, alphas, n_alpha, delta, info, (size_t)np);
+    if (dad) {
+        const std::vector<double> zb(np, 0.0);
+        sq_errors(F, np, A0, zb, 0.0, delta, n_alpha, dad);
+    }
 }
 
 }  // namespace plm

@@ -305,38 +305,90 @@ class FitResult:
 @dataclass
 class DeviceNormal:
     """Normal equations left on the GPU: the accumulator (X^T X, X^T y, y^T y
-    of [X | y]) and what the host needs from it."""
+    of [X | y]) and what the host needs from it. parts: the device batches
+    with their row weights and weighted targets, when kept for a residual
+    pass over the same rows."""
     acc: capi.Normal
     scales: np.ndarray
     zero: np.ndarray
     min_energy: float
     total_n_data: int
     y_sq_norm: float
+    parts: Optional[list] = None
+
+
+def _parts(model, datasets: Sequence[Dataset], min_energy, weight_stress, batch_size):
+    """(device batch, w, y_w) per batch of batch_size structures, datasets
+    sorted by atom count (the batching of _accumulate)."""
+    for ds in datasets:
+        ds.sort_by_atoms()
+        for b0 in range(0, len(ds.structures), batch_size):
+            sub = ds.subset(slice(b0, min(len(ds.structures), b0 + batch_size)))
+            w, yw = row_weights(sub, min_energy, weight_stress)
+            sts = sub.structures
+            batch = capi.Batch(model, [s.axis for s in sts], [s.positions_cartesian for s in sts],
+                               [s.types for s in sts])
+            yield batch, w, yw
 
 
 def accumulate_normal(model: capi.Model, datasets: Sequence[Dataset], precision=capi.PLM_FP64,
                       scales: Optional[np.ndarray] = None, min_energy: Optional[float] = None,
                       weight_stress: float = 0.1, batch_size: int = 4096,
-                      scale_threshold: float = 1e-10) -> DeviceNormal:
+                      scale_threshold: float = 1e-10, keep_parts: bool = False) -> DeviceNormal:
     """calc_xtx_xty (data_sequential.py:25-94) without reading X^T X back: the
     accumulation is the same; only the energy-row sums and y^T y come to the
     host, where the scales are computed (utils_scales.py:6-40). The scaling
     of X^T X and X^T y is applied on the device by DeviceNormal.acc.ridge /
-    sq_error."""
+    sq_error. keep_parts: keep the device batches for a residual pass."""
     if min_energy is None:
         min_energy = get_min_energy(datasets)
     acc = capi.Normal(model)
-    for ds in datasets:
-        ds.sort_by_atoms()
-        _accumulate(model, acc, ds, 0, len(ds.structures), min_energy, weight_stress,
-                    batch_size, precision)
+    kept = [] if keep_parts else None
+    for batch, w, yw in _parts(model, datasets, min_energy, weight_stress, batch_size):
+        acc.accumulate(batch, w, yw, precision)
+        if keep_parts:
+            kept.append((batch, w, yw))
     xe_sum, xe_sq, y_sq, _ = acc.vectors()
     n_data = sum(len(d.energies) for d in datasets)
     scales, zero = compute_scales(None if scales is None else np.array(scales, dtype=float),
                                   xe_sum, xe_sq, n_data, include_force=datasets[0].include_force,
                                   threshold=scale_threshold)
     return DeviceNormal(acc=acc, scales=scales, zero=zero, min_energy=float(min_energy),
-                        total_n_data=sum(reference_n_rows(d) for d in datasets), y_sq_norm=y_sq)
+                        total_n_data=sum(reference_n_rows(d) for d in datasets), y_sq_norm=y_sq,
+                        parts=kept)
+
+
+def residual_sums(acc: capi.Normal, parts, coeffs_unscaled, precision=capi.PLM_FP64):
+    """(X^T (y - X c) (F, k), |y - X c|^2 (k)) over the rows of parts for each
+    column c of coeffs_unscaled (F, k) (plm_normal_residual per batch)."""
+    C = np.asarray(coeffs_unscaled, dtype=float)
+    R = np.zeros(C.shape)
+    sq = np.zeros(C.shape[1])
+    for batch, w, yw in parts:
+        r, s = acc.residual(batch, w, yw, C, precision)
+        R += r
+        sq += s
+    return R, sq
+
+
+def refine(dn: DeviceNormal, coefs, alphas, precision):
+    """One step of iterative refinement of ridge solutions (F, n_alpha) of
+    the scaled system of dn, whose Gram is the quantised one: the residual of
+    the normal equations of X itself, D^-1 X^T (y - X D^-1 c) - alpha c, from a
+    residual pass over the kept rows, then the correction solved with the
+    accumulated system. Returns (coefs, squared errors of the corrected
+    solutions, info). The squared error is |y - X c|^2 of the pass minus
+    2 delta^T g plus delta^T A delta (the last term from the accumulated
+    system, second order in the correction)."""
+    alphas = np.asarray(alphas, dtype=float)
+    sc = dn.scales[:, None]
+    R, sq_u = residual_sums(dn.acc, dn.parts, coefs / sc, precision)
+    g = R / sc
+    g[dn.zero] = 0.0
+    rhs = g - alphas[None, :] * coefs
+    delta, info, dad = dn.acc.ridge_correct(dn.scales, dn.zero, alphas, rhs)
+    delta[:, info != 0] = 0.0
+    return coefs + delta, sq_u - 2.0 * np.sum(delta * g, axis=0) + dad, info
 
 
 def _rmse_from_sq(sq, n_data):
@@ -346,20 +398,35 @@ def _rmse_from_sq(sq, n_data):
 
 
 def fit(model: capi.Model, train: List[Dataset], test: List[Dataset], alphas,
-        precision=capi.PLM_FP64) -> FitResult:
+        precision=capi.PLM_FP64, refine_quantised: bool = True, weight_stress: float = 0.1,
+        batch_size: int = 4096) -> FitResult:
     """Sequential ridge fit with model selection on the test RMSE (fit.py:12-63).
     The normal equations stay on the GPU: they are scaled and solved there
-    (plm_normal_ridge), and the training and test RMSEs come from device
-    products with them (plm_normal_sq_error); only vectors cross to the host."""
+    (plm_normal_ridge); only vectors cross to the host. With the quantised
+    Gram (PLM_MIXED, PLM_MIXED64) each solution gets one step of iterative
+    refinement against the rows themselves (refine). The training RMSE comes
+    from the device system (plm_normal_sq_error's form) or, after refinement,
+    from the residual pass; the test RMSE from a residual pass over the test
+    rows with the X build of the fit (no test Gram is formed)."""
     alphas = np.asarray(alphas, dtype=float)
-    tr = accumulate_normal(model, train, precision=precision)
+    quantised = precision != capi.PLM_FP64
+    do_refine = quantised and refine_quantised
+    tr = accumulate_normal(model, train, precision=precision, weight_stress=weight_stress,
+                           batch_size=batch_size, keep_parts=do_refine)
     coefs, info, sq = tr.acc.ridge(tr.scales, tr.zero, alphas)
-    sq = np.where(info == 0, sq, np.inf)
+    ok = info == 0
+    coefs[:, ~ok] = 0.0
+    if do_refine:
+        coefs, sq, info2 = refine(tr, coefs, alphas, precision)
+        ok &= info2 == 0
+        tr.parts = None
+    sq = np.where(ok, sq, np.inf)
     rmse_train = _rmse_from_sq(sq, tr.total_n_data)
-    te = accumulate_normal(model, test, precision=precision, scales=tr.scales.copy(),
-                           min_energy=tr.min_energy)
-    sq_te = np.where(info == 0, te.acc.sq_error(te.scales, te.zero, coefs), np.inf)
-    rmse_test = _rmse_from_sq(sq_te, te.total_n_data)
+    te_parts = _parts(model, test, tr.min_energy, weight_stress, batch_size)
+    _, sq_te = residual_sums(tr.acc, te_parts, coefs / tr.scales[:, None], precision)
+    sq_te = np.where(ok, sq_te, np.inf)
+    rmse_test = _rmse_from_sq(sq_te, sum(reference_n_rows(d) for d in test))
+    coefs[:, ~ok] = 1e30
     k = int(np.argmin(rmse_test))
     return FitResult(coeffs=coefs[:, k], scales=tr.scales, alpha=float(alphas[k]),
                      rmse_train=float(rmse_train[k]), rmse_test=float(rmse_test[k]))
