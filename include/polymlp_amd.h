@@ -200,6 +200,14 @@ int plm_normal_sq_error(plm_normal* acc, const double* scales, const unsigned ch
  * Gram is the quantised one. */
 int plm_normal_residual(plm_normal* acc, plm_batch* b, const double* row_w, const double* y_w,
                         int precision, const double* coeffs, int n_coef, double* xtr, double* sq);
+/* Keep the weighted rows [X | y] that later plm_normal_accumulate calls build
+ * (per sub-batch, while their total stays within max_bytes of device
+ * memory), so that plm_normal_residual over the same batch with the same
+ * row_w / y_w and precision reads them instead of building them again.
+ * max_bytes = 0 releases the kept rows and keeps none. No counterpart in the
+ * reference (its X is a host array, data_sequential.py:97-155); it serves the
+ * refinement step of a fit from the quantised Gram. */
+int plm_normal_keep_rows(plm_normal* acc, unsigned long long max_bytes);
 /* Correction step of iterative refinement on an accumulator's scaled system
  * (scaling as in plm_normal_ridge): delta_k = (A + alpha_k I)^{-1} rhs_k
  * (Cholesky of the accumulated Gram's system in FP64, with one inner
@@ -212,6 +220,16 @@ int plm_normal_residual(plm_normal* acc, plm_batch* b, const double* row_w, cons
 int plm_normal_ridge_correct(plm_normal* acc, const double* scales, const unsigned char* zero,
                              const double* alphas, int n_alpha, const double* rhs, double* delta,
                              int* info, double* dad);
+
+/* The launch shapes the autotuner chose for this model so far, one text line
+ * per class (precision, largest-structure row bucket, structure-count
+ * bucket): tasks per workgroup, waves per task, slot parts, static grid or
+ * persistent work queue, and its time against the rule-based shape. buf (may
+ * be NULL) receives at most cap - 1 characters and a terminating zero; len (may
+ * be NULL) the full length. No counterpart in the reference (its X build is
+ * OpenMP over structures, py_model.cpp:41); it documents the shapes of this
+ * library's signature-group kernel. */
+int plm_model_tuning(const plm_model* m, char* buf, int cap, int* len);
 
 /* FeaturesAttr (src/cxx/src/compute/py_features_attr.cpp:11-44): the
  * attributes of the columns of X. Per linear feature f (n_linear of them):
@@ -298,8 +316,10 @@ int plm_api_sum_of_prod(plm_model* m, int type, const double* anlm_re, const dou
  * with HIP events recorded on the library's own stream, for roofline
  * figures. Names: "pairdata", "anlm", "wprime", "xenergy", "xgroup" (linear
  * columns by signature groups), "xrows" (linear columns by column blocks),
- * "poly_deriv", "poly_gemm", "poly_gather" (polynomial columns), "quantize",
- * "gram", "eval_anlm", "eval_prodsum", "eval_pair", "cholesky". */
+ * "poly_deriv", "poly_z", "poly_rows" (polynomial columns: factor
+ * derivatives, entry products, column sums into the rows), "quantize",
+ * "gram", "residual" (residual pass), "eval_anlm", "eval_prodsum",
+ * "eval_pair", "cholesky", "chol_solve". */
 int plm_profile_enable(int on);
 int plm_profile_reset(void);
 int plm_profile_get(const char* name, double* total_ms, long* launches);

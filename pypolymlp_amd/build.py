@@ -37,6 +37,7 @@ SOURCES = [
     ("profile.cpp", []),
     ("kernels_solve.hip", []),
     ("kernels_resid.hip", []),
+    ("kernels_polyz.hip", []),
     ("eval.cpp", []),
     ("eval_noalloc.hip", []),
     ("kernels_eval.hip", []),
@@ -71,7 +72,7 @@ def build_library(jobs: int = 8) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda a: _compile(*a), SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lrocblas"]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stderr}")

@@ -29,6 +29,7 @@ EXPORTS = [
     "plm_normal_accumulate", "plm_normal_result", "plm_batch_row_order", "plm_ridge",
     "plm_normal_ridge", "plm_normal_sq_error", "plm_normal_graph_create", "plm_normal_graph_run",
     "plm_normal_graph_destroy", "plm_normal_residual", "plm_normal_ridge_correct",
+    "plm_model_tuning", "plm_normal_keep_rows",
     "plm_solver_cg", "plm_model_feature_attrs",
     "plm_eval", "plm_eval_noalloc", "plm_eval_batch", "plm_eval_graph_create", "plm_eval_graph_run",
     "plm_eval_graph_destroy", "plm_profile_enable", "plm_profile_reset", "plm_profile_get",
@@ -342,6 +343,12 @@ class Normal:
                                          C.shape[0], _pd(out)))
         return out
 
+    def keep_rows(self, max_bytes: int):
+        """Keep the weighted rows of later accumulations (up to max_bytes of
+        device memory) for residual passes over the same batches
+        (plm_normal_keep_rows); 0 releases them."""
+        _check(lib().plm_normal_keep_rows(self._h, ctypes.c_ulonglong(int(max_bytes))))
+
     def residual(self, batch: Batch, w_ref, y_ref, coeffs_unscaled, precision: int = PLM_FP64):
         """(X^T (y - X c) (F, k), |y - X c|^2 (k)) for each column c of
         coeffs_unscaled (F, k), from the weighted rows of batch built as the
@@ -517,6 +524,15 @@ def ridge(xtx, xty, alphas) -> np.ndarray:
     _check(lib().plm_ridge(F, _pd(A), _pd(b), _pd(al), len(al), _pd(out), _pi(info)))
     out[info != 0] = 1e30
     return out
+
+
+def tuning_report(model: "Model") -> str:
+    """The launch shapes the autotuner chose for model so far (plm_model_tuning)."""
+    n = ctypes.c_int()
+    _check(lib().plm_model_tuning(model.handle, None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _check(lib().plm_model_tuning(model.handle, buf, n.value + 1, None))
+    return buf.value.decode()
 
 
 def feature_attrs(model: "Model"):

@@ -2,8 +2,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -441,6 +443,8 @@ const ColumnTables& ModelHandle::columns(int CB, int cover) {
 BatchHandle::BatchHandle(ModelHandle* m, double cutoff_, int n_st_, const double* axes,
                          const double* pos, const int* types_in, const int* n_atoms_in)
     : model(m), cutoff(cutoff_), n_st(n_st_) {
+    static std::atomic<unsigned long long> next_serial{0};
+    serial = ++next_serial;
     Device& dev = device();
     atom_off.assign(n_st + 1, 0);
     for (int s = 0; s < n_st; ++s) {
@@ -641,23 +645,160 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover) {
     throw std::runtime_error("column block too large for LDS");
 }
 
-// Signature-group path for the linear columns: tasks per workgroup (TB) and
-// waves per task (A) for the largest structure, or TB = 0 when its rows do
-// not fit the LDS tile (or the path is switched off: PLM_XROWS=generic or global).
-static void plan_group(ModelHandle& m, int rows_max, bool fp64, int& TB, int& A) {
-    TB = 0, A = 0;
+// Signature-group path for the linear columns: the launch shapes whose row
+// tile holds the largest structure's rows in LDS, the rule-based shape first
+// (the largest TB <= 8 dividing the waves, all waves used, static grid). TB
+// tasks per workgroup, A waves per task; when not even one task's tile fits,
+// the tile is cut into slot parts (split 2 or 4, TB = 1). Empty when the path
+// is off (PLM_XROWS=generic or global) or the model has no signature groups.
+struct GroupShape {
+    int TB, A, split;
+    bool persistent;
+};
+
+static std::vector<GroupShape> group_shapes(ModelHandle& m, int rows_max, bool fp64) {
+    std::vector<GroupShape> out;
     const char* env = std::getenv("PLM_XROWS");
-    if (env && (std::string(env) == "generic" || std::string(env) == "global")) return;
-    if (!m.groups().enabled) return;
+    if (env && (std::string(env) == "generic" || std::string(env) == "global")) return out;
+    if (!m.groups().enabled) return out;
     const int NSW = xgroup_slots(fp64), elem = 8;  // the row tile is FP64 in both precisions
     const size_t per_col = (size_t)(rows_max - 1) * elem;
-    const int cols = (int)((kLdsBytes - 1024) / per_col) - 1;
+    const int cap = (int)((kLdsBytes - 1024) / per_col);  // tile columns that fit, padding included
     const int waves = xgroup_max_waves(fp64);
-    int tb = std::min(8, cols / NSW);
-    while (tb > 1 && waves % tb) --tb;  // the waves split evenly over the tasks
-    if (tb < 1) return;
-    TB = tb;
-    A = waves / TB;
+    int split = 1;
+    while (split <= 4 && NSW / split + 1 > cap) split *= 2;
+    if (split > 4) return out;
+    if (split > 1) {
+        out.push_back({1, waves, split, false});
+        out.push_back({1, waves, split, true});
+        return out;
+    }
+    int tb_rule = std::min(8, (cap - 1) / NSW);
+    while (tb_rule > 1 && waves % tb_rule) --tb_rule;
+    out.push_back({tb_rule, waves / tb_rule, 1, false});
+    for (int tb : {1, 2, 4, 8}) {
+        if (tb * NSW + 1 > cap || waves % tb) continue;
+        for (int a : {waves / tb, waves / tb / 2}) {
+            if (a < 1) continue;
+            for (bool pers : {false, true})
+                if (!(tb == tb_rule && a == waves / tb_rule && !pers))
+                    out.push_back({tb, a, 1, pers});
+        }
+    }
+    return out;
+}
+
+// One signature-group pass (one launch per atom type) over structures [s0, s1).
+static void run_group(hipStream_t st, bool fp64, ModelHandle& m, const BatchDev& bd,
+                      const GroupShape& g, int s0, int s1, int row_base, int a0, long pb, int ld,
+                      int rows_max, XWork& work, const PairRec& R, void* X, const double* roww) {
+    const GroupTables& G = m.groups();
+    const int H = kGroupSlots / xgroup_slots(fp64);
+    const int nt = m.host.n_type();
+    if (g.persistent) {
+        work.queue.alloc((size_t)nt * 8 * sizeof(int));
+        // cleared by a kernel (a memset node would not replay correctly in a graph)
+        launch_fill_zero(st, (double*)work.queue.ptr, (size_t)nt * 4);
+    }
+    for (int t = 0; t < nt; ++t) {
+        XGroupParams P{};
+        P.TB = g.TB, P.A = g.A, P.split = g.split;
+        P.vt0 = G.type_task0[t] * H, P.vt1 = G.type_task0[t + 1] * H;
+        if (P.vt1 <= P.vt0) continue;
+        P.n_blocks = (P.vt1 - P.vt0 + g.TB - 1) / g.TB;
+        P.ld = ld, P.st0 = s0, P.st_end = s1, P.row_base = row_base;
+        P.xcd_map = (s1 - s0) >= 16 ? 1 : 0;
+        P.atom_base = a0;
+        P.pair_base = pb;
+        P.w_stride = std::max(m.wz_max_type, 1);
+        if (g.persistent) {
+            P.queue = (int*)work.queue.ptr + 8 * t;
+            const long items = (long)(s1 - s0) * P.n_blocks * P.split;
+            P.grid = (int)std::min<long>(items, xgroup_persistent_grid(P, rows_max, fp64));
+        }
+        launch_xgroup(st, fp64, G.dev, bd, P, rows_max, work.w.ptr, work.pd.ptr, R,
+                      (const int*)work.tpo.ptr, X, roww);
+    }
+}
+
+static bool stream_capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    check_hip(hipStreamIsCapturing(st, &cs), "capture status");
+    return cs != hipStreamCaptureStatusNone;
+}
+
+// The autotuner of the signature-group launch: on the first sub-batch of a
+// (precision, largest-structure row bucket, structure-count bucket) class,
+// every candidate shape runs the pass over the sub-batch's largest structures
+// (at most 64, and at most 1 GiB of linear-column rows) into scratch rows,
+// once to warm up and once timed with HIP events; the fastest is kept on the
+// model for the class. Off with PLM_AUTOTUNE=0 and while the stream is being
+// captured (then the rule-based shape, or the class's earlier choice, is used).
+// PLM_XGROUP="TB,A,split,persistent" forces one shape.
+static GroupShape choose_group(ModelHandle& m, BatchHandle& b, const BatchDev& bd, int s0, int s1,
+                               int rows_max, bool fp64, int a0, long pb, XWork& work,
+                               const PairRec& R, const std::vector<GroupShape>& shapes,
+                               hipStream_t st) {
+    // PLM_XGROUP="TB,A,split,persistent" forces a shape (the launch checks it)
+    if (const char* f = std::getenv("PLM_XGROUP")) {
+        GroupShape g{};
+        int pers = 0;
+        if (std::sscanf(f, "%d,%d,%d,%d", &g.TB, &g.A, &g.split, &pers) != 4)
+            throw std::invalid_argument("PLM_XGROUP: TB,A,split,persistent");
+        g.persistent = pers != 0;
+        return g;
+    }
+    const int n_st = s1 - s0;
+    int nb = 1;
+    while (nb < std::min(n_st, 64)) nb *= 2;
+    const auto key = std::make_tuple(fp64 ? 1 : 0, (rows_max + 95) / 96, nb);
+    // a class's earlier choice, when it also holds this sub-batch's rows
+    auto it = m.group_tuning.find(key);
+    if (it != m.group_tuning.end())
+        for (const GroupShape& g : shapes)
+            if (g.TB == it->second.TB && g.A == it->second.A && g.split == it->second.split &&
+                g.persistent == it->second.persistent)
+                return g;
+    const char* env = std::getenv("PLM_AUTOTUNE");
+    if (shapes.size() == 1 || (env && std::string(env) == "0") || stream_capturing(st))
+        return shapes[0];
+    // tuning structures: the largest ones (batches are in ascending size order)
+    const int elem = fp64 ? 8 : 4;
+    const int nlin = m.host.n_linear();
+    const int ldt = (nlin + 3) / 4 * 4;
+    int t0 = s1;
+    size_t bytes = 0;
+    while (t0 > s0 && s1 - t0 < 64) {
+        const size_t add = (size_t)(b.row_off[t0] - b.row_off[t0 - 1]) * ldt * elem;
+        if (t0 < s1 && bytes + add > (1ull << 30)) break;
+        bytes += add, --t0;
+    }
+    work.tune_x.alloc(bytes);
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), "event");
+    check_hip(hipEventCreate(&e1), "event");
+    ModelHandle::GroupChoice best;
+    best.ms = 1e30;
+    for (size_t k = 0; k < shapes.size(); ++k) {
+        const GroupShape& g = shapes[k];
+        run_group(st, fp64, m, bd, g, t0, s1, b.row_off[t0], a0, pb, ldt, rows_max, work, R,
+                  work.tune_x.ptr, nullptr);
+        check_hip(hipEventRecord(e0, st), "event");
+        run_group(st, fp64, m, bd, g, t0, s1, b.row_off[t0], a0, pb, ldt, rows_max, work, R,
+                  work.tune_x.ptr, nullptr);
+        check_hip(hipEventRecord(e1, st), "event");
+        check_hip(hipEventSynchronize(e1), "event sync");
+        float ms = 0.f;
+        check_hip(hipEventElapsedTime(&ms, e0, e1), "event time");
+        if (k == 0) best.rule_ms = ms;
+        if (ms < best.ms) best.TB = g.TB, best.A = g.A, best.split = g.split,
+                          best.persistent = g.persistent, best.ms = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    best.candidates = (int)shapes.size();
+    m.group_tuning[key] = best;
+    return {best.TB, best.A, best.split, best.persistent};
 }
 
 void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, void* X, int ld,
@@ -674,9 +815,8 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     int rows_max = 0;
     for (int s = s0; s < s1; ++s) rows_max = std::max(rows_max, 7 + 3 * (b.atom_off[s + 1] - b.atom_off[s]));
     const bool use_poly = m.poly().enabled;
-    int TB = 0, A = 0;
-    plan_group(m, rows_max, fp64, TB, A);
-    const GroupDev* gd = TB > 0 ? &m.groups().dev : nullptr;
+    const std::vector<GroupShape> shapes = group_shapes(m, rows_max, fp64);
+    const GroupDev* gd = shapes.empty() ? nullptr : &m.groups().dev;
     // pair records (grouped by type pair), then the order parameters from them
     const long pb = b.nb_off[a0], pe = b.nb_off[a1];
     const PairRec R(m.md.n_fn, m.md.n_lmh);
@@ -697,18 +837,15 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
     check_hip(hipGetLastError(), "anlm launch");
     work.w.alloc((size_t)na * w_stride * 2 * elem);
     work.x.alloc((size_t)na * x_stride * elem);
-    if (gd) work.wg.alloc((size_t)na * gd->wg_stride * 2 * elem);
     {
         ProfScope ps("wprime", st);
         launch_wprime(st, fp64, m.md, bd, work.a.ptr, a_stride, a0, na, work.w.ptr, w_stride,
-                      work.x.ptr, x_stride, m.max_nlm, m.w_maxf, m.v_maxf, gd,
-                      gd ? work.wg.ptr : nullptr);
+                      work.x.ptr, x_stride, m.max_nlm, m.w_maxf, m.v_maxf);
     }
     check_hip(hipGetLastError(), "wprime launch");
     if (gd) {
         // linear columns by signature groups: energy rows from the feature
         // values, then one launch per atom type adding its stress and force rows
-        const GroupTables& G = m.groups();
         const int nlin = m.host.n_linear();
         const long rows = b.row_off[s1] - b.row_off[s0];
         if (!x_is_zero)
@@ -719,21 +856,11 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
             launch_xenergy(st, fp64, *gd, bd, nlin, s0, s1 - s0, b.row_off[s0], a0, work.x.ptr,
                            x_stride, X, ld, roww, xe_sum, xe_sq);
         }
-        const int H = kGroupSlots / xgroup_slots(fp64);
+        const GroupShape g = choose_group(m, b, bd, s0, s1, rows_max, fp64, a0, pb, work, R,
+                                          shapes, st);
         ProfScope pg("xgroup", st);
-        for (int t = 0; t < m.host.n_type(); ++t) {
-            XGroupParams P{};
-            P.TB = TB, P.A = A;
-            P.vt0 = G.type_task0[t] * H, P.vt1 = G.type_task0[t + 1] * H;
-            if (P.vt1 <= P.vt0) continue;
-            P.n_blocks = (P.vt1 - P.vt0 + TB - 1) / TB;
-            P.ld = ld, P.st0 = s0, P.st_end = s1, P.row_base = b.row_off[s0];
-            P.xcd_map = (s1 - s0) >= 16 ? 1 : 0;
-            P.atom_base = a0;
-            P.pair_base = pb;
-            launch_xgroup(st, fp64, *gd, bd, P, rows_max, work.wg.ptr, work.pd.ptr, R,
-                          (const int*)work.tpo.ptr, X, roww);
-        }
+        run_group(st, fp64, m, bd, g, s0, s1, b.row_off[s0], a0, pb, ld, rows_max, work, R, X,
+                  roww);
     } else {
         XPlan pl = plan_x(m, rows_max, fp64, use_poly ? m.host.n_linear() : m.host.n_features());
         pl.P.ld = ld;
@@ -755,7 +882,7 @@ void build_x_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, voi
 size_t atom_work_bytes(ModelHandle& m, bool fp64) {
     const size_t elem = fp64 ? 8 : 4;
     const GroupTables& G = m.groups();
-    const size_t grouped = G.enabled ? 2 * (size_t)G.dev.wg_stride * elem + (m.md.n_tp + 1) * 4 : 0;
+    const size_t grouped = G.enabled ? (size_t)(m.md.n_tp + 1) * 4 : 0;
     return elem * (2 * (size_t)m.max_nlm + 2 * (size_t)std::max(m.wz_max_type, 1) +
                    (size_t)std::max(m.lin_max_type, 1)) + grouped;
 }
@@ -1156,6 +1283,13 @@ int plm_normal_residual(plm_normal* ap, plm_batch* bp, const double* row_w, cons
     });
 }
 
+int plm_normal_keep_rows(plm_normal* ap, unsigned long long max_bytes) {
+    PLM_TRY({
+        if (!ap) throw std::invalid_argument("null argument");
+        normal_keep_rows(*reinterpret_cast<NormalHandle*>(ap), (size_t)max_bytes);
+    });
+}
+
 int plm_normal_ridge_correct(plm_normal* ap, const double* scales, const unsigned char* zero,
                              const double* alphas, int n_alpha, const double* rhs, double* delta,
                              int* info, double* dad) {
@@ -1164,6 +1298,32 @@ int plm_normal_ridge_correct(plm_normal* ap, const double* scales, const unsigne
             throw std::invalid_argument("null argument");
         normal_ridge_correct(*reinterpret_cast<NormalHandle*>(ap), scales, zero, alphas, n_alpha,
                              rhs, delta, info, dad);
+    });
+}
+
+int plm_model_tuning(const plm_model* mp, char* buf, int cap, int* len) {
+    PLM_TRY({
+        if (!mp) throw std::invalid_argument("null model");
+        const auto* m = reinterpret_cast<const ModelHandle*>(mp);
+        std::string out;
+        char line[256];
+        for (const auto& kv : m->group_tuning) {
+            const auto& c = kv.second;
+            std::snprintf(line, sizeof line,
+                          "xgroup %s rows<=%d structures~%d: TB=%d A=%d split=%d %s, %.3f ms "
+                          "(rule-based shape %.3f ms, %d candidates)\n",
+                          std::get<0>(kv.first) ? "fp64" : "fp32", 96 * std::get<1>(kv.first),
+                          std::get<2>(kv.first), c.TB, c.A, c.split,
+                          c.persistent ? "persistent queue" : "static grid", c.ms, c.rule_ms,
+                          c.candidates);
+            out += line;
+        }
+        if (len) *len = (int)out.size();
+        if (buf && cap > 0) {
+            const int n = std::min<int>(cap - 1, (int)out.size());
+            std::memcpy(buf, out.data(), n);
+            buf[n] = 0;
+        }
     });
 }
 

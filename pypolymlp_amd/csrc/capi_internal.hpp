@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <tuple>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -106,6 +107,16 @@ struct ModelHandle {
     // signature groups of the linear features (kernels_xgroup.hip)
     std::unique_ptr<GroupTables> group_tables;
     const GroupTables& groups();
+    // launch shapes of the signature-group kernel chosen by timing, per
+    // (precision, row bucket of the largest structure, structure-count bucket)
+    struct GroupChoice {
+        int TB = 0, A = 0, split = 1;
+        bool persistent = false;
+        double ms = 0.0;       // time of the chosen shape on the tuning structures
+        double rule_ms = 0.0;  // time of the rule-based shape on the same structures
+        int candidates = 0;
+    };
+    std::map<std::tuple<int, int, int>, GroupChoice> group_tuning;
 };
 
 struct BatchHandle {
@@ -123,6 +134,7 @@ struct BatchHandle {
     // cell-list search (single large structure)
     bool binned = false;
     BinGridHost grid{};
+    unsigned long long serial = 0;  // creation number (identifies the batch's rows)
     DevBuf d_tindex, d_bin, d_wrap, d_bin_start, d_bin_sorted;
     BatchHandle(ModelHandle* m, double cutoff, int n_st, const double* axes, const double* pos,
                 const int* types, const int* n_atoms);
@@ -145,8 +157,10 @@ XPlan plan_x(ModelHandle& m, int rows_max, bool fp64, int cover);
 // feature values per atom, pair records.
 struct XWork {
     DevBuf a, w, x, pd;
-    DevBuf wg, tpo;            // grouped W', per-atom type-pair ranges of the pair records
-    DevBuf xf, m, d, doff, z, msel;  // polynomial columns: factor values, monomials, D, Y, gathered monomials
+    DevBuf tpo;                // per-atom type-pair ranges of the pair records
+    DevBuf queue;              // work-queue counters of the persistent signature-group launches
+    DevBuf tune_x;             // scratch rows for timing signature-group launch shapes
+    DevBuf xf, m, d, doff, z;  // polynomial columns: factor values, monomials, D, entry products
 };
 
 // X rows of structures [s0, s1) into X (row-major, leading dimension ld),
@@ -176,6 +190,15 @@ struct AccumWork {
     XWork work;
 };
 
+// The weighted rows [X | y] of one sub-batch kept in device memory after
+// the accumulation, for a later residual pass over the same rows.
+struct KeptRows {
+    unsigned long long batch = 0, wy_hash = 0;  // batch serial, hash of the weights and targets
+    int s0 = 0, s1 = 0;
+    bool fp64 = false;
+    DevBuf x;
+};
+
 struct NormalHandle {
     ModelHandle* model = nullptr;
     int F = 0, F_pad = 0;
@@ -183,7 +206,11 @@ struct NormalHandle {
     DevBuf xtx;   // F_pad x F_pad Gram of [X | y], upper triangle accumulated
     DevBuf vec;   // [xe_sum | xe_sq_sum], each F_pad
     AccumWork ws;
+    size_t keep_budget = 0, kept_bytes = 0;  // plm_normal_keep_rows
+    std::vector<std::unique_ptr<KeptRows>> kept;
 };
+// FNV-1a over the row weights and weighted targets of n rows
+unsigned long long hash_rows(const double* w, const double* y, long n);
 
 // fp64: the X build in FP64 (else FP32); quant: the exact three-slice int8
 // Gram (else the FP64 Gram, which needs fp64)
@@ -207,9 +234,15 @@ struct AccumPlan {
 // quant: also the quantisation row maps (the int8 Gram's rows)
 void plan_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
                      bool fp64, bool quant, AccumPlan& plan);
-// launches only (no allocation beyond growing ws, no copy, no synchronisation)
+// launches only (no allocation beyond growing ws, no copy, no synchronisation);
+// with keep (eager accumulation only) each sub-batch's rows go to a buffer of
+// their own, kept on the accumulator while its budget allows
+struct KeepRequest {
+    unsigned long long batch, wy_hash;
+};
 void enqueue_accumulate(NormalHandle& acc, BatchHandle& b, const AccumPlan& plan, AccumWork& ws,
-                        double* gram, double* vec, hipStream_t st);
+                        double* gram, double* vec, hipStream_t st,
+                        const KeepRequest* keep = nullptr);
 
 // The fit step (one accumulation pass over a batch) captured as a HIP graph.
 struct NormalGraph {
@@ -234,6 +267,9 @@ void normal_graph_run(NormalGraph& g, int n_steps);
 // supplies the model and its work buffers; its sums are not touched.
 void normal_residual(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
                      bool fp64, const double* coeffs, int n_coef, double* xtr, double* sq);
+// keep the rows of later eager accumulations up to max_bytes (0: keep none,
+// release the kept rows)
+void normal_keep_rows(NormalHandle& acc, size_t max_bytes);
 
 void upload_potential(ModelHandle& m);
 

@@ -113,14 +113,19 @@ struct PolyDev {
     int E;                   // number of entries
     const int* ent_u;        // [E] monomial of each entry
     const int* col_pos;      // [n_cols][3] entry of each slot of a column, -1 if none
+    const int4* col_pos4;    // [n_cols] the same, one 16-byte record per column (w = -1)
+    // tiles of the entry products: (factor, first entry, end entry), at most
+    // 128 entries of one factor each
+    int n_ztiles;
+    const int* ztiles;       // [n_ztiles][3]
 };
 
 // Signature groups of the linear features: features of one atom type whose
 // derivative weights W' use the same Y keys for every type pair (they differ
 // in the radial index and the coefficients). A task is up to kGroupSlots such
-// features (slots); tasks are sorted by atom type. The atom's W' entries of a
-// task and type pair are stored [key][slot] (re, im), so one key's weights of
-// all slots are contiguous.
+// features (slots); tasks are sorted by atom type. The grouped positions of a
+// task and type pair run [key][slot]; wg_src maps each position to the atom's
+// W' nonzero stored there in the feature-major W' of the type (-1: none).
 constexpr int kGroupSlots = 16;
 struct GroupDev {
     int n_tasks;
@@ -130,14 +135,11 @@ struct GroupDev {
     const int* tt_nkey;     // [n_tasks][ntp] Y keys of the task for the type pair
     const int* tt_keyoff;   // [n_tasks][ntp] offset into keys
     const int* keys;        // Y keys (-1: the constant Y_00 folded into W')
-    const int* tt_woff;     // [n_tasks][ntp] complex offset of the task's W' block per atom
+    const int* tt_woff;     // [n_tasks][ntp] first grouped position of the task and type pair
     const int* tt_nid;      // [n_tasks][ntp][16] radial index of each slot, -1 when empty
     const int* g2l;         // [n_type][n_lin] local linear feature of a global column, -1 if none
-    const int* wg_pos;      // per type, concatenated: grouped position of each W' nonzero
-    const int* wg_pos_off;  // [n_type + 1]
-    const int* wg_hole;     // per type, concatenated: grouped positions without a nonzero
-    const int* wg_hole_off; // [n_type + 1]
-    int wg_stride;          // complex entries of the grouped W' per atom (max over types)
+    const int* wg_src;      // [n_type][wg_stride] W' nonzero at each grouped position, -1: none
+    int wg_stride;          // grouped positions per type (max over types)
 };
 
 // A batch of structures resident in device memory.

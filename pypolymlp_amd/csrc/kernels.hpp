@@ -80,31 +80,38 @@ void launch_pairdata(hipStream_t st, bool fp64, const ModelDev& md, const BatchD
 void launch_anlm_rec(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd, int atom0,
                      int n_atoms, long pair_base, const void* pd, const PairRec& R,
                      const int* tpo, void* a_full, int a_stride, int max_noconj);
-// With wg given, W' is also written in the signature-group layout of gd
-// (wg[(i - atom0) * gd.wg_stride + position], holes zero).
 void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,
                    const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
-                   void* x, int x_stride, int max_nlm, int w_maxf, int v_maxf,
-                   const GroupDev* gd = nullptr, void* wg = nullptr);
+                   void* x, int x_stride, int max_nlm, int w_maxf, int v_maxf);
 
 // Linear columns by signature groups (kernels_xgroup.hip).
 struct XGroupParams {
     int TB;         // tasks (column groups of NSW slots) per workgroup
     int A;          // waves per task: atom streams
     int vt0, vt1;   // virtual task range of the launch (one atom type)
-    int n_blocks;   // workgroups per structure
+    int n_blocks;   // task blocks per structure
     int ld;         // leading dimension of X
     int st0, st_end;
     int row_base;   // first X row of the structure range
-    int xcd_map;    // 1: all blocks of a structure on one XCD
+    int xcd_map;    // 1: all blocks of a structure on one XCD (static grid)
     int atom_base;  // first atom of the work buffers
     long pair_base;
+    int split;      // slot parts per task (1, 2, 4; > 1 needs TB = 1): the row
+                    // tile of a part holds NSW / split slot columns, so that the
+                    // rows of large structures fit LDS
+    int* queue;     // persistent launch: 8 item counters, zero before the
+                    // launch (one queue per XCD); null: one workgroup per item
+    int grid;       // persistent launch: workgroups
+    int w_stride;   // complex W' entries per atom (launch_wprime's w_stride)
 };
 int xgroup_slots(bool fp64);      // slots per wave (NSW): 16 in FP32, 8 in FP64
 int xgroup_max_waves(bool fp64);  // waves per workgroup the kernel is built for
 size_t xgroup_smem(const XGroupParams& P, int rows_max, bool fp64);
+// resident workgroups of a persistent launch with P's shape (all CUs)
+int xgroup_persistent_grid(const XGroupParams& P, int rows_max, bool fp64);
+// w: the atoms' feature-major W' (launch_wprime), read through gd.wg_src
 void launch_xgroup(hipStream_t st, bool fp64, const GroupDev& gd, const BatchDev& bd,
-                   const XGroupParams& P, int rows_max, const void* wg, const void* pd,
+                   const XGroupParams& P, int rows_max, const void* w, const void* pd,
                    const PairRec& R, const int* tpo, void* X, const double* roww);
 // energy rows of the linear columns [0, n_lin) of structures [st0, st0 + n_st)
 // from the feature values x (wprime), with the scale sums of the raw rows
@@ -162,13 +169,18 @@ void launch_poly_deriv(hipStream_t st, bool fp64, const ModelDev& md, const Poly
                        const BatchDev& bd, int atom0, int n_atoms, long pair_base,
                        const void* prec, const PairRec& R, const void* wall, int w_stride, int k0,
                        int kc, int rd_max, void* D, int st0, const long* d_off);
-void launch_poly_msel(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,
-                      int s_first, int n_st, int atom_base, int n_max, const void* M, void* Ms,
-                      long ms_stride);
-void launch_poly_gather(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,
-                        int s_first, int n_st, int atom_base, const void* Z, long z_stride,
-                        const void* XF, void* X, int ld, int row_base, const double* roww,
-                        double* xe_sum, double* xe_sq, int rd_max);
+// the polynomial columns of the n_st equal-size structures from s_first (rd
+// rows each) of the sub-batch starting at structure s0 / atom atom_base
+// (kernels_polyz.hip): the entry products Z, then the force / stress rows from
+// Z and the energy rows from the factor values XF
+void launch_poly_z(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd, int s_first,
+                   int n_st, int s0, int atom_base, int rd, const void* D, const long* d_off,
+                   const void* M, void* Z, long z_stride);
+void launch_poly_rows(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,
+                      int s_first, int n_st, int atom_base, int rd, const void* XF, const void* Z,
+                      long z_stride, void* X, int ld, int row_base, const double* roww,
+                      double* xe_sum, double* xe_sq);
+
 
 // Surrogate network of the hyperparameter search (FP32 MLP 5-64-64-2, SiLU).
 constexpr int kSurrIn = 5, kSurrHidden = 64, kSurrOut = 2;

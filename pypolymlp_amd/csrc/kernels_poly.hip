@@ -1,5 +1,5 @@
 // Polynomial columns of X (products of 2 or 3 linear features, the
-// "factors") as one GEMM per structure instead of per-pair column work.
+// "factors"), first half: factor values, monomials and factor derivatives.
 //
 // For a column c = prod_s x_{f_s} (Model::model_order2/3, model.cpp:207-267):
 //   force/stress rows:  X[r, c] = sum_i sum_s D_{f_s}[r, i] * m_{c\s}(i)
@@ -7,10 +7,8 @@
 // where D_f[r, i] = d x_f(i) / d xi_r is the derivative of factor f at atom i
 // (rows r: 6 stress, then 3N forces) and m_{c\s}(i) the product of the other
 // factors at atom i. Each (slot s, column) pair is an entry e = (f, u) with
-// f the slot's factor and u the monomial of the others; entries are sorted by
-// slot and factor, and Y[r][e] = sum_i D[i][f][r] M[i][u] is one GEMM per
-// (slot, factor) over that factor's entries (B: the gathered monomials).
-// Every column is then the sum of its nfac entries of Y.
+// f the slot's factor and u the monomial of the others; the products of the
+// entries and the column sums are in kernels_polyz.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -130,66 +128,6 @@ __global__ void __launch_bounds__(128) poly_deriv_kernel(ModelDev md, PolyDev pd
     for (int e = threadIdx.x; e < kc * RD; e += blockDim.x) out[e] = acc[e];
 }
 
-// Gathered monomials of one GEMM group: Ms[s][i][e] = M[i][u_e] over the slot
-// entries e, the B operand of the per-factor products.
-template <class T>
-__global__ void poly_msel_kernel(PolyDev pd, BatchDev bd, int s_first, int n_st, int atom_base,
-                                 const T* __restrict__ M, T* __restrict__ Ms, long ms_stride) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    const int s = s_first + blockIdx.z;
-    if (e >= pd.E || blockIdx.z >= n_st) return;
-    const int a0 = bd.st_atom_off[s];
-    if (i >= bd.st_atom_off[s + 1] - a0) return;
-    Ms[(size_t)blockIdx.z * ms_stride + (size_t)i * pd.E + e] =
-        M[(size_t)(a0 + i - atom_base) * pd.U + pd.ent_u[e]];
-}
-
-// X[row][c0 + c] for the structures of one GEMM group: force/stress rows as
-// the sum of the column's slot entries of Y[s][r][e], the energy row from the
-// factor values; rows weighted, raw energy rows into the scale sums.
-template <class T>
-__global__ void poly_gather_kernel(PolyDev pd, BatchDev bd, int s_first, int n_st, int atom_base,
-                                   const T* __restrict__ Y, long y_stride,
-                                   const T* __restrict__ XF, T* __restrict__ X, int ld,
-                                   int row_base, const double* __restrict__ roww,
-                                   double* __restrict__ xe_sum, double* __restrict__ xe_sq) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = s_first + blockIdx.z;
-    if (c >= pd.n_cols || blockIdx.z >= n_st) return;
-    const int a0 = bd.st_atom_off[s];
-    const int nat = bd.st_atom_off[s + 1] - a0;
-    const int RD = 6 + 3 * nat;
-    const int rb = bd.st_row_off[s] - row_base;
-    const int* ph = pd.col_phi + 3 * (size_t)c;
-    const int* pos = pd.col_pos + 3 * (size_t)c;
-    const int nf = ph[2] >= 0 ? 3 : 2;
-    const T* Ys = Y + (size_t)(s - s_first) * y_stride;
-    const int E = pd.E;
-    for (int r = blockIdx.y; r < RD; r += gridDim.y) {
-        const T* yr = Ys + (size_t)r * E;
-        T v = yr[pos[0]] + yr[pos[1]];
-        if (nf == 3) v += yr[pos[2]];
-        const double w = roww ? roww[rb + 1 + r] : 1.0;
-        X[(size_t)(rb + 1 + r) * ld + pd.c0 + c] = T(w * double(v));
-    }
-    if (blockIdx.y == 0) {
-        T e = T(0);
-        for (int i = 0; i < nat; ++i) {
-            const T* xf = XF + (size_t)(a0 + i - atom_base) * pd.K;
-            T p = xf[ph[0]] * xf[ph[1]];
-            if (nf == 3) p *= xf[ph[2]];
-            e += p;
-        }
-        const double w = roww ? roww[rb] : 1.0;
-        X[(size_t)rb * ld + pd.c0 + c] = T(w * double(e));
-        if (xe_sum) {
-            atomicAdd(xe_sum + pd.c0 + c, double(e));
-            atomicAdd(xe_sq + pd.c0 + c, double(e) * double(e));
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -227,39 +165,6 @@ void launch_poly_deriv(hipStream_t st, bool fp64, const ModelDev& md, const Poly
             reinterpret_cast<const Cpx<float>*>(wall), w_stride, k0, kc,
             reinterpret_cast<float*>(D), st0, d_off);
     }
-}
-
-void launch_poly_msel(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,
-                      int s_first, int n_st, int atom_base, int n_max, const void* M, void* Ms,
-                      long ms_stride) {
-    const int B = 256;
-    dim3 grid((pd.E + B - 1) / B, n_max, n_st);
-    if (fp64)
-        poly_msel_kernel<double><<<grid, B, 0, st>>>(pd, bd, s_first, n_st, atom_base,
-                                                     reinterpret_cast<const double*>(M),
-                                                     reinterpret_cast<double*>(Ms), ms_stride);
-    else
-        poly_msel_kernel<float><<<grid, B, 0, st>>>(pd, bd, s_first, n_st, atom_base,
-                                                    reinterpret_cast<const float*>(M),
-                                                    reinterpret_cast<float*>(Ms), ms_stride);
-}
-
-void launch_poly_gather(hipStream_t st, bool fp64, const PolyDev& pd, const BatchDev& bd,
-                        int s_first, int n_st, int atom_base, const void* Z, long z_stride,
-                        const void* XF, void* X, int ld, int row_base, const double* roww,
-                        double* xe_sum, double* xe_sq, int rd_max) {
-    const int B = 256;
-    dim3 grid((pd.n_cols + B - 1) / B, std::min(rd_max, 64), n_st);
-    if (fp64)
-        poly_gather_kernel<double><<<grid, B, 0, st>>>(
-            pd, bd, s_first, n_st, atom_base, reinterpret_cast<const double*>(Z), z_stride,
-            reinterpret_cast<const double*>(XF), reinterpret_cast<double*>(X), ld, row_base, roww,
-            xe_sum, xe_sq);
-    else
-        poly_gather_kernel<float><<<grid, B, 0, st>>>(
-            pd, bd, s_first, n_st, atom_base, reinterpret_cast<const float*>(Z), z_stride,
-            reinterpret_cast<const float*>(XF), reinterpret_cast<float*>(X), ld, row_base, roww,
-            xe_sum, xe_sq);
 }
 
 }  // namespace plm

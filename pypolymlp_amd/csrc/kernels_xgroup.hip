@@ -28,11 +28,18 @@
 //   PyModel row layout (energy, 6 stress, 3N forces)  src/cxx/src/compute/py_model.cpp:58-106
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <tuple>
+
 #include "device_tables.hpp"
 #include "kernels.hpp"
 #include "pair_data.hpp"
 
 namespace plm {
+
+void check_hip(hipError_t e, const char* what);
 
 typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
 typedef double f64x4 __attribute__((__vector_size__(4 * sizeof(double))));
@@ -84,7 +91,7 @@ template <class T, int KS>
 __device__ __forceinline__ void group_pairs(const T* __restrict__ pd, long pair_base, int stride,
                                             int nfn, int pc, int pe, const GroupKeys<T>& K,
                                             int lane, int nid, double* tile, int ldt, int col,
-                                            T (&self)[3], T (&st)[6]) {
+                                            bool keep, T (&self)[3], T (&st)[6]) {
     using M = GroupMfma<T>;
     const int i = lane & 15;
     const bool const_one = M::a_comp(i) == 0 && ((lane >> 4) & 1) == 0;
@@ -119,7 +126,7 @@ __device__ __forceinline__ void group_pairs(const T* __restrict__ pd, long pair_
         const T qx = g * dx[m] + f * acc[1];
         const T qy = g * dy[m] + f * acc[2];
         const T qz = g * dz[m] + f * acc[3];
-        if (ok) {
+        if (ok && keep) {
             double* rj = tile + (size_t)(6 + 3 * int(jl[m])) * ldt + col;
             atomicAdd(rj, -double(qx));
             atomicAdd(rj + ldt, -double(qy));
@@ -131,27 +138,20 @@ __device__ __forceinline__ void group_pairs(const T* __restrict__ pd, long pair_
     }
 }
 
+// One item: structure s, task block blk (TB tasks), slot part `part` (of
+// P.split). The tile holds the item's stress and force rows for its TC
+// columns; the slot lanes outside the part compute and drop their sums.
 template <class T>
-__global__ void __launch_bounds__(64 * GroupMfma<T>::waves) xgroup_kernel(
-    GroupDev gd, BatchDev bd, XGroupParams P, const T* __restrict__ wg, const T* __restrict__ pd,
-    PairRec R, const int* __restrict__ tpo, T* __restrict__ X, const double* __restrict__ roww) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    double* tile = reinterpret_cast<double*>(smem_raw);
-    int blk, s;
-    if (P.xcd_map) {
-        // the blocks of one structure run on one XCD (workgroups are dealt to
-        // the 8 XCDs round robin), so its pair records are shared in one L2
-        const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
-        s = P.st0 + (k / P.n_blocks) * 8 + xcd;
-        blk = k % P.n_blocks;
-        if (s >= P.st_end) return;
-    } else {
-        blk = blockIdx.x % P.n_blocks;
-        s = P.st0 + blockIdx.x / P.n_blocks;
-    }
+__device__ __forceinline__ void xgroup_item(const GroupDev& gd, const BatchDev& bd,
+                                            const XGroupParams& P, const T* __restrict__ wp,
+                                            const T* __restrict__ pd, const PairRec& R,
+                                            const int* __restrict__ tpo, T* __restrict__ X,
+                                            const double* __restrict__ roww, double* tile, int s,
+                                            int blk, int part) {
     const int a0 = bd.st_atom_off[s], nat = bd.st_atom_off[s + 1] - a0;
     const int rows = 6 + 3 * nat;  // stress rows, then 3 force rows per atom
-    const int TC = P.TB * kGroupSlots, ldt = TC + 1;
+    const int SPP = kGroupSlots / P.split;  // slots per part
+    const int TC = P.TB * SPP, ldt = TC + 1;
     const int tid = threadIdx.x;
     for (int e = tid; e < rows * ldt; e += blockDim.x) tile[e] = 0.0;
     __syncthreads();
@@ -166,9 +166,10 @@ __global__ void __launch_bounds__(64 * GroupMfma<T>::waves) xgroup_kernel(
     const int kq = lane >> 4;                 // k index of this lane in the A / B operands
     const int kh = kq >> 1, reim = kq & 1;    // key of the step, real / imaginary part
     const int ac = GroupMfma<T>::a_comp(lane & 15);
+    const bool keep = j / SPP == part;        // this lane's slot is in the item's part
     if (task < P.vt1 && stream < P.A) {
         const int t = gd.task_type[task];
-        const int col = tw * kGroupSlots + j;
+        const int col = tw * SPP + j - part * SPP;
         // per type pair tables of the task, one type pair per lane: read with
         // readlane, so the atom loop waits on no scalar load (scalar loads
         // share their counter with the LDS atomics)
@@ -186,7 +187,7 @@ __global__ void __launch_bounds__(64 * GroupMfma<T>::waves) xgroup_kernel(
                 if (nth++ % P.A != stream) continue;
                 const int i = a0 + ia;
                 T self[3] = {T(0), T(0), T(0)};
-                const T* wi = wg + (size_t)(i - P.atom_base) * gd.wg_stride * 2;
+                const T* wi = wp + (size_t)(i - P.atom_base) * P.w_stride * 2;  // (re, im)
                 const int v_to = tpo[(size_t)(i - P.atom_base) * (ntp + 1) + min(lane, ntp)];
                 for (int tp = 0; tp < ntp; ++tp) {
                     const int nkey = __builtin_amdgcn_readlane(v_nkey, tp);
@@ -194,7 +195,8 @@ __global__ void __launch_bounds__(64 * GroupMfma<T>::waves) xgroup_kernel(
                     const int pe = __builtin_amdgcn_readlane(v_to, tp + 1);
                     if (nkey == 0 || pb >= pe) continue;
                     const int* keys = gd.keys + __builtin_amdgcn_readlane(v_koff, tp);
-                    const T* W = wi + (size_t)__builtin_amdgcn_readlane(v_woff, tp) * 2;
+                    const int* src = gd.wg_src + (size_t)t * gd.wg_stride +
+                                     __builtin_amdgcn_readlane(v_woff, tp);
                     const int nid = gd.tt_nid[(size_t)(task * ntp + tp) * kGroupSlots + j];
                     const int ks = (nkey + 1) >> 1;
                     for (int s0 = 0; s0 < ks; s0 += kGroupKSteps) {
@@ -206,8 +208,10 @@ __global__ void __launch_bounds__(64 * GroupMfma<T>::waves) xgroup_kernel(
                             const int key_i = 2 * (s0 + q) + kh;
                             const bool real = key_i < nkey;
                             const int kc = real ? key_i : 0;
-                            const T w = W[((size_t)kc * kGroupSlots + j) * 2 + reim];
-                            K.b[q] = real ? (reim ? -w : w) : T(0);
+                            // the W' nonzero at grouped position (key, slot), if any
+                            const int g = real ? src[kc * kGroupSlots + j] : -1;
+                            const T w = wi[(size_t)max(g, 0) * 2 + reim];
+                            K.b[q] = g >= 0 ? (reim ? -w : w) : T(0);
                             const int key = real ? keys[kc] : -2;
                             K.off[q] = key >= 0 ? R.ylm(key) + 2 * ac + reim : key;
                         }
@@ -217,7 +221,7 @@ __global__ void __launch_bounds__(64 * GroupMfma<T>::waves) xgroup_kernel(
 #define PLM_KS(N)                                                                               \
     case N:                                                                                     \
         group_pairs<T, N>(pd, P.pair_base, R.stride, R.nfn, pc, pe, K, lane, nid, tile, ldt, \
-                          col, self, st);                                                       \
+                          col, keep, self, st);                                                 \
         break;
                                 PLM_KS(1) PLM_KS(2) PLM_KS(3) PLM_KS(4) PLM_KS(5) PLM_KS(6)
 #undef PLM_KS
@@ -227,28 +231,93 @@ __global__ void __launch_bounds__(64 * GroupMfma<T>::waves) xgroup_kernel(
                     }
                 }
                 // rows of atom i: the four lanes of a slot add their sums
-                double* ri = tile + (size_t)(6 + 3 * ia) * ldt + col;
-                atomicAdd(ri, double(self[0]));
-                atomicAdd(ri + ldt, double(self[1]));
-                atomicAdd(ri + 2 * ldt, double(self[2]));
+                if (keep) {
+                    double* ri = tile + (size_t)(6 + 3 * ia) * ldt + col;
+                    atomicAdd(ri, double(self[0]));
+                    atomicAdd(ri + ldt, double(self[1]));
+                    atomicAdd(ri + 2 * ldt, double(self[2]));
+                }
             }
         }
         // stress rows (xx, yy, zz, xy, yz, zx)
+        if (keep) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) atomicAdd(tile + (size_t)c * ldt + col, double(st[c]));
+            for (int c = 0; c < 6; ++c) atomicAdd(tile + (size_t)c * ldt + col, double(st[c]));
+        }
     }
     __syncthreads();
     // add into X (weights applied per row); tile row r is X row 1 + r
     const int rbase = bd.st_row_off[s] - P.row_base;
     for (int e = tid; e < rows * TC; e += blockDim.x) {
         const int r = e / TC, cc = e % TC;
-        const int vv = P.vt0 + blk * P.TB + cc / kGroupSlots;
+        const int vv = P.vt0 + blk * P.TB + cc / SPP;
         if (vv >= P.vt1) continue;
-        const int g = gd.task_gcol[vv * kGroupSlots + cc % kGroupSlots];
+        const int g = gd.task_gcol[vv * kGroupSlots + part * SPP + cc % SPP];
         if (g < 0) continue;
         const double w = roww ? roww[rbase + 1 + r] : 1.0;
         T* dst = X + (size_t)(rbase + 1 + r) * P.ld + g;
         *dst += T(w * tile[(size_t)r * ldt + cc]);
+    }
+    __syncthreads();  // the tile is reused by the workgroup's next item
+}
+
+// Static grid: one workgroup per (structure, task block, part).
+template <class T>
+__global__ void __launch_bounds__(64 * GroupMfma<T>::waves) xgroup_kernel(
+    GroupDev gd, BatchDev bd, XGroupParams P, const T* __restrict__ wp, const T* __restrict__ pd,
+    PairRec R, const int* __restrict__ tpo, T* __restrict__ X, const double* __restrict__ roww) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    double* tile = reinterpret_cast<double*>(smem_raw);
+    const int nbp = P.n_blocks * P.split;  // items per structure
+    int item, s;
+    if (P.xcd_map) {
+        // the blocks of one structure run on one XCD (workgroups are dealt to
+        // the 8 XCDs round robin), so its pair records are shared in one L2
+        const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+        s = P.st0 + (k / nbp) * 8 + xcd;
+        item = k % nbp;
+        if (s >= P.st_end) return;
+    } else {
+        item = blockIdx.x % nbp;
+        s = P.st0 + blockIdx.x / nbp;
+    }
+    xgroup_item<T>(gd, bd, P, wp, pd, R, tpo, X, roww, tile, s, item / P.split, item % P.split);
+}
+
+// Persistent grid with a work queue: the structures of the launch are dealt
+// to 8 queues (structure k to queue k mod 8), each queue hands out its items
+// (task block, part) largest structure first (the structures of a batch are
+// in ascending size order), and a workgroup drains the queue of its own XCD
+// first, so that one structure's items mostly run on one XCD and share its
+// pair records in that L2, then takes items from the other queues until all
+// are empty. Uneven structure sizes balance over the resident workgroups;
+// every workgroup exits when the eight queues are drained.
+template <class T>
+__global__ void __launch_bounds__(64 * GroupMfma<T>::waves) xgroup_queue_kernel(
+    GroupDev gd, BatchDev bd, XGroupParams P, const T* __restrict__ wp, const T* __restrict__ pd,
+    PairRec R, const int* __restrict__ tpo, T* __restrict__ X, const double* __restrict__ roww) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    __shared__ int next;
+    double* tile = reinterpret_cast<double*>(smem_raw);
+    const int nbp = P.n_blocks * P.split;
+    const int n_st = P.st_end - P.st0;
+    const int xcd = blockIdx.x & 7;
+    for (int qq = 0; qq < 8; ++qq) {
+        const int q = (xcd + qq) & 7;
+        const int nsq = q < n_st ? (n_st - q + 7) / 8 : 0;  // structures of queue q
+        const int n_items = nsq * nbp;
+        for (;;) {
+            if (threadIdx.x == 0) next = atomicAdd(P.queue + q, 1);
+            __syncthreads();
+            // uniform: kept in scalar registers
+            const int k = __builtin_amdgcn_readfirstlane(next);
+            __syncthreads();  // read by every thread before thread 0 writes again
+            if (k >= n_items) break;
+            const int m = nsq - 1 - k / nbp;  // largest structure of the queue first
+            const int item = k % nbp;
+            xgroup_item<T>(gd, bd, P, wp, pd, R, tpo, X, roww, tile, P.st0 + q + 8 * m,
+                           item / P.split, item % P.split);
+        }
     }
 }
 
@@ -288,32 +357,77 @@ int xgroup_max_waves(bool fp64) {
 
 size_t xgroup_smem(const XGroupParams& P, int rows_max, bool fp64) {
     const int rows = rows_max - 1;  // the energy row is not in the tile
-    return (size_t)rows * (P.TB * xgroup_slots(fp64) + 1) * sizeof(double);
+    return (size_t)rows * (P.TB * xgroup_slots(fp64) / P.split + 1) * sizeof(double);
+}
+
+template <class T>
+static const void* xgroup_fn(bool persistent) {
+    return persistent ? reinterpret_cast<const void*>(&xgroup_queue_kernel<T>)
+                      : reinterpret_cast<const void*>(&xgroup_kernel<T>);
+}
+
+int xgroup_persistent_grid(const XGroupParams& P, int rows_max, bool fp64) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, size_t>, int> cache;
+    const size_t sm = xgroup_smem(P, rows_max, fp64);
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "device");
+    const auto key = std::make_tuple(dev * 2 + (fp64 ? 1 : 0), 64 * P.TB * P.A, sm);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int n_cu = 0;
+    check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev),
+              "device attribute");
+    const void* fn = fp64 ? xgroup_fn<double>(true) : xgroup_fn<float>(true);
+    if (sm > 160 * 1024 - 1024) throw std::invalid_argument("xgroup row tile exceeds LDS");
+    check_hip(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm),
+              "xgroup LDS attribute");
+    int per_cu = 0;
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * P.TB * P.A, sm),
+              "occupancy");
+    const int g = std::max(1, per_cu) * std::max(1, n_cu);
+    cache[key] = g;
+    return g;
 }
 
 template <class T>
 static void launch_xgroup_t(hipStream_t st, const GroupDev& gd, const BatchDev& bd,
-                            const XGroupParams& P, int rows_max, const void* wg, const void* pd,
+                            const XGroupParams& P, int rows_max, const void* wp, const void* pd,
                             const PairRec& R, const int* tpo, void* X, const double* roww) {
     const size_t sm = xgroup_smem(P, rows_max, sizeof(T) == 8);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&xgroup_kernel<T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    const bool persistent = P.queue != nullptr;
+    check_hip(hipFuncSetAttribute(xgroup_fn<T>(persistent),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm),
+              "xgroup LDS attribute");
     const int n_st = P.st_end - P.st0;
-    const long grid = P.xcd_map ? (long)((n_st + 7) / 8) * 8 * P.n_blocks : (long)n_st * P.n_blocks;
+    const long nbp = (long)P.n_blocks * P.split;
+    if (persistent) {
+        xgroup_queue_kernel<T><<<P.grid, 64 * P.TB * P.A, sm, st>>>(
+            gd, bd, P, reinterpret_cast<const T*>(wp), reinterpret_cast<const T*>(pd), R, tpo,
+            reinterpret_cast<T*>(X), roww);
+        return;
+    }
+    const long grid = P.xcd_map ? (long)((n_st + 7) / 8) * 8 * nbp : (long)n_st * nbp;
     xgroup_kernel<T><<<grid, 64 * P.TB * P.A, sm, st>>>(
-        gd, bd, P, reinterpret_cast<const T*>(wg), reinterpret_cast<const T*>(pd), R, tpo,
+        gd, bd, P, reinterpret_cast<const T*>(wp), reinterpret_cast<const T*>(pd), R, tpo,
         reinterpret_cast<T*>(X), roww);
 }
 
 void launch_xgroup(hipStream_t st, bool fp64, const GroupDev& gd, const BatchDev& bd,
-                   const XGroupParams& P, int rows_max, const void* wg, const void* pd,
+                   const XGroupParams& P, int rows_max, const void* wp, const void* pd,
                    const PairRec& R, const int* tpo, void* X, const double* roww) {
     if (P.TB * P.A > xgroup_max_waves(fp64) || P.TB < 1 || P.A < 1)
         throw std::invalid_argument("xgroup launch shape");
+    if (!(P.split == 1 || ((P.split == 2 || P.split == 4) && P.TB == 1)))
+        throw std::invalid_argument("xgroup slot split");
+    if (P.queue && P.grid < 1) throw std::invalid_argument("xgroup persistent grid");
+    if (xgroup_smem(P, rows_max, fp64) > 160 * 1024 - 1024)
+        throw std::invalid_argument("xgroup row tile exceeds LDS");
     if (fp64)
-        launch_xgroup_t<double>(st, gd, bd, P, rows_max, wg, pd, R, tpo, X, roww);
+        launch_xgroup_t<double>(st, gd, bd, P, rows_max, wp, pd, R, tpo, X, roww);
     else
-        launch_xgroup_t<float>(st, gd, bd, P, rows_max, wg, pd, R, tpo, X, roww);
+        launch_xgroup_t<float>(st, gd, bd, P, rows_max, wp, pd, R, tpo, X, roww);
 }
 
 void launch_xenergy(hipStream_t st, bool fp64, const GroupDev& gd, const BatchDev& bd, int n_lin,

@@ -228,8 +228,7 @@ __global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
                                                      const Cpx<T>* __restrict__ a_full,
                                                      int a_stride, int atom0, int n_atoms,
                                                      Cpx<T>* __restrict__ w, int w_stride,
-                                                     T* __restrict__ x, int x_stride, GroupDev gd,
-                                                     Cpx<T>* __restrict__ wg) {
+                                                     T* __restrict__ x, int x_stride) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     Cpx<T>* a = reinterpret_cast<Cpx<T>*>(smem_raw);
     const int ia = blockIdx.x;
@@ -257,16 +256,9 @@ __global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
                 vr += cc[2 * r] * pr.re;
                 vi += cc[2 * r + 1] * pr.im;
             }
-            if (g >= 0) {
-                const Cpx<T> v = {vr, vi};
-                wo[g] = v;
-                if (wg) wg[(size_t)ia * gd.wg_stride + gd.wg_pos[gd.wg_pos_off[ti] + g]] = v;
-            }
+            if (g >= 0) wo[g] = Cpx<T>{vr, vi};
         }
     }
-    if (wg)
-        for (int e = gd.wg_hole_off[ti] + threadIdx.x; e < gd.wg_hole_off[ti + 1]; e += blockDim.x)
-            wg[(size_t)ia * gd.wg_stride + gd.wg_hole[e]] = Cpx<T>{T(0), T(0)};
     // feature values, same layout; products with |Re| <= 1e-20 dropped where
     // the reference drops them
     T* xo = x + (size_t)ia * x_stride;
@@ -642,29 +634,24 @@ int xrows_max_waves(bool fp64) { return fp64 ? 8 : 16; }
 template <class T, int NF, int NV>
 static void launch_wprime_t(hipStream_t st, const ModelDev& md, const BatchDev& bd,
                             const void* a_full, int a_stride, int atom0, int n_atoms, void* w,
-                            int w_stride, void* x, int x_stride, int max_nlm, const GroupDev& g,
-                            void* wg) {
+                            int w_stride, void* x, int x_stride, int max_nlm) {
     const size_t sm = (size_t)(max_nlm + 1) * 2 * sizeof(T);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&wprime_kernel<T, NF, NV>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     wprime_kernel<T, NF, NV><<<n_atoms, 256, sm, st>>>(
         md, bd, reinterpret_cast<const Cpx<T>*>(a_full), a_stride, atom0, n_atoms,
-        reinterpret_cast<Cpx<T>*>(w), w_stride, reinterpret_cast<T*>(x), x_stride, g,
-        reinterpret_cast<Cpx<T>*>(wg));
+        reinterpret_cast<Cpx<T>*>(w), w_stride, reinterpret_cast<T*>(x), x_stride);
 }
 
 void launch_wprime(hipStream_t st, bool fp64, const ModelDev& md, const BatchDev& bd,
                    const void* a_full, int a_stride, int atom0, int n_atoms, void* w, int w_stride,
-                   void* x, int x_stride, int max_nlm, int w_maxf, int v_maxf, const GroupDev* gd,
-                   void* wg) {
-    const GroupDev g = gd ? *gd : GroupDev{};
-    if (!gd) wg = nullptr;
+                   void* x, int x_stride, int max_nlm, int w_maxf, int v_maxf) {
     if (w_maxf > 3 || v_maxf > 4) throw std::invalid_argument("invariant order above 4");
     // instantiations: W' terms with <= 2 or 3 factors, value terms with <= 3 or 4
     const int nf = w_maxf <= 2 ? 2 : 3, nv = v_maxf <= 3 ? 3 : 4;
 #define PLM_WP(T, NF, NV)                                                                       \
     launch_wprime_t<T, NF, NV>(st, md, bd, a_full, a_stride, atom0, n_atoms, w, w_stride, x, \
-                               x_stride, max_nlm, g, wg)
+                               x_stride, max_nlm)
     if (fp64) {
         if (nf == 2 && nv == 3) PLM_WP(double, 2, 3);
         else if (nf == 2) PLM_WP(double, 2, 4);

@@ -15,6 +15,7 @@
 // pass can run eagerly or be captured once as a HIP graph and replayed.
 #include <algorithm>
 #include <cstdlib>
+#include <memory>
 #include <cstring>
 #include <vector>
 
@@ -113,7 +114,7 @@ void plan_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, con
 // on a second stream while the next X is built was measured slower for the
 // mixed form: the two contend for the CUs' registers and LDS.)
 void enqueue_accumulate(NormalHandle& acc, BatchHandle& b, const AccumPlan& plan, AccumWork& ws,
-                        double* gram, double* vec, hipStream_t st) {
+                        double* gram, double* vec, hipStream_t st, const KeepRequest* keep) {
     ModelHandle& m = *acc.model;
     const bool fp64 = plan.fp64;
     const int elem = fp64 ? 8 : 4;
@@ -125,24 +126,33 @@ void enqueue_accumulate(NormalHandle& acc, BatchHandle& b, const AccumPlan& plan
     const double* y = (const double*)plan.y.ptr;
     for (const SubBatch& sb : plan.sb) {
         const long r0 = b.row_off[sb.s0];
-        ws.xbuf.alloc((size_t)sb.rows_pad * row_bytes);
+        const size_t xbytes = (size_t)sb.rows_pad * row_bytes;
+        std::unique_ptr<KeptRows> kr;
+        if (keep && acc.kept_bytes + xbytes <= acc.keep_budget) {
+            kr = std::make_unique<KeptRows>();
+            kr->batch = keep->batch, kr->wy_hash = keep->wy_hash;
+            kr->s0 = sb.s0, kr->s1 = sb.s1, kr->fp64 = fp64;
+            kr->x.alloc(xbytes);
+        } else {
+            ws.xbuf.alloc(xbytes);
+        }
+        void* xb = kr ? kr->x.ptr : ws.xbuf.ptr;
         // cleared by a kernel (a memset node would not replay correctly in a graph)
-        launch_fill_zero(st, (double*)ws.xbuf.ptr, (size_t)sb.rows_pad * row_bytes / 8);
-        build_x_rows(m, b, sb.s0, sb.s1, fp64, ws.xbuf.ptr, ld, w + r0, xe_sum, xe_sq, ws.work,
-                     true, (const long*)plan.doffs.ptr + sb.doff_off);
-        launch_set_column(st, fp64, ws.xbuf.ptr, ld, F, y + r0, (int)sb.rows);
+        launch_fill_zero(st, (double*)xb, xbytes / 8);
+        build_x_rows(m, b, sb.s0, sb.s1, fp64, xb, ld, w + r0, xe_sum, xe_sq, ws.work, true,
+                     (const long*)plan.doffs.ptr + sb.doff_off);
+        launch_set_column(st, fp64, xb, ld, F, y + r0, (int)sb.rows);
         check_hip(hipGetLastError(), "target column");
         if (!plan.quant) {
             ProfScope ps("gram", st);
-            launch_gram_f64(st, reinterpret_cast<const double*>(ws.xbuf.ptr), ld, (int)sb.rows_pad,
+            launch_gram_f64(st, reinterpret_cast<const double*>(xb), ld, (int)sb.rows_pad,
                             acc.F_pad, gram, acc.F_pad);
         } else {
             ws.qbuf.alloc(gram_quant_bytes((int)sb.rows_q, (int)sb.h0, ld));
             {
                 ProfScope ps("quantize", st);
-                launch_gram_quantize(st, fp64, ws.xbuf.ptr,
-                                     (const int*)plan.rmaps.ptr + sb.rmap_off, ld, (int)sb.rows_q,
-                                     (int)sb.h0, acc.F_pad, ws.qbuf.ptr);
+                launch_gram_quantize(st, fp64, xb, (const int*)plan.rmaps.ptr + sb.rmap_off, ld,
+                                     (int)sb.rows_q, (int)sb.h0, acc.F_pad, ws.qbuf.ptr);
             }
             check_hip(hipGetLastError(), "quantize launch");
             ProfScope ps("gram", st);
@@ -150,7 +160,22 @@ void enqueue_accumulate(NormalHandle& acc, BatchHandle& b, const AccumPlan& plan
                            acc.F_pad);
         }
         check_hip(hipGetLastError(), "gram launch");
+        if (kr) {
+            acc.kept_bytes += xbytes;
+            acc.kept.push_back(std::move(kr));
+        }
     }
+}
+
+unsigned long long hash_rows(const double* w, const double* y, long n) {
+    unsigned long long h = 1469598103934665603ull;
+    auto mix = [&](const double* p) {
+        const unsigned char* c = reinterpret_cast<const unsigned char*>(p);
+        for (size_t k = 0; k < (size_t)n * 8; ++k) h = (h ^ c[k]) * 1099511628211ull;
+    };
+    mix(w);
+    mix(y);
+    return h ^ (unsigned long long)n;
 }
 
 void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, const double* y_w,
@@ -159,9 +184,21 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
     hipStream_t st = device().stream;
     AccumPlan plan;
     plan_accumulate(acc, b, row_w, y_w, fp64, quant, plan);
-    enqueue_accumulate(acc, b, plan, acc.ws, (double*)acc.xtx.ptr, (double*)acc.vec.ptr, st);
+    KeepRequest kq{b.serial, 0};
+    if (acc.keep_budget > acc.kept_bytes) kq.wy_hash = hash_rows(row_w, y_w, plan.n_rows);
+    enqueue_accumulate(acc, b, plan, acc.ws, (double*)acc.xtx.ptr, (double*)acc.vec.ptr, st,
+                       acc.keep_budget > acc.kept_bytes ? &kq : nullptr);
     check_hip(hipStreamSynchronize(st), "sync");
     acc.n_rows += plan.n_rows;
+}
+
+void normal_keep_rows(NormalHandle& acc, size_t max_bytes) {
+    check_hip(hipStreamSynchronize(device().stream), "sync");
+    acc.keep_budget = max_bytes;
+    if (max_bytes == 0) {
+        acc.kept.clear();
+        acc.kept_bytes = 0;
+    }
 }
 
 NormalGraph::~NormalGraph() {
@@ -233,18 +270,28 @@ void normal_residual(NormalHandle& acc, BatchHandle& b, const double* row_w, con
     AccumWork& ws = acc.ws;
     const double* w = (const double*)plan.w.ptr;
     const double* y = (const double*)plan.y.ptr;
+    const unsigned long long wy = acc.kept.empty() ? 0 : hash_rows(row_w, y_w, plan.n_rows);
     for (const SubBatch& sb : plan.sb) {
         if (sb.rows == 0) continue;
         const long r0 = b.row_off[sb.s0];
-        ws.xbuf.alloc((size_t)sb.rows_pad * ld * elem);
-        launch_fill_zero(st, (double*)ws.xbuf.ptr, (size_t)sb.rows_pad * ld * elem / 8);
-        build_x_rows(m, b, sb.s0, sb.s1, fp64, ws.xbuf.ptr, ld, w + r0, nullptr, nullptr, ws.work,
-                     true, (const long*)plan.doffs.ptr + sb.doff_off);
-        launch_set_column(st, fp64, ws.xbuf.ptr, ld, F, y + r0, (int)sb.rows);
-        check_hip(hipGetLastError(), "target column");
+        // the rows as the accumulation built them, when kept; else built again
+        const void* xb = nullptr;
+        for (const auto& k : acc.kept)
+            if (k->batch == b.serial && k->wy_hash == wy && k->s0 == sb.s0 && k->s1 == sb.s1 &&
+                k->fp64 == fp64)
+                xb = k->x.ptr;
+        if (!xb) {
+            ws.xbuf.alloc((size_t)sb.rows_pad * ld * elem);
+            launch_fill_zero(st, (double*)ws.xbuf.ptr, (size_t)sb.rows_pad * ld * elem / 8);
+            build_x_rows(m, b, sb.s0, sb.s1, fp64, ws.xbuf.ptr, ld, w + r0, nullptr, nullptr,
+                         ws.work, true, (const long*)plan.doffs.ptr + sb.doff_off);
+            launch_set_column(st, fp64, ws.xbuf.ptr, ld, F, y + r0, (int)sb.rows);
+            check_hip(hipGetLastError(), "target column");
+            xb = ws.xbuf.ptr;
+        }
         ProfScope ps("residual", st);
-        launch_residual_pass(st, fp64, ws.xbuf.ptr, ld, (int)sb.rows, F, (const double*)C.ptr,
-                             n_coef, (double*)U.ptr, (double*)R.ptr, (double*)S.ptr);
+        launch_residual_pass(st, fp64, xb, ld, (int)sb.rows, F, (const double*)C.ptr, n_coef,
+                             (double*)U.ptr, (double*)R.ptr, (double*)S.ptr);
         check_hip(hipGetLastError(), "residual launch");
     }
     std::vector<double> hr(hc.size()), hs(n_coef);

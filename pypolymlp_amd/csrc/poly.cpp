@@ -1,11 +1,9 @@
-// Polynomial columns of X by factor-derivative GEMMs: host tables and the
+// Polynomial columns of X by factor-derivative products: host tables and the
 // per-sub-batch sequence (factor values and monomials, factor derivatives,
-// per group of equal-size structures the gathered monomials of the entries
-// and one rocBLAS strided-batched GEMM per factor over its entries, then the
-// sum of each column's entries into the X rows). The column order and the products follow
-// Model::model_polynomial / model_order{2,3} (src/cxx/src/compute/model.cpp:159-267).
-#include <rocblas/rocblas.h>
-
+// then per group of equal-size structures the entry products on MFMA and the
+// sum of each column's entries into the X rows, kernels_polyz.hip). The
+// column order and the products follow Model::model_polynomial /
+// model_order{2,3} (src/cxx/src/compute/model.cpp:159-267).
 #include <algorithm>
 #include <array>
 #include <map>
@@ -17,17 +15,6 @@
 #include "profile.hpp"
 
 namespace plm {
-
-static rocblas_handle blas_handle() {
-    static rocblas_handle h = nullptr;
-    if (!h) {
-        if (rocblas_create_handle(&h) != rocblas_status_success)
-            throw DeviceError("rocblas_create_handle failed");
-    }
-    if (rocblas_set_stream(h, device().stream) != rocblas_status_success)
-        throw DeviceError("rocblas_set_stream failed");
-    return h;
-}
 
 const PolyTables& ModelHandle::poly() {
     if (poly_tables) return *poly_tables;
@@ -92,7 +79,17 @@ const PolyTables& ModelHandle::poly() {
     for (size_t c = 0; c < cols.size(); ++c)
         for (int sl = 0; sl < 3; ++sl)
             if (cphi[3 * c + sl] >= 0) col_pos[3 * c + sl] = ent_id.at({cphi[3 * c + sl], cu[3 * c + sl]});
+    std::vector<int> zt;
+    for (int f = 0; f < K; ++f)
+        for (int e0 = pt->ent_off[f]; e0 < pt->ent_off[f + 1]; e0 += 128)
+            zt.insert(zt.end(), {f, e0, std::min(e0 + 128, pt->ent_off[f + 1])});
     PolyDev& d = pt->dev;
+    d.n_ztiles = (int)zt.size() / 3;
+    d.ztiles = pt->store.upload(zt.data(), zt.size() * 4);
+    std::vector<int> cp4(cols.size() * 4, -1);
+    for (size_t c = 0; c < cols.size(); ++c)
+        for (int sl = 0; sl < 3; ++sl) cp4[4 * c + sl] = col_pos[3 * c + sl];
+    d.col_pos4 = reinterpret_cast<const int4*>(pt->store.upload(cp4.data(), cp4.size() * 4));
     d.E = (int)ent_u.size();
     d.ent_u = pt->store.upload(ent_u.data(), ent_u.size() * 4);
     d.col_pos = pt->store.upload(col_pos.data(), col_pos.size() * 4);
@@ -176,63 +173,32 @@ void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, 
                               work.d.ptr, s0, doff_dev);
     }
     check_hip(hipGetLastError(), "poly launch");
-    rocblas_handle h = blas_handle();
-    const size_t z_budget = 8ull << 30;  // Y and gathered monomials of one GEMM group
-    const int E = pd.E, K = pd.K;
+    // per group of equal-size structures: the entry products Z[s][r][e] (at
+    // most z_budget bytes), then the rows of the polynomial columns from them
+    const size_t z_budget = 8ull << 30;
+    const int E = pd.E;
     int s = s0;
     while (s < s1) {
         const int N = b.atom_off[s + 1] - b.atom_off[s];
         const long RD = 6 + 3 * N;
-        const long KR = (long)K * RD;
-        // Y[r][e] and the gathered monomials Ms[i][e] of one structure
-        const size_t y_one = (size_t)RD * E, ms_one = (size_t)N * E;
-        const size_t one = (y_one + ms_one) * elem;
+        const size_t y_one = (size_t)RD * E;
         int g = 1;
         while (s + g < s1 && b.atom_off[s + g + 1] - b.atom_off[s + g] == N &&
-               (size_t)(g + 1) * one <= z_budget)
+               (size_t)(g + 1) * y_one * elem <= z_budget)
             ++g;
         work.z.alloc(y_one * g * elem);
-        work.msel.alloc(ms_one * g * elem);
-        const size_t boff = (size_t)doff[s - s0];
         {
-            ProfScope ps("poly_gemm", st);
-            launch_poly_msel(st, fp64, pd, bd, s, g, a0, N, work.m.ptr, work.msel.ptr, (long)ms_one);
-            check_hip(hipGetLastError(), "poly monomial gather");
-            // one product per factor: Y[:, e range] = D_f (RD x N) * Ms[:, e range]
-            for (int f = 0; f < K; ++f) {
-                const int e0 = P.ent_off[f];
-                const int m_e = P.ent_off[f + 1] - e0;
-                if (m_e == 0) continue;
-                rocblas_status rs;
-                if (fp64) {
-                    const double one_d = 1.0, zero_d = 0.0;
-                    rs = rocblas_dgemm_strided_batched(
-                        h, rocblas_operation_none, rocblas_operation_transpose, m_e, (int)RD, N,
-                        &one_d, reinterpret_cast<const double*>(work.msel.ptr) + e0, E,
-                        (long)ms_one,
-                        reinterpret_cast<const double*>(work.d.ptr) + boff + (size_t)f * RD,
-                        (int)KR, (long)N * KR, &zero_d,
-                        reinterpret_cast<double*>(work.z.ptr) + e0, E, (long)y_one, g);
-                } else {
-                    const float one_f = 1.f, zero_f = 0.f;
-                    rs = rocblas_sgemm_strided_batched(
-                        h, rocblas_operation_none, rocblas_operation_transpose, m_e, (int)RD, N,
-                        &one_f, reinterpret_cast<const float*>(work.msel.ptr) + e0, E,
-                        (long)ms_one,
-                        reinterpret_cast<const float*>(work.d.ptr) + boff + (size_t)f * RD,
-                        (int)KR, (long)N * KR, &zero_f,
-                        reinterpret_cast<float*>(work.z.ptr) + e0, E, (long)y_one, g);
-                }
-                if (rs != rocblas_status_success)
-                    throw DeviceError("rocBLAS GEMM of the polynomial columns failed");
-            }
+            ProfScope ps("poly_z", st);
+            launch_poly_z(st, fp64, pd, bd, s, g, s0, a0, (int)RD, work.d.ptr, doff_dev,
+                          work.m.ptr, work.z.ptr, (long)y_one);
         }
+        check_hip(hipGetLastError(), "polynomial entry launch");
         {
-            ProfScope ps("poly_gather", st);
-            launch_poly_gather(st, fp64, pd, bd, s, g, a0, work.z.ptr, (long)y_one,
-                               work.xf.ptr, X, ld, b.row_off[s0], roww, xe_sum, xe_sq, (int)RD);
+            ProfScope ps("poly_rows", st);
+            launch_poly_rows(st, fp64, pd, bd, s, g, a0, (int)RD, work.xf.ptr, work.z.ptr,
+                             (long)y_one, X, ld, b.row_off[s0], roww, xe_sum, xe_sq);
         }
-        check_hip(hipGetLastError(), "poly gather launch");
+        check_hip(hipGetLastError(), "polynomial column launch");
         s += g;
     }
 }

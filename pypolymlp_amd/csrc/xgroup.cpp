@@ -4,11 +4,12 @@
 // their derivative weights W'[f, .] for every type pair (the reference's
 // invariants of one angular combination differ only in the radial index and
 // the coefficients). Each group is cut into tasks of up to 16 features
-// (slots). Per task and type pair the atom's W' entries are stored
-// [key][slot], so a wave reads one key's weights of all its slots with scalar
-// loads. The W' nonzeros themselves come from Features::compute_features_deriv
-// (src/cxx/src/polymlp/polymlp_features.cpp:208-248) as listed in
-// ModelHandle::build_device.
+// (slots). Per task and type pair the grouped positions run [key][slot]; a
+// table maps each position to the W' nonzero of the slot's feature and key,
+// so the kernel reads one key's weights of all its slots from the atom's
+// feature-major W'. The W' nonzeros themselves come from
+// Features::compute_features_deriv (src/cxx/src/polymlp/polymlp_features.cpp:208-248)
+// as listed in ModelHandle::build_device.
 #include <algorithm>
 #include <map>
 #include <stdexcept>
@@ -26,7 +27,7 @@ const GroupTables& ModelHandle::groups() {
     const int nt = M.n_type(), ntp = M.n_tp(), nlin = M.n_linear();
     constexpr int NS = kGroupSlots;
     std::vector<int> task_type, task_gcol, tt_nkey, tt_keyoff, keys, tt_woff, tt_nid;
-    std::vector<int> wg_pos, wg_pos_off(1, 0), wg_hole, wg_hole_off(1, 0);
+    std::vector<std::vector<int>> type_src;  // per type: W' nonzero of each grouped position
     std::vector<int> g2l((size_t)nt * std::max(nlin, 1), -1);
     gt->type_task0.assign(1, 0);
     bool ok = nlin > 0;
@@ -78,10 +79,7 @@ const GroupTables& ModelHandle::groups() {
                     for (int sl = 0; sl < NS; ++sl)
                         tt_nid.push_back(sl < nf && nk > 0 ? fn[feats[k0 + sl]][tp] : -1);
                     for (int sl = 0; sl < NS; ++sl) {
-                        if (sl >= nf) {
-                            for (int k = 0; k < nk; ++k) wg_hole.push_back(cur + k * NS + sl);
-                            continue;
-                        }
+                        if (sl >= nf) continue;  // no feature: positions stay empty
                         const int f = feats[k0 + sl];
                         for (int g = wzo[(size_t)f * ntp + tp]; g < wzo[(size_t)f * ntp + tp + 1]; ++g) {
                             const int k = (int)(std::lower_bound(ks.begin(), ks.end(), wy[g]) - ks.begin());
@@ -92,11 +90,12 @@ const GroupTables& ModelHandle::groups() {
                 }
             }
         }
-        for (int p : pos)
-            if (p < 0) ok = false;
-        wg_pos.insert(wg_pos.end(), pos.begin(), pos.end());
-        wg_pos_off.push_back((int)wg_pos.size());
-        wg_hole_off.push_back((int)wg_hole.size());
+        std::vector<int> src(cur, -1);
+        for (size_t g = 0; g < pos.size(); ++g) {
+            if (pos[g] < 0) ok = false;
+            else src[pos[g]] = (int)g;
+        }
+        type_src.push_back(std::move(src));
         wg_stride = std::max(wg_stride, cur);
         gt->type_task0.push_back((int)task_type.size());
     }
@@ -114,10 +113,10 @@ const GroupTables& ModelHandle::groups() {
         d.tt_woff = S.upload(tt_woff.data(), tt_woff.size() * 4);
         d.tt_nid = S.upload(tt_nid.data(), tt_nid.size() * 4);
         d.g2l = S.upload(g2l.data(), g2l.size() * 4);
-        d.wg_pos = S.upload(wg_pos.data(), wg_pos.size() * 4);
-        d.wg_pos_off = S.upload(wg_pos_off.data(), wg_pos_off.size() * 4);
-        d.wg_hole = S.upload(wg_hole.data(), wg_hole.size() * 4);
-        d.wg_hole_off = S.upload(wg_hole_off.data(), wg_hole_off.size() * 4);
+        std::vector<int> src((size_t)nt * wg_stride, -1);
+        for (int t = 0; t < nt; ++t)
+            std::copy(type_src[t].begin(), type_src[t].end(), src.begin() + (size_t)t * wg_stride);
+        d.wg_src = S.upload(src.data(), src.size() * 4);
         d.wg_stride = wg_stride;
     }
     group_tables = std::move(gt);

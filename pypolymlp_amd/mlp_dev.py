@@ -18,6 +18,7 @@ without the HIP library every entry point raises.
 
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -339,11 +340,15 @@ def accumulate_normal(model: capi.Model, datasets: Sequence[Dataset], precision=
     accumulation is the same; only the energy-row sums and y^T y come to the
     host, where the scales are computed (utils_scales.py:6-40). The scaling
     of X^T X and X^T y is applied on the device by DeviceNormal.acc.ridge /
-    sq_error. keep_parts: keep the device batches for a residual pass."""
+    sq_error. keep_parts: keep the device batches for a residual pass, and
+    their weighted rows on the device up to PLM_KEEP_ROWS_GB (default 64) GiB,
+    so that the pass reads them instead of building them again."""
     if min_energy is None:
         min_energy = get_min_energy(datasets)
     acc = capi.Normal(model)
     kept = [] if keep_parts else None
+    if keep_parts:
+        acc.keep_rows(int(float(os.environ.get("PLM_KEEP_ROWS_GB", "64")) * (1 << 30)))
     for batch, w, yw in _parts(model, datasets, min_energy, weight_stress, batch_size):
         acc.accumulate(batch, w, yw, precision)
         if keep_parts:
@@ -420,6 +425,7 @@ def fit(model: capi.Model, train: List[Dataset], test: List[Dataset], alphas,
         coefs, sq, info2 = refine(tr, coefs, alphas, precision)
         ok &= info2 == 0
         tr.parts = None
+        tr.acc.keep_rows(0)
     sq = np.where(ok, sq, np.inf)
     rmse_train = _rmse_from_sq(sq, tr.total_n_data)
     te_parts = _parts(model, test, tr.min_energy, weight_stress, batch_size)

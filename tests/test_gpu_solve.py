@@ -69,28 +69,34 @@ def test_ridge_not_positive_definite():
     assert np.all(Cr == 1e30)
 
 
+def _oracle_normal(R, ds, min_e, scales=None, weight_stress=0.1):
+    """Oracle normal equations: reference X, numpy X^T X, the reference's
+    scale rule (data_sequential.py:72-92)."""
+    ds.sort_by_atoms()
+    sts = ds.structures
+    X = R.x_batch([s.axis for s in sts], [s.positions_cartesian for s in sts],
+                  [s.types for s in sts], force=True)
+    w, y = mlp_dev.row_weights(ds, min_e, weight_stress)
+    ne = len(sts)
+    xe_sum = X[:ne].sum(axis=0)
+    xe_sq = (X[:ne] ** 2).sum(axis=0)
+    scales, zero = mlp_dev.compute_scales(scales, xe_sum, xe_sq, ne)
+    Xw = X * w[:, None]
+    xtx = Xw.T @ Xw
+    xty = Xw.T @ y
+    xtx[zero] = 0.0
+    xtx[:, zero] = 0.0
+    xty[zero] = 0.0
+    xtx /= scales[:, None] * scales[None, :]
+    xty /= scales
+    return mlp_dev.NormalEquations(xtx=xtx, xty=xty, y_sq_norm=float(y @ y),
+                                   total_n_data=X.shape[0], scales=scales, min_energy=min_e)
+
+
 def _oracle_fit(R, train, test, alphas, min_e, weight_stress=0.1):
-    """Oracle fit: reference X, numpy X^T X, the reference's scale rule, posv."""
+    """Oracle fit: _oracle_normal, posv, model selection on the test RMSE."""
     def normal(ds, scales=None):
-        ds.sort_by_atoms()
-        sts = ds.structures
-        X = R.x_batch([s.axis for s in sts], [s.positions_cartesian for s in sts],
-                      [s.types for s in sts], force=True)
-        w, y = mlp_dev.row_weights(ds, min_e, weight_stress)
-        ne = len(sts)
-        xe_sum = X[:ne].sum(axis=0)
-        xe_sq = (X[:ne] ** 2).sum(axis=0)
-        scales, zero = mlp_dev.compute_scales(scales, xe_sum, xe_sq, ne)
-        Xw = X * w[:, None]
-        xtx = Xw.T @ Xw
-        xty = Xw.T @ y
-        xtx[zero] = 0.0
-        xtx[:, zero] = 0.0
-        xty[zero] = 0.0
-        xtx /= scales[:, None] * scales[None, :]
-        xty /= scales
-        return mlp_dev.NormalEquations(xtx=xtx, xty=xty, y_sq_norm=float(y @ y),
-                                       total_n_data=X.shape[0], scales=scales, min_energy=min_e)
+        return _oracle_normal(R, ds, min_e, scales, weight_stress)
 
     tr = normal(train)
     coefs = solver_ridge_posv(tr.xtx, tr.xty, alphas)
@@ -245,3 +251,34 @@ def test_fit_device_resident_matches_host_path():
     assert np.all(c3[zero, 0] == 0.0)
     bound = max(100 * np.linalg.cond(Aa) * np.finfo(float).eps, 1e-12)
     assert np.linalg.norm(c3[:, 0] - c_ref) <= bound * np.linalg.norm(c_ref)
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+def test_fp64_coefficients_cfg1_vs_oracle(tmp_path):
+    """The FP64 fit's coefficients on cfg1's own normal matrix (F = 2277): the
+    device path (plm_normal_accumulate, plm_normal_ridge) against the
+    reference X with numpy normal equations and LAPACK, normwise within
+    max(1e-10, 100 cond eps), and within 1e-10 wherever cond eps < 1e-12."""
+    from tests.helpers import write_model_yaml
+
+    spec = datasets.config1_model()
+    ds = datasets.config1_dataset(24, seed=4)
+    m = capi.Model(spec.as_dict())
+    min_e = mlp_dev.get_min_energy([ds])
+    alphas = np.array([1e-3, 1e-1, 10.0])
+    say("cfg1 FP64 coefficients: 24 structures, device normal equations and ridge")
+    dn = mlp_dev.accumulate_normal(m, [ds], min_energy=min_e)
+    c_dev, info, _ = dn.acc.ridge(dn.scales, dn.zero, alphas)
+    assert np.all(info == 0)
+    R = ref.RefModel(write_model_yaml(spec, str(tmp_path / "m.yaml")))
+    ne = _oracle_normal(R, ds, min_e)
+    np.testing.assert_allclose(dn.scales, ne.scales, rtol=1e-10)
+    for k, a in enumerate(alphas):
+        A = ne.xtx + a * np.eye(len(ne.xty))
+        c_ref = np.linalg.solve(A, ne.xty)
+        cond = np.linalg.cond(A)
+        err = np.linalg.norm(c_dev[:, k] - c_ref) / np.linalg.norm(c_ref)
+        say(f"  alpha {a:g}: cond {cond:.2e}, relative coefficient error {err:.2e}")
+        assert err <= max(1e-10, 100 * cond * np.finfo(float).eps)
+        if cond * np.finfo(float).eps < 1e-12:
+            assert err <= 1e-10

@@ -114,12 +114,19 @@ def test_neighbor_full_and_half_bit_exact():
 def _use_path(path, monkeypatch):
     """The linear columns are built by the signature-group kernel unless
     PLM_XROWS=generic selects the column-block kernel; PLM_XROWS=global makes
-    that kernel keep its row tile in X itself (as for structures whose rows do
-    not fit LDS)."""
+    that kernel keep its row tile in X itself. "group-queue" forces the
+    signature-group kernel's persistent work-queue launch with one task per
+    workgroup and "group-split" its row tile cut into two slot parts
+    (PLM_XGROUP); "group" lets the autotuner choose."""
+    monkeypatch.delenv("PLM_XGROUP", raising=False)
     if path in ("generic", "global"):
         monkeypatch.setenv("PLM_XROWS", path)
     else:
         monkeypatch.delenv("PLM_XROWS", raising=False)
+    if path == "group-queue":
+        monkeypatch.setenv("PLM_XGROUP", "1,8,1,1")
+    elif path == "group-split":
+        monkeypatch.setenv("PLM_XGROUP", "1,8,2,0")
 
 
 def _features_x(m, b, prec, path):
@@ -128,7 +135,7 @@ def _features_x(m, b, prec, path):
     X = capi.features_x(m, b, prec)
     capi.profile_enable(False)
     n_group = capi.profile_get("xgroup")[1]
-    assert (n_group > 0) == (path == "group"), f"path {path}: {n_group} signature-group launches"
+    assert (n_group > 0) == path.startswith("group"), f"path {path}: {n_group} signature-group launches"
     return X
 
 
@@ -153,7 +160,7 @@ def test_x_rows_mgo(model, path, monkeypatch):
     check_mixed(_features_x(m, b, capi.PLM_MIXED, path), Xr, noise)
 
 
-@pytest.mark.parametrize("path", ["group", "generic", "global"])
+@pytest.mark.parametrize("path", ["group", "group-queue", "group-split", "generic", "global"])
 @pytest.mark.parametrize("cfg,n_st", [("cfg1", 3), ("cfg2", 2), ("cfg3", 4)])
 def test_x_rows_configs(cfg, n_st, path, tmp_path, monkeypatch):
     _use_path(path, monkeypatch)
@@ -260,21 +267,35 @@ def test_normal_equations_sub_batches(prec, monkeypatch):
     assert c[4] == pytest.approx(a[4], rel=tol)
 
 
-def test_x_rows_largest_cfg3():
+@pytest.mark.parametrize("shape", [None, "1,8,2,1", "1,8,4,0", "global"])
+def test_x_rows_largest_cfg3(shape, monkeypatch):
     """The largest structure of the seeded 40-structure ternary set (429 atoms,
-    1294 rows x 25,428 columns) does not fit the LDS row tile, so its block of X
-    is accumulated in place. FP64 against the oracle's seeded projections,
-    energy row and column norms (tests/golden/make_cfg3_large.py); the FP32
-    stream against FP64 column-relative."""
+    1294 rows x 25,428 columns): its rows do not fit one task's LDS row tile,
+    so the signature-group kernel cuts the tile into slot parts (two by
+    default; four, and the persistent work queue, forced by PLM_XGROUP), or
+    the column-block kernel accumulates it in X (global). FP64 against the
+    oracle's seeded projections, energy row and column norms
+    (tests/golden/make_cfg3_large.py); the FP32 stream against FP64
+    column-relative."""
+    monkeypatch.delenv("PLM_XGROUP", raising=False)
+    monkeypatch.delenv("PLM_XROWS", raising=False)
+    if shape == "global":
+        monkeypatch.setenv("PLM_XROWS", "global")
+    elif shape is not None:
+        monkeypatch.setenv("PLM_XGROUP", shape)
     sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
     import make_cfg3_large as mk
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg3_large.npz"))
     st = mk.largest_structure()
     assert st.n_total == int(g["n_atoms"])
     m = capi.Model(datasets.config3_model().as_dict())
-    say(f"x rows cfg3 largest structure, {st.n_total} atoms")
+    say(f"x rows cfg3 largest structure, {st.n_total} atoms, shape {shape}")
     b = capi.Batch(m, [st.axis], [st.positions_cartesian], [st.types])
+    capi.profile_reset()
+    capi.profile_enable(True)
     X = capi.features_x(m, b, capi.PLM_FP64)
+    capi.profile_enable(False)
+    assert (capi.profile_get("xgroup")[1] > 0) == (shape != "global")
     assert X.shape == tuple(g["shape"])
     p, q = mk.projections(X)
     for name, a, r in (("u^T X", p, g["ut_x"]), ("X v", q, g["x_v"]), ("energy row", X[0], g["energy_row"])):
@@ -290,3 +311,26 @@ def test_x_rows_largest_cfg3():
     say(f"  mixed column-relative: median {np.median(e):.2e} p99 {np.quantile(e, 0.99):.2e} max {e.max():.2e}")
     assert np.median(e) < 1e-5
     assert np.quantile(e, 0.99) < 1e-3
+
+
+def test_autotuner_choice_matches_rule(monkeypatch):
+    """The signature-group launch shape chosen by timing (plm_model_tuning
+    reports it) builds the same X as the rule-based shape (PLM_AUTOTUNE=0), up
+    to the order of the FP64 atomic adds."""
+    monkeypatch.delenv("PLM_XGROUP", raising=False)
+    monkeypatch.delenv("PLM_XROWS", raising=False)
+    ds = datasets.config2_dataset(24, seed=2, targets=False)
+    axes, pos, types = _structs(ds.structures)
+    say("autotuner: binary alloy, 24 structures")
+    m = capi.Model(datasets.config2_model().as_dict())
+    b = capi.Batch(m, axes, pos, types)
+    X = capi.features_x(m, b, capi.PLM_FP64)
+    rep = capi.tuning_report(m)
+    say(rep)
+    assert "xgroup fp64" in rep
+    monkeypatch.setenv("PLM_AUTOTUNE", "0")
+    m2 = capi.Model(datasets.config2_model().as_dict())
+    b2 = capi.Batch(m2, axes, pos, types)
+    X2 = capi.features_x(m2, b2, capi.PLM_FP64)
+    assert capi.tuning_report(m2) == ""
+    assert np.linalg.norm(X - X2) <= 1e-12 * np.linalg.norm(X2)
