@@ -118,9 +118,10 @@ size_t poly_bytes(ModelHandle& m, const BatchHandle& b, int s0, int s1, bool fp6
     return n * (fp64 ? 8 : 4);
 }
 
-// LDS per workgroup of the factor-derivative kernel: smaller chunks of
-// factors mean more passes over the pairs but more workgroups per CU
-constexpr size_t kDerivLds = 64 * 1024;  // 32 KB: 16 % slower (more passes)
+// LDS per workgroup of the factor-derivative kernel. Every pass walks all of
+// an atom's pairs with one thread per factor of the chunk, so the factors are
+// split into as few passes as the LDS holds, in chunks of equal size.
+constexpr size_t kDerivLds = 150 * 1024;
 
 std::vector<long> poly_d_offsets(ModelHandle& m, const BatchHandle& b, int s0, int s1) {
     const PolyDev& pd = m.poly().dev;
@@ -164,7 +165,11 @@ void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, 
         launch_poly_factor(st, fp64, m.md, pd, bd, a0, na, work.x.ptr, x_stride, work.xf.ptr,
                            work.m.ptr);
         // factor chunks whose per-atom rows fit kDerivLds bytes of LDS
-        const int kc = std::max(1, std::min(pd.K, (int)(kDerivLds / ((size_t)rd_max * elem))));
+        // (at most 128: one thread of the 128-thread workgroup per factor)
+        const int kc_max = std::max(
+            1, std::min({pd.K, 128, (int)(kDerivLds / ((size_t)rd_max * elem))}));
+        const int passes = (pd.K + kc_max - 1) / kc_max;
+        const int kc = (pd.K + passes - 1) / passes;
         if ((size_t)kc * rd_max * elem > 160 * 1024)
             throw std::runtime_error("structure too large for the factor-derivative rows");
         for (int k0 = 0; k0 < pd.K; k0 += kc)
