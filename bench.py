@@ -344,6 +344,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-structures", type=int, default=1000)
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--graph-steps", type=int, default=2,
+                    help="replays of the mixed step captured as a HIP graph (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--eval-steps", type=int, default=5)
@@ -407,6 +409,17 @@ def main():
     log("timing mixed ...")
     dt_mixed, k_mixed = run(capi.PLM_MIXED, args.steps, args.warmup)
     log(f"mixed: {dt_mixed * 1e3:.1f} ms/step, {R / dt_mixed:.4g} rows/s, kernels {k_mixed}")
+    # the same step captured once as a HIP graph and replayed (plm_normal_graph_*)
+    dt_graph = None
+    if args.graph_steps > 0:
+        acc_g = capi.Normal(model)
+        graph = capi.NormalGraph(acc_g, batch, w, y, capi.PLM_MIXED)
+        graph.run(1)
+        t0 = time.perf_counter()
+        graph.run(args.graph_steps)
+        dt_graph = (time.perf_counter() - t0) / args.graph_steps
+        del graph, acc_g
+        log(f"mixed as a HIP graph: {dt_graph * 1e3:.1f} ms/step over {args.graph_steps} replays")
     dt_fp64, k_fp64 = None, {}
     if args.fp64_steps > 0:
         log("timing fp64 ...")
@@ -442,16 +455,21 @@ def main():
         gram_f64 = {"bound": "mfma", "kernel": "gram_f64_kernel", "unit": "TFLOP/s (fp64)",
                     "achieved": a64, "peak": FP64_PEAK_TFLOPS, "frac": a64 / FP64_PEAK_TFLOPS,
                     "launches_per_step": n_g64 / args.fp64_steps}
-    # linear columns of X by signature groups (xgroup_kernel): algorithmic
-    # bytes = its output (stress and force rows of the linear columns, FP32)
-    # plus its inputs read once (the pair records of the step)
+    # linear columns of X by signature groups (xgroup_kernel on the static
+    # grid, xgroup_queue_kernel on the persistent one, as the autotuner chose):
+    # algorithmic bytes = its output (stress and force rows of the linear
+    # columns, FP32) plus its inputs read once (the pair records of the step)
     n_fn = len(spec.pair_params)
     n_lmh = (spec.max_l + 1) * (spec.max_l + 2) // 2
     rec_bytes = 4.0 * ((8 + 2 * n_fn + 8 * n_lmh + 3) // 4 * 4)
     n_lin = len(capi.feature_attrs(model)[0])
     xg_bytes = float(R - len(sts)) * n_lin * 4.0 + batch.n_pairs * rec_bytes
     ms_x, n_x = k_mixed["xgroup"]
-    xgroup = {"bound": "hbm", "kernel": "xgroup_kernel", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+    tuned = [ln for ln in capi.tuning_report(model).splitlines() if "fp32" in ln]
+    forced = os.environ.get("PLM_XGROUP", "")
+    persistent = forced.endswith(",1") if forced else bool(tuned) and "persistent" in tuned[-1]
+    xg_kernel = "xgroup_queue_kernel" if persistent else "xgroup_kernel"
+    xgroup = {"bound": "hbm", "kernel": xg_kernel, "unit": "GB/s", "peak": HBM_PEAK_GBS,
               "launches_per_step": n_x / args.steps,
               "achieved": xg_bytes * args.steps / (ms_x * 1e-3) / 1e9 if ms_x else None}
     if xgroup["achieved"]:
@@ -505,6 +523,7 @@ def main():
         "fp64_rows_per_s": R / dt_fp64 if dt_fp64 else None,
         "fp64_ms_per_step": dt_fp64 * 1e3 if dt_fp64 else None,
         "speedup_mixed_vs_fp64": dt_fp64 / dt_mixed if dt_fp64 else None,
+        "graph_ms_per_step_mixed": dt_graph * 1e3 if dt_graph else None,
         "kernel_ms_per_step_mixed": {k: v[0] / args.steps for k, v in k_mixed.items()},
         "kernel_ms_per_step_fp64": {k: v[0] / args.fp64_steps for k, v in k_fp64.items()},
         "roofline": roof,
