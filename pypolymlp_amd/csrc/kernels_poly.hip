@@ -44,12 +44,13 @@ __global__ void poly_factor_kernel(ModelDev md, PolyDev pd, BatchDev bd, int ato
     }
 }
 
-// D[i][f][r] for the factors f in [k0, k0 + kc): one workgroup per atom, one
-// thread per factor walking the atom's pairs (rows of neighbours j and of
-// atom i and the 6 stress rows accumulate in LDS / registers without
-// atomics), then a coalesced write of the atom's rows.
+// D[i][f][r] for the factors f in [k0, k0 + kc) (kc <= 128): one 256-thread
+// workgroup per atom, blockDim / kc threads per factor, each walking every
+// (blockDim / kc)-th pair of the atom (rows of neighbours j added with LDS
+// atomics, the rows of atom i and the 6 stress rows summed in registers and
+// added once), then a coalesced write of the atom's rows.
 template <class T>
-__global__ void __launch_bounds__(128) poly_deriv_kernel(ModelDev md, PolyDev pd, BatchDev bd,
+__global__ void __launch_bounds__(256) poly_deriv_kernel(ModelDev md, PolyDev pd, BatchDev bd,
                                                          int atom0, int n_atoms, long pair_base,
                                                          const T* __restrict__ prec, PairRec R,
                                                          const Cpx<T>* __restrict__ wall,
@@ -70,8 +71,11 @@ __global__ void __launch_bounds__(128) poly_deriv_kernel(ModelDev md, PolyDev pd
     const int ntp = md.n_tp;
     for (int e = threadIdx.x; e < kc * RD; e += blockDim.x) acc[e] = T(0);
     __syncthreads();
-    const int k = threadIdx.x;
-    if (k < kc) {
+    // thread -> (factor k, pair slice sl): the S slices of a factor take every
+    // S-th pair of the atom and add into the factor's rows with LDS atomics
+    const int S = blockDim.x / kc;
+    const int k = threadIdx.x % kc, sl = threadIdx.x / kc;
+    if (sl < S) {
         const int fl = pd.fac_local[t1 * pd.K + k0 + k];
         if (fl >= 0) {
             T* a = acc + (size_t)k * RD;
@@ -79,7 +83,7 @@ __global__ void __launch_bounds__(128) poly_deriv_kernel(ModelDev md, PolyDev pd
             T s0 = T(0), s1 = T(0), s2 = T(0);
             T st[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
             const int p0 = bd.nb_off[i], p1 = bd.nb_off[i + 1];
-            for (int p = p0; p < p1; ++p) {
+            for (int p = p0 + sl; p < p1; p += S) {
                 const T* rec = prec + (size_t)(p - pair_base) * R.stride;
                 if (rec[4] == T(0)) continue;
                 const int tp = int(rec[5]);
@@ -110,17 +114,17 @@ __global__ void __launch_bounds__(128) poly_deriv_kernel(ModelDev md, PolyDev pd
                 }
                 s0 += q0, s1 += q1, s2 += q2;
                 const int jl = int(rec[6]);
-                a[6 + 3 * jl] -= q0;
-                a[6 + 3 * jl + 1] -= q1;
-                a[6 + 3 * jl + 2] -= q2;
+                atomicAdd(a + 6 + 3 * jl, -q0);
+                atomicAdd(a + 6 + 3 * jl + 1, -q1);
+                atomicAdd(a + 6 + 3 * jl + 2, -q2);
                 const T dx = rec[0], dy = rec[1], dz = rec[2];
                 st[0] -= q0 * dx, st[1] -= q1 * dy, st[2] -= q2 * dz;
                 st[3] -= q0 * dy, st[4] -= q1 * dz, st[5] -= q2 * dx;
             }
-            a[6 + 3 * (i - a0)] += s0;
-            a[6 + 3 * (i - a0) + 1] += s1;
-            a[6 + 3 * (i - a0) + 2] += s2;
-            for (int c = 0; c < 6; ++c) a[c] += st[c];
+            atomicAdd(a + 6 + 3 * (i - a0), s0);
+            atomicAdd(a + 6 + 3 * (i - a0) + 1, s1);
+            atomicAdd(a + 6 + 3 * (i - a0) + 2, s2);
+            for (int c = 0; c < 6; ++c) atomicAdd(a + c, st[c]);
         }
     }
     __syncthreads();
@@ -153,14 +157,14 @@ void launch_poly_deriv(hipStream_t st, bool fp64, const ModelDev& md, const Poly
     if (fp64) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&poly_deriv_kernel<double>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        poly_deriv_kernel<double><<<n_atoms, 128, sm, st>>>(
+        poly_deriv_kernel<double><<<n_atoms, 256, sm, st>>>(
             md, pd, bd, atom0, n_atoms, pair_base, reinterpret_cast<const double*>(prec), R,
             reinterpret_cast<const Cpx<double>*>(wall), w_stride, k0, kc,
             reinterpret_cast<double*>(D), st0, d_off);
     } else {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&poly_deriv_kernel<float>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        poly_deriv_kernel<float><<<n_atoms, 128, sm, st>>>(
+        poly_deriv_kernel<float><<<n_atoms, 256, sm, st>>>(
             md, pd, bd, atom0, n_atoms, pair_base, reinterpret_cast<const float*>(prec), R,
             reinterpret_cast<const Cpx<float>*>(wall), w_stride, k0, kc,
             reinterpret_cast<float*>(D), st0, d_off);

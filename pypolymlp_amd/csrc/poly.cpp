@@ -165,7 +165,8 @@ void build_poly_rows(ModelHandle& m, BatchHandle& b, int s0, int s1, bool fp64, 
         launch_poly_factor(st, fp64, m.md, pd, bd, a0, na, work.x.ptr, x_stride, work.xf.ptr,
                            work.m.ptr);
         // factor chunks whose per-atom rows fit kDerivLds bytes of LDS
-        // (at most 128: one thread of the 128-thread workgroup per factor)
+        // (at most 128, so that the 256-thread workgroup has at least two
+        // threads per factor)
         const int kc_max = std::max(
             1, std::min({pd.K, 128, (int)(kDerivLds / ((size_t)rd_max * elem))}));
         const int passes = (pd.K + kc_max - 1) / kc_max;
