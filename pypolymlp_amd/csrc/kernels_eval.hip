@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <stdexcept>
 
 #include "device_math.hpp"
 #include "device_tables.hpp"
@@ -32,7 +33,8 @@ __device__ __forceinline__ Cpx<T> prod_keys3(const Cpx<T>* a, const int* keys) {
 // ---------------------------------------------------------------------------
 // Per atom: features, products of features, and
 //   prod_sum_{e,f}[h] = sum_terms coeff_{e,f} * prod_features * prod_anlm
-// One 64-lane workgroup per atom; features staged in LDS.
+// One 64-lane workgroup per atom; the atom's order parameters and features
+// staged in LDS (the term products gather from them at random keys).
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, BatchDev bd,
@@ -40,14 +42,18 @@ __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, Ba
                                                      int a_stride, Cpx<T>* __restrict__ ps_e,
                                                      Cpx<T>* __restrict__ ps_f, int ps_stride,
                                                      int pair_model) {
-    extern __shared__ unsigned char smem_raw[];
-    T* xs = reinterpret_cast<T*>(smem_raw);
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    // the atom's order parameters (a_stride complex values), then its features
+    Cpx<T>* a = reinterpret_cast<Cpx<T>*>(smem_raw);
+    T* xs = reinterpret_cast<T*>(a + a_stride);
     const int i = blockIdx.x;
     if (i >= bd.n_atoms_total) return;
     const int t1 = bd.types[i];
     const TypeDev& TT = md.t[t1];
     const EvalTypeDev& E = ed.t[t1];
-    const Cpx<T>* a = a_full + (size_t)i * a_stride;
+    const Cpx<T>* ag = a_full + (size_t)i * a_stride;
+    for (int k = threadIdx.x; k < TT.n_nlm; k += blockDim.x) a[k] = ag[k];
+    __syncthreads();
     for (int f = threadIdx.x; f < TT.n_lin; f += blockDim.x) {
         T v = T(0);
         for (int e = TT.fv_off[f]; e < TT.fv_off[f + 1]; ++e) {
@@ -187,7 +193,8 @@ template <class T>
 void launch_prodsum_t(hipStream_t st, const ModelDev& md, const EvalDev& ed, const BatchDev& bd,
                       const void* a_full, int a_stride, void* ps_e, void* ps_f, int ps_stride,
                       int max_lin, int pair_model) {
-    const size_t sm = (size_t)std::max(1, max_lin) * sizeof(T);
+    const size_t sm = (size_t)a_stride * 2 * sizeof(T) + (size_t)std::max(1, max_lin) * sizeof(T);
+    if (sm > 160 * 1024) throw std::invalid_argument("per-atom evaluation data exceed LDS");
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&prodsum_kernel<T>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     prodsum_kernel<T><<<bd.n_atoms_total, 64, sm, st>>>(

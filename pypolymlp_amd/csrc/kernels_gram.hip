@@ -125,9 +125,10 @@ __device__ __forceinline__ int exponent_of(double m) {
 }
 
 // E[chunk][c]: smallest E with |x| < 2^E over the chunk's rows (0 for a zero column).
-// 64 columns per workgroup, four row phases of 64 threads; each thread keeps
-// four loads in flight (the maximum is order independent, so the result does
-// not depend on the split)
+// 64 columns per workgroup, four row phases of 64 threads; each thread reads
+// the row indices of its next eight rows, then keeps their eight loads in
+// flight (the maximum is order independent, so the result does not depend on
+// the split)
 template <class T>
 __global__ void __launch_bounds__(256) colexp_kernel(const T* __restrict__ X,
                                                      const int* __restrict__ rmap, int ld, int rows,
@@ -138,18 +139,26 @@ __global__ void __launch_bounds__(256) colexp_kernel(const T* __restrict__ X,
     const int chunk = blockIdx.y;
     const int r0 = chunk == 0 ? 0 : h0 + (chunk - 1) * kQChunk;
     const int r1 = min(rows, chunk == 0 ? h0 : r0 + kQChunk);
-    T m0 = T(0), m1 = T(0), m2 = T(0), m3 = T(0);
+    T m[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = T(0);
     if (c < ncols) {
         int r = r0 + ty;
-        for (; r + 12 < r1; r += 16) {
-            m0 = fmax(m0, fabs(xrow(X, rmap, r, ld, c)));
-            m1 = fmax(m1, fabs(xrow(X, rmap, r + 4, ld, c)));
-            m2 = fmax(m2, fabs(xrow(X, rmap, r + 8, ld, c)));
-            m3 = fmax(m3, fabs(xrow(X, rmap, r + 12, ld, c)));
+        for (; r + 28 < r1; r += 32) {
+            int rr[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rr[u] = rmap[r + 4 * u];
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = rr[u] >= 0 ? X[(size_t)rr[u] * ld + c] : T(0);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) m[u] = fmax(m[u], fabs(v[u]));
         }
-        for (; r < r1; r += 4) m0 = fmax(m0, fabs(xrow(X, rmap, r, ld, c)));
+        for (; r < r1; r += 4) m[0] = fmax(m[0], fabs(xrow(X, rmap, r, ld, c)));
     }
-    part[ty][cx] = fmax(fmax(m0, m1), fmax(m2, m3));
+#pragma unroll
+    for (int u = 1; u < 8; ++u) m[0] = fmax(m[0], m[u]);
+    part[ty][cx] = m[0];
     __syncthreads();
     if (ty == 0 && c < ncols) {
         const T m = fmax(fmax(part[0][cx], part[1][cx]), fmax(part[2][cx], part[3][cx]));

@@ -70,6 +70,11 @@ template <> struct GroupMfma<double> {
 
 constexpr int kGroupKSteps = 6;  // key steps (2 keys each) held per atom and type pair
 
+// four consecutive record elements in one load (records are 4-element aligned)
+template <class T> struct GroupVec4;
+template <> struct GroupVec4<float> { using type = float4; };
+template <> struct GroupVec4<double> { using type = double4; };
+
 // Per (atom, type pair) state of a wave: the B operand of each key step and
 // the A-operand offset of this lane in a pair record (-1: the constant Y_00,
 // -2: padding).
@@ -106,7 +111,10 @@ __device__ __forceinline__ void group_pairs(const T* __restrict__ pd, long pair_
         for (int s = 0; s < KS; ++s) a[m][s] = a_value<T>(ra, K.off[s], const_one);
         const int pb = min(pc + 4 * m + (lane >> 4), pe - 1);
         const T* rb = pd + (size_t)(pb - pair_base) * stride;
-        dx[m] = rb[0], dy[m] = rb[1], dz[m] = rb[2], r[m] = rb[3], valid[m] = rb[4], jl[m] = rb[6];
+        using V4 = typename GroupVec4<T>::type;
+        const V4 h0 = *reinterpret_cast<const V4*>(rb);      // d, r
+        const V4 h1 = *reinterpret_cast<const V4*>(rb + 4);  // valid, tp, j
+        dx[m] = h0.x, dy[m] = h0.y, dz[m] = h0.z, r[m] = h0.w, valid[m] = h1.x, jl[m] = h1.z;
         const int n = max(nid, 0);
         fv[m] = rb[8 + n];
         fd[m] = rb[8 + nfn + n];
