@@ -204,7 +204,8 @@ int plm_normal_residual(plm_normal* acc, plm_batch* b, const double* row_w, cons
  * (per sub-batch, while their total stays within max_bytes of device
  * memory), so that plm_normal_residual over the same batch with the same
  * row_w / y_w and precision reads them instead of building them again.
- * max_bytes = 0 releases the kept rows and keeps none. No counterpart in the
+ * max_bytes = 0 releases the kept rows (and the Cholesky factors that
+ * plm_normal_ridge keeps for plm_normal_ridge_correct) and keeps none. No counterpart in the
  * reference (its X is a host array, data_sequential.py:97-155); it serves the
  * refinement step of a fit from the quantised Gram. */
 int plm_normal_keep_rows(plm_normal* acc, unsigned long long max_bytes);
@@ -212,6 +213,9 @@ int plm_normal_keep_rows(plm_normal* acc, unsigned long long max_bytes);
  * (scaling as in plm_normal_ridge): delta_k = (A + alpha_k I)^{-1} rhs_k
  * (Cholesky of the accumulated Gram's system in FP64, with one inner
  * refinement step), rhs / delta: n_alpha x F, info as in plm_normal_ridge.
+ * The factors of the last plm_normal_ridge call on the same accumulator state
+ * and scaling are reused (plm_normal_ridge keeps up to 32 GiB of them until the
+ * next accumulation).
  * With rhs_k = xtr_k / s - alpha_k c_k from plm_normal_residual, c_k + delta_k
  * solves the normal equations of X to the accuracy of the X build instead of
  * that of the quantised Gram. dad (n_alpha, may be NULL): delta_k^T A delta_k,

@@ -199,6 +199,22 @@ struct KeptRows {
     DevBuf x;
 };
 
+// Cholesky factors of an accumulator's scaled system, kept by plm_normal_ridge
+// for plm_normal_ridge_correct: valid for the accumulator's state (rows
+// accumulated) and the scales / dropped columns they were made with.
+struct FactorCache {
+    long n_rows = -1;
+    std::vector<double> scales;
+    std::vector<unsigned char> zero;
+    std::vector<std::pair<double, std::unique_ptr<DevBuf>>> f;  // (alpha, factor)
+    size_t bytes = 0;
+    void clear() {
+        f.clear();
+        bytes = 0;
+        n_rows = -1;
+    }
+};
+
 struct NormalHandle {
     ModelHandle* model = nullptr;
     int F = 0, F_pad = 0;
@@ -208,6 +224,7 @@ struct NormalHandle {
     AccumWork ws;
     size_t keep_budget = 0, kept_bytes = 0;  // plm_normal_keep_rows
     std::vector<std::unique_ptr<KeptRows>> kept;
+    FactorCache factors;
 };
 // FNV-1a over the row weights and weighted targets of n rows
 unsigned long long hash_rows(const double* w, const double* y, long n);

@@ -182,6 +182,7 @@ void normal_accumulate(NormalHandle& acc, BatchHandle& b, const double* row_w, c
                        bool fp64, bool quant) {
     if (!fp64 && !quant) throw std::invalid_argument("the FP64 Gram needs the FP64 X build");
     hipStream_t st = device().stream;
+    acc.factors.clear();  // the system changes
     AccumPlan plan;
     plan_accumulate(acc, b, row_w, y_w, fp64, quant, plan);
     KeepRequest kq{b.serial, 0};
@@ -198,6 +199,7 @@ void normal_keep_rows(NormalHandle& acc, size_t max_bytes) {
     if (max_bytes == 0) {
         acc.kept.clear();
         acc.kept_bytes = 0;
+        acc.factors.clear();
     }
 }
 
@@ -238,6 +240,7 @@ void normal_graph_capture(NormalGraph& g, const double* row_w, const double* y_w
 void normal_graph_run(NormalGraph& g, int n_steps) {
     if (n_steps < 1) throw std::invalid_argument("steps must be positive");
     hipStream_t st = device().stream;
+    g.acc->factors.clear();  // the system changes
     for (int k = 0; k < n_steps; ++k) check_hip(hipGraphLaunch(g.exec, st), "graph launch");
     check_hip(hipStreamSynchronize(st), "sync");
     g.acc->n_rows += (long)n_steps * g.plan.n_rows;

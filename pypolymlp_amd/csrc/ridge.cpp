@@ -2,7 +2,9 @@
 // src/pypolymlp/mlp_dev/standard/solvers.py:48-84) on the GPU in FP64:
 // Cholesky of (X^T X + alpha I), triangular solves and one step of iterative
 // refinement with the residual of the unfactored system.
+#include <algorithm>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "capi_internal.hpp"
@@ -27,39 +29,60 @@ void launch_symv(hipStream_t st, const double* A, int lda, int n, const double* 
 // refinement step with the residual of the unfactored system. coeffs: n_alpha
 // x F on the host. b: one right-hand side of np entries on the device, or,
 // with b_stride np, one per alpha.
+// cache: factors of successful alphas are kept in it while it has room
+// (kFactorBudget) and, for alphas it already holds, reused instead of
+// factoring again.
+static constexpr size_t kFactorBudget = 32ull << 30;
+
 static void ridge_core(int F, int np, const DevBuf& A0, const DevBuf& b, const double* alphas,
-                       int n_alpha, double* coeffs, int* info, size_t b_stride = 0) {
+                       int n_alpha, double* coeffs, int* info, size_t b_stride = 0,
+                       FactorCache* cache = nullptr) {
     hipStream_t st = device().stream;
     const size_t mat = (size_t)np * np * 8;
-    DevBuf Af, x, r, di;
-    Af.alloc(mat), x.alloc(np * 8), r.alloc(np * 8), di.alloc(4);
+    auto Af = std::make_unique<DevBuf>();
+    DevBuf x, r, di;
+    x.alloc(np * 8), r.alloc(np * 8), di.alloc(4);
     for (int k = 0; k < n_alpha; ++k) {
         const double a = alphas[k];
-        check_hip(hipMemcpyAsync(Af.ptr, A0.ptr, mat, hipMemcpyDeviceToDevice, st), "copy");
-        launch_add_diag(st, (double*)Af.ptr, np, F, a);
-        check_hip(hipMemsetAsync(di.ptr, 0, 4, st), "memset");
-        {
-            ProfScope ps("cholesky", st);
-            launch_cholesky(st, (double*)Af.ptr, np, np, (int*)di.ptr);
+        const DevBuf* L = nullptr;
+        if (cache)
+            for (const auto& e : cache->f)
+                if (e.first == a) L = e.second.get();
+        int hi = 0;
+        if (!L) {
+            Af->alloc(mat);
+            check_hip(hipMemcpyAsync(Af->ptr, A0.ptr, mat, hipMemcpyDeviceToDevice, st), "copy");
+            launch_add_diag(st, (double*)Af->ptr, np, F, a);
+            check_hip(hipMemsetAsync(di.ptr, 0, 4, st), "memset");
+            {
+                ProfScope ps("cholesky", st);
+                launch_cholesky(st, (double*)Af->ptr, np, np, (int*)di.ptr);
+            }
+            check_hip(hipGetLastError(), "cholesky launch");
+            check_hip(hipMemcpyAsync(&hi, di.ptr, 4, hipMemcpyDeviceToHost, st), "copy");
+            check_hip(hipStreamSynchronize(st), "sync");
+            L = Af.get();
         }
-        check_hip(hipGetLastError(), "cholesky launch");
         {
             ProfScope ps("chol_solve", st);
             const double* bk = (const double*)b.ptr + k * b_stride;
             check_hip(hipMemcpyAsync(x.ptr, bk, np * 8, hipMemcpyDeviceToDevice, st), "copy");
-            launch_chol_solve(st, (double*)Af.ptr, np, np, (double*)x.ptr);
+            launch_chol_solve(st, (const double*)L->ptr, np, np, (double*)x.ptr);
             // one step of iterative refinement
             check_hip(hipMemsetAsync(r.ptr, 0, np * 8, st), "memset");
             launch_residual(st, (double*)A0.ptr, np, F, a, (double*)x.ptr, bk, (double*)r.ptr);
-            launch_chol_solve(st, (double*)Af.ptr, np, np, (double*)r.ptr);
+            launch_chol_solve(st, (const double*)L->ptr, np, np, (double*)r.ptr);
             launch_axpy(st, F, (double*)r.ptr, (double*)x.ptr);
         }
         check_hip(hipGetLastError(), "solve launch");
-        int hi = 0;
         check_hip(hipMemcpyAsync(coeffs + (size_t)k * F, x.ptr, (size_t)F * 8, hipMemcpyDeviceToHost, st), "copy");
-        check_hip(hipMemcpyAsync(&hi, di.ptr, 4, hipMemcpyDeviceToHost, st), "copy");
         check_hip(hipStreamSynchronize(st), "sync");
         info[k] = hi;
+        if (cache && L == Af.get() && hi == 0 && cache->bytes + mat <= kFactorBudget) {
+            cache->bytes += mat;
+            cache->f.emplace_back(a, std::move(Af));
+            Af = std::make_unique<DevBuf>();
+        }
     }
 }
 
@@ -136,7 +159,14 @@ void normal_ridge(NormalHandle& acc, const double* scales, const unsigned char* 
     std::vector<double> hb;
     double yty = 0.0;
     const int np = build_system(acc, scales, zero, A0, b, hb, yty);
-    ridge_core(acc.F, np, A0, b, alphas, n_alpha, coeffs, info);
+    // the factors are kept for a later correction step on the same system
+    FactorCache& fc = acc.factors;
+    fc.clear();
+    fc.n_rows = acc.n_rows;
+    fc.scales.assign(scales, scales + acc.F);
+    fc.zero.assign(acc.F, 0);
+    if (zero) fc.zero.assign(zero, zero + acc.F);
+    ridge_core(acc.F, np, A0, b, alphas, n_alpha, coeffs, info, 0, &fc);
     if (sq_err) sq_errors(acc.F, np, A0, hb, yty, coeffs, n_alpha, sq_err);
 }
 
@@ -176,7 +206,12 @@ void normal_ridge_correct(NormalHandle& acc, const double* scales, const unsigne
     DevBuf R;
     R.alloc(hr.size() * 8);
     check_hip(hipMemcpyAsync(R.ptr, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, st), "copy");
-    ridge_core(F, np, A0, R, alphas, n_alpha, delta, info, (size_t)np);
+    // the factors plm_normal_ridge kept, when they belong to this state and scaling
+    FactorCache& fc = acc.factors;
+    const bool same = fc.n_rows == acc.n_rows && fc.scales.size() == (size_t)F &&
+                      std::equal(fc.scales.begin(), fc.scales.end(), scales) &&
+                      std::equal(fc.zero.begin(), fc.zero.end(), z.begin());
+    ridge_core(F, np, A0, R, alphas, n_alpha, delta, info, (size_t)np, same ? &fc : nullptr);
     if (dad) {
         const std::vector<double> zb(np, 0.0);
         sq_errors(F, np, A0, zb, 0.0, delta, n_alpha, dad);
