@@ -232,6 +232,8 @@ constexpr int GI_PB = 3 * GI_RB * GI_TJ * 2;     // 16-byte B pieces per stage
 constexpr int GI_LA = (GI_PA + GI_NT - 1) / GI_NT;  // A pieces per thread
 constexpr int GI_LB = (GI_PB + GI_NT - 1) / GI_NT;  // B pieces per thread
 constexpr int GI_MFMA = 9 * GI_RB;               // MFMAs per wave and stage
+static_assert(GI_PA % 64 == 0 && GI_PB % 64 == 0 && GI_TJ % 32 == 0,
+              "staging pieces come in runs of 64 (32 columns x 2 halves)");
 
 // Tile order: workgroups are dealt to the 8 XCDs round robin; each XCD runs
 // one super-tile of 4 x 8 tiles (512 x 512 outputs) at a time, so the panels
@@ -276,13 +278,23 @@ __global__ void __launch_bounds__(GI_NT) gram_i8_kernel(const signed char* __res
 #pragma unroll
     for (int v = 0; v < 16; ++v) sacc[v] = 0.0;
     // piece p -> ((s * GI_RB + rbo) * cols + col) * 2 + half
+    // within each run of 64 pieces, pieces 0-31 are the first 16-byte halves of
+    // 32 consecutive columns and pieces 32-63 their second halves, so the 16
+    // lanes of one LDS store pass write 16 columns of one half (distinct banks
+    // at the 48-byte column stride, as the operand reads)
+    auto piece = [&](int p, int cols, int& half, int& col, int& sr) {
+        const int q = (p >> 6) * 32 + (p & 31);
+        half = (p >> 5) & 1, col = q % cols, sr = q / cols;
+    };
     auto g_off = [&](int p, int cols, int col0, int st) -> size_t {
-        const int half = p & 1, col = (p >> 1) % cols, sr = (p >> 1) / cols;
+        int half, col, sr;
+        piece(p, cols, half, col, sr);
         const int s = sr / GI_RB, rbo = sr % GI_RB;
         return s * plane + ((size_t)(st * GI_RB + rbo) * ld + col0 + col) * kQK + 16 * half;
     };
     auto l_off = [&](int p, int cols) -> int {
-        const int half = p & 1, col = (p >> 1) % cols, sr = (p >> 1) / cols;
+        int half, col, sr;
+        piece(p, cols, half, col, sr);
         return (sr * cols + col) * GI_CS + 16 * half;
     };
     // thread pieces past a panel's end wrap around (branch-free): such a
