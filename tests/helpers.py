@@ -109,3 +109,45 @@ def x_parity(X, Xr, noise, resolved=1e-12):
     floor = np.maximum(colmax, 1e-9 * np.median(colmax))
     loose = float((err / floor)[~ok].max()) if (~ok).any() else 0.0
     return strict, loose, int((~ok).sum())
+
+
+# --- the mixed Gram's quantisation, restated in numpy
+# (pypolymlp_amd/csrc/kernels_gram.hip: colexp_kernel, quantize_kernel)
+QUANT_CHUNK = 4096
+
+
+def quant_digits(x, e):
+    """Three signed digits of x at exponent e, x~ = 2^(e-7) (d0 + d1/128 + d2/16384)."""
+    m = np.ldexp(x, 7 - e)
+    t0 = np.trunc(m)
+    r1 = (m - t0) * 128
+    t1 = np.trunc(r1)
+    t2 = np.clip(np.rint((r1 - t1) * 128), -127, 127)
+    return t0, t1, t2
+
+
+def quantised_gram(A, bounds, max_level=4):
+    """Gram of A quantised per (row chunk, column): the smallest exponent e with
+    |x| < 2^e over the chunk, three digits each; the sum of the digit products
+    d_s^T d_t with s + t <= max_level (4: all nine, the exact Gram of the
+    quantised matrix). Each product is an exact integer in FP64."""
+    F1 = A.shape[1]
+    G = np.zeros((F1, F1))
+    for a, b in bounds:
+        blk = A[a:b]
+        m = np.abs(blk).max(axis=0)
+        e = np.where(m > 0, np.frexp(m)[1], 0)
+        d = quant_digits(blk, e[None, :])
+        ee = e[:, None] + e[None, :] - 14
+        for s in range(3):
+            for t in range(3):
+                if s + t <= max_level:
+                    G += np.ldexp(d[s].T @ d[t], ee) * 2.0 ** (-7 * (s + t))
+    return G
+
+
+def quant_chunks(n_es, n_rows):
+    """Row chunks of a quantisation order that puts the n_es energy and stress
+    rows first: one chunk of them, then force-row chunks of QUANT_CHUNK rows."""
+    return [(0, n_es)] + [(a, min(a + QUANT_CHUNK, n_rows))
+                          for a in range(n_es, n_rows, QUANT_CHUNK)]

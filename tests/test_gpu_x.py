@@ -17,7 +17,8 @@ is made on fitted predictions (meV/atom, meV/A) in test_gpu_solve.py.
 
 X^T X / X^T y / y^T y: normwise 1e-10 (FP64 MFMA) against numpy on the FP64
 X; the mixed form (exact Gram of the FP32 X quantised to three 7-bit slices)
-within 1e-5 and positive semidefinite to FP64 rounding.
+within 1e-5 and positive semidefinite to FP64 rounding; the int8 Gram kernel
+equal to the numpy restatement of the quantised Gram to FP64 rounding.
 """
 
 import os
@@ -229,6 +230,57 @@ def test_normal_equations_cfg1():
     ev = np.linalg.eigvalsh(G * d[:, None] * d[None, :])
     say(f"  mixed [X|y] Gram, unit diagonal: eigenvalue min {ev.min():.2e} max {ev.max():.2e}")
     assert ev.min() > -1e-10 * ev.max()
+
+
+def test_int8_gram_is_exact_gram_of_quantised_x():
+    """The int8 Gram kernel against the numpy restatement of its quantisation
+    (tests.helpers.quantised_gram): through PLM_MIXED64 (FP64 X, the same
+    quantisation and Gram kernels as PLM_MIXED) with unit row weights, X^T X,
+    X^T y and y^T y equal the exact Gram of [X | y] quantised per (chunk,
+    column), the energy and stress rows in a chunk of their own and the force
+    rows in batch order in 4096-row chunks (normal.cpp's quantisation order),
+    to FP64 rounding: 1e-13 of sqrt(G_ii G_jj) per entry. A dropped or
+    misplaced slice product, a wrong exponent or a transposed tile is off by
+    1e-8 or more of the Gram (the quantisation error itself is 5e-8). The X the
+    accumulation builds can differ from plm_features_x's in the last bit (its
+    row sums use atomics); a last digit that flips with it moves one row and
+    column of the Gram by up to 2^-21 of its scale, so the per-entry bound holds
+    on 99.9 % of the entries and the normwise difference is bounded at 1e-10."""
+    from tests.helpers import quant_chunks, quantised_gram
+    ds = datasets.config1_dataset(30)  # 4,860 force rows: a full and a partial chunk
+    m = capi.Model(datasets.config1_model().as_dict())
+    axes, pos, types = _structs(ds.structures)
+    b = capi.Batch(m, axes, pos, types)
+    say("int8 Gram vs quantised Gram: FP64 X of 30 cfg1 structures")
+    X = capi.features_x(m, b, capi.PLM_FP64)
+    rng = np.random.default_rng(11)
+    y = rng.normal(size=X.shape[0])
+    acc = capi.Normal(m)
+    acc.accumulate(b, np.ones(X.shape[0]), y, capi.PLM_MIXED64)
+    xtx, xty, _, _, ysq, _ = acc.result()
+    # quantisation order: the 7 energy and stress rows of every structure, then
+    # every force row in batch order
+    order = b.row_order()
+    ref_of_batch = np.empty_like(order)
+    ref_of_batch[order] = np.arange(len(order))
+    n_es = 7 * len(ds.structures)
+    starts = np.concatenate([[0], np.cumsum(7 + 3 * np.asarray(b.n_atoms))])
+    force_batch = np.concatenate([np.arange(s + 7, e) for s, e in zip(starts[:-1], starts[1:])])
+    rows = np.concatenate([np.arange(n_es), ref_of_batch[force_batch]])
+    A = np.concatenate([X, y[:, None]], axis=1)[rows]
+    G = quantised_gram(A, quant_chunks(n_es, A.shape[0]))
+    F = X.shape[1]
+    Gk = np.empty_like(G)
+    Gk[:F, :F], Gk[:F, F], Gk[F, :F], Gk[F, F] = xtx, xty, xty, ysq
+    d = np.sqrt(np.maximum(np.diag(G), 1e-300))
+    rel = np.abs(Gk - G) / (d[:, None] * d[None, :])
+    norm = np.linalg.norm(Gk - G) / np.linalg.norm(G)
+    exact = np.linalg.norm(A.T @ A - G) / np.linalg.norm(G)
+    say(f"  per entry: 99.9 % {np.quantile(rel, 0.999):.1e}, max {rel.max():.1e}; "
+        f"normwise {norm:.1e} (quantisation error {exact:.1e})")
+    assert np.quantile(rel, 0.999) <= 1e-13
+    assert norm <= 1e-10
+    assert exact > 1e-9  # the comparison distinguishes quantised from exact
 
 
 @pytest.mark.parametrize("prec", [capi.PLM_FP64, capi.PLM_MIXED])
