@@ -309,8 +309,8 @@ struct EvalGraph {
     ~EvalGraph();
 };
 void enqueue_eval_dev(ModelHandle& m, const BatchDev& bd, const int* hb_off, const int* hb_j,
-                      const double* hb_d, bool fp64, void* a_buf, void* pse, void* psf,
-                      double* out, size_t n_out, hipStream_t st);
+                      const double* hb_d, int n_half, bool fp64, void* a_buf, void* pse,
+                      void* psf, double* out, size_t n_out, hipStream_t st);
 void eval_graph_capture(EvalGraph& g);
 void eval_graph_run(EvalGraph& g, int steps, double* energies, double* forces, double* stresses);
 void eval_single(ModelHandle& m, const double* axis, const double* pos_c, const int* types, int n,

@@ -56,20 +56,6 @@ struct TypeDev {
     const double* fq_c;
     const float* fq_cf;
     const int* lin_global;     // [n_lin] global linear feature id
-    // eval: potential terms per noconj head
-    const int* pot_off;        // [n_noconj+1]
-    const double* pot_ce;
-    const double* pot_cf;
-    const int* pot_prod;       // index into prod_deriv
-    const int* pot_pf;         // index into prod_features
-    const int* pd_keys;        // [n_prod_deriv*3] -1 padded (eval products)
-    int n_prod_deriv;
-    const int* pf_keys;        // [n_prod_features*3] -1 padded: products of features
-    int n_prod_features;
-    // eval: noconj heads grouped by tp with (n_id, ylm, m==0, sign_j, jlocal)
-    const int* ev_jlocal;      // [n_noconj] jlocal_noconj of each noconj head (per tp list)
-    const float* ev_signj;     // [n_noconj]
-    const unsigned char* ev_m0;// [n_noconj]
 };
 
 struct ModelDev {

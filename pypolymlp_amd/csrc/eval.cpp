@@ -61,8 +61,28 @@ void upload_potential(ModelHandle& m) {
             using V = typename std::decay_t<decltype(v)>::value_type;
             return m.pot_store.upload(v.data(), v.size() * sizeof(V));
         };
-        E.pot_off = upv(off), E.pot_ce = upv(ce), E.pot_cf = upv(cf);
-        E.pot_pd = upv(pd), E.pot_pf = upv(pf);
+        E.pot_off = upv(off);
+        // interleaved lane chunks of the contraction (keys and heads fit 16 bits:
+        // the order parameters and features they index are held in LDS)
+        const long n_terms = (long)ce.size();
+        const long iters = (n_terms + 63) / 64;
+        std::vector<int4> pq_key((size_t)std::max(1L, iters) * 64, int4{-1, -1, -1, 0});
+        std::vector<double2> pq_c(pq_key.size(), double2{0.0, 0.0});
+        auto k16 = [](int v) -> unsigned {
+            if (v >= 0xffff) throw std::invalid_argument("evaluation table index exceeds 16 bits");
+            return v < 0 ? 0xffffu : (unsigned)v;
+        };
+        for (int h = 0; h < E.n_heads; ++h)
+            for (int e = off[h]; e < off[h + 1]; ++e) {
+                const size_t r = (size_t)(e % iters) * 64 + e / iters;
+                pq_key[r] = int4{(int)(k16(pd[3 * e]) | k16(pd[3 * e + 1]) << 16),
+                                 (int)(k16(pd[3 * e + 2]) | k16(h) << 16),
+                                 (int)(k16(pf[2 * e]) | k16(pf[2 * e + 1]) << 16), 0};
+                pq_c[r] = double2{ce[e], cf[e]};
+            }
+        E.pq_iters = (int)iters;
+        E.pq_key = static_cast<const int4*>(upv(pq_key));
+        E.pq_c = static_cast<const double2*>(upv(pq_c));
         E.hd_off = upv(hoff), E.hd_nid = upv(nid), E.hd_ylm = upv(ylm);
         E.hd_i = upv(hi), E.hd_j = upv(hj), E.hd_signj = upv(sj), E.hd_m0 = upv(m0);
     }
@@ -93,8 +113,8 @@ static void alloc_eval(const ModelHandle& m, const BatchHandle& b, bool fp64, De
 // node when captured), then filled by the half-pair loop. bd carries the full
 // neighbour list (order parameters); hb_* the half list (pair loop).
 void enqueue_eval_dev(ModelHandle& m, const BatchDev& bd, const int* hb_off, const int* hb_j,
-                      const double* hb_d, bool fp64, void* a_buf, void* pse, void* psf,
-                      double* out, size_t n_out, hipStream_t st) {
+                      const double* hb_d, int n_half, bool fp64, void* a_buf, void* pse,
+                      void* psf, double* out, size_t n_out, hipStream_t st) {
     const int A = bd.n_atoms_total;
     launch_fill_zero(st, out, n_out);
     double* d_e = out;
@@ -114,7 +134,7 @@ void enqueue_eval_dev(ModelHandle& m, const BatchDev& bd, const int* hb_off, con
     check_hip(hipGetLastError(), "prodsum launch");
     {
         ProfScope ps("eval_pair", st);
-        launch_pair(st, fp64, m.md, m.ed, bd, bd.atom_st, hb_off, hb_j, hb_d, pse, psf,
+        launch_pair(st, fp64, m.md, m.ed, bd, bd.atom_st, hb_off, hb_j, hb_d, n_half, pse, psf,
                     std::max(1, m.max_heads), d_e, d_f, d_s, pair_model);
     }
     check_hip(hipGetLastError(), "pair launch");
@@ -123,7 +143,8 @@ void enqueue_eval_dev(ModelHandle& m, const BatchDev& bd, const int* hb_off, con
 static void enqueue_eval(ModelHandle& m, BatchHandle& b, bool fp64, void* a_buf, void* pse,
                          void* psf, double* out, size_t n_out, hipStream_t st) {
     enqueue_eval_dev(m, b.view(), (const int*)b.d_hb_off.ptr, (const int*)b.d_hb_j.ptr,
-                     (const double*)b.d_hb_d.ptr, fp64, a_buf, pse, psf, out, n_out, st);
+                     (const double*)b.d_hb_d.ptr, (int)b.n_half, fp64, a_buf, pse, psf, out,
+                     n_out, st);
 }
 
 static void copy_out(const BatchHandle& b, const DevBuf& out, size_t n_out, hipStream_t st,

@@ -177,7 +177,8 @@ void eval_single_noalloc(ModelHandle& m, const double* axis, const double* pos_c
     bd.nb_off = d_nb_off;
     bd.nb_j = d_nb_j;
     bd.nb_d = d_nb_d;
-    enqueue_eval_dev(m, bd, d_hb_off, d_hb_j, d_hb_d, fp64, a_buf, pse, psf, out, n_out, st);
+    enqueue_eval_dev(m, bd, d_hb_off, d_hb_j, d_hb_d, off_h[n], fp64, a_buf, pse, psf, out, n_out,
+                     st);
     std::vector<double> h(n_out);
     check_hip(hipMemcpyAsync(h.data(), out, n_out * 8, hipMemcpyDeviceToHost, st), "copy");
     check_hip(hipStreamSynchronize(st), "sync");

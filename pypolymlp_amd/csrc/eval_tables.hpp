@@ -7,11 +7,17 @@ namespace plm {
 
 struct EvalTypeDev {
     int n_heads;            // non-conjugate components (gtinv) or pair components
-    const int* pot_off;     // [n_heads+1]
-    const double* pot_ce;   // energy coefficient of each term
-    const double* pot_cf;   // force coefficient of each term
-    const int* pot_pd;      // [nterm*3] order-parameter keys of the product, -1 padded
-    const int* pot_pf;      // [nterm*2] feature keys of the product of features, -1 padded
+    const int* pot_off;     // [n_heads+1] first term of each head
+    // The terms (coefficients and the order-parameter and feature keys of their
+    // products, ordered by head) for the per-atom contraction, dealt to the 64
+    // lanes of a wave in equal contiguous chunks of pq_iters terms and
+    // interleaved: record i * 64 + l is the i-th term of lane l's chunk (past
+    // the chunk: padding, head 0xffff). Keys and head in 16-bit fields,
+    // 0xffff: none:
+    //   (pd0 | pd1 << 16, pd2 | head << 16, pf0 | pf1 << 16, 0), coefficients (e, f).
+    int pq_iters;
+    const int4* pq_key;
+    const double2* pq_c;
     // half-pair contraction: heads of this type grouped by type pair
     const int* hd_off;      // [n_tp+1]
     const int* hd_nid;

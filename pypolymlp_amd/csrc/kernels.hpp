@@ -132,9 +132,10 @@ void launch_prodsum(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev
                     int ps_stride, int max_lin, int pair_model);
 // x[0, n) = 0 by a kernel (a graph of it replays correctly; a memset node is avoided)
 void launch_fill_zero(hipStream_t st, double* x, size_t n);
+// half-pair loop over the n_half pairs of the half list (hb_off[n_atoms] = n_half)
 void launch_pair(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,
                  const BatchDev& bd, const int* atom_st, const int* hb_off, const int* hb_j,
-                 const double* hb_d, const void* ps_e, const void* ps_f, int ps_stride,
+                 const double* hb_d, int n_half, const void* ps_e, const void* ps_f, int ps_stride,
                  double* energy, double* forces, double* stress, int pair_model);
 
 // X^T X with the weighted targets in column F of X (ld >= F + 1, multiple of

@@ -19,22 +19,19 @@
 
 namespace plm {
 
-template <class T>
-__device__ __forceinline__ Cpx<T> prod_keys3(const Cpx<T>* a, const int* keys) {
-    Cpx<T> v = {T(1), T(0)};
-    for (int k = 0; k < 3; ++k) {
-        const int id = keys[k];
-        if (id < 0) break;
-        v = cmul(v, a[id]);
-    }
-    return v;
-}
+void check_hip(hipError_t e, const char* what);
 
 // ---------------------------------------------------------------------------
 // Per atom: features, products of features, and
 //   prod_sum_{e,f}[h] = sum_terms coeff_{e,f} * prod_features * prod_anlm
 // One 64-lane workgroup per atom; the atom's order parameters and features
-// staged in LDS (the term products gather from them at random keys).
+// staged in LDS (the term products gather from them at random keys). The
+// terms of all heads are dealt to the lanes in equal contiguous chunks
+// (every lane does the same number of terms, whatever the heads' lengths),
+// read as one 16-byte key record and one coefficient pair per term, one
+// coalesced load per wave and iteration. A lane writes the heads that begin
+// and end in its chunk; the heads that cross a chunk boundary are summed from
+// the lanes' boundary slots in LDS.
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, BatchDev bd,
@@ -43,8 +40,11 @@ __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, Ba
                                                      Cpx<T>* __restrict__ ps_f, int ps_stride,
                                                      int pair_model) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    // the atom's order parameters (a_stride complex values), then its features
-    Cpx<T>* a = reinterpret_cast<Cpx<T>*>(smem_raw);
+    // two slots per lane of head sums (4 values each) and their heads, the
+    // atom's order parameters (a_stride complex values), then its features
+    double* sv = reinterpret_cast<double*>(smem_raw);
+    int* sh = reinterpret_cast<int*>(sv + 4 * 128);
+    Cpx<T>* a = reinterpret_cast<Cpx<T>*>(sh + 128);
     T* xs = reinterpret_cast<T*>(a + a_stride);
     const int i = blockIdx.x;
     if (i >= bd.n_atoms_total) return;
@@ -66,126 +66,188 @@ __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, Ba
         xs[f] = v;
     }
     __syncthreads();
+    const int lane = threadIdx.x;
     Cpx<T>* oe = ps_e + (size_t)i * ps_stride;
     Cpx<T>* of = ps_f + (size_t)i * ps_stride;
-    for (int h = threadIdx.x; h < E.n_heads; h += blockDim.x) {
-        T er = T(0), ei = T(0), fr = T(0), fi = T(0);
-        for (int e = E.pot_off[h]; e < E.pot_off[h + 1]; ++e) {
-            const int* pk = E.pot_pf + 2 * (size_t)e;
-            T fval = T(1);
-            if (pk[0] >= 0) fval *= xs[pk[0]];
-            if (pk[1] >= 0) fval *= xs[pk[1]];
-            const Cpx<T> pr = prod_keys3<T>(a, E.pot_pd + 3 * (size_t)e);
-            const T se = fval * T(E.pot_ce[e]), sf = fval * T(E.pot_cf[e]);
-            er += pr.re * se, ei += pr.im * se;
-            fr += pr.re * sf, fi += pr.im * sf;
+    const double scale = pair_model ? 0.5 : 1.0;
+    // a head that begins and ends inside the lane's chunk is written at once;
+    // the chunk's first and last heads (which may continue in the neighbouring
+    // chunks) go to the lane's two slots, summed over lanes afterwards
+    int cur = -1, first = -1;
+    T er = T(0), ei = T(0), fr = T(0), fi = T(0);
+    auto to_slot = [&](int sl) {
+        sh[sl] = cur;
+        sv[4 * sl] = double(er), sv[4 * sl + 1] = double(ei);
+        sv[4 * sl + 2] = double(fr), sv[4 * sl + 3] = double(fi);
+    };
+    sh[2 * lane] = sh[2 * lane + 1] = -1;
+    for (int h = lane; h < E.n_heads; h += blockDim.x)
+        if (E.pot_off[h + 1] == E.pot_off[h]) oe[h] = {T(0), T(0)}, of[h] = {T(0), T(0)};
+    for (int it = 0; it < E.pq_iters; ++it) {
+        const int4 k = E.pq_key[(size_t)it * 64 + lane];
+        const double2 c = E.pq_c[(size_t)it * 64 + lane];
+        const int head = (unsigned)k.y >> 16;
+        if (head == 0xffff) break;  // the rest of this lane's chunk is padding
+        if (head != cur) {
+            if (cur == first && cur >= 0) {
+                to_slot(2 * lane);
+            } else if (cur >= 0) {
+                oe[cur] = {T(scale * er), T(scale * ei)};
+                of[cur] = {T(scale * fr), T(scale * fi)};
+            }
+            if (first < 0) first = head;
+            cur = head, er = ei = fr = fi = T(0);
         }
-        if (pair_model) er *= T(0.5), ei *= T(0.5), fr *= T(0.5), fi *= T(0.5);
-        oe[h] = {er, ei};
-        of[h] = {fr, fi};
+        const int pd0 = k.x & 0xffff, pd1 = (unsigned)k.x >> 16, pd2 = k.y & 0xffff;
+        const int pf0 = k.z & 0xffff, pf1 = (unsigned)k.z >> 16;
+        T fval = T(1);
+        if (pf0 != 0xffff) fval *= xs[pf0];
+        if (pf1 != 0xffff) fval *= xs[pf1];
+        Cpx<T> pr = {T(1), T(0)};
+        if (pd0 != 0xffff) {
+            pr = a[pd0];
+            if (pd1 != 0xffff) {
+                pr = cmul(pr, a[pd1]);
+                if (pd2 != 0xffff) pr = cmul(pr, a[pd2]);
+            }
+        }
+        const T se = fval * T(c.x), sf = fval * T(c.y);
+        er += pr.re * se, ei += pr.im * se;
+        fr += pr.re * sf, fi += pr.im * sf;
+    }
+    if (cur >= 0) to_slot(cur == first ? 2 * lane : 2 * lane + 1);
+    __syncthreads();
+    // slots in term order, empty ones skipped: the first slot of each head sums
+    // the slots of that head and writes it
+    for (int sl = 2 * lane; sl < 2 * lane + 2; ++sl) {
+        const int h = sh[sl];
+        if (h < 0) continue;
+        int p = sl - 1;
+        while (p >= 0 && sh[p] < 0) --p;
+        if (p >= 0 && sh[p] == h) continue;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        for (int q = sl; q < 128 && (sh[q] == h || sh[q] < 0); ++q) {
+            if (sh[q] < 0) continue;
+            s0 += sv[4 * q], s1 += sv[4 * q + 1], s2 += sv[4 * q + 2], s3 += sv[4 * q + 3];
+        }
+        oe[h] = {T(scale * s0), T(scale * s1)};
+        of[h] = {T(scale * s2), T(scale * s3)};
     }
 }
 
 // ---------------------------------------------------------------------------
-// Half-pair loop: one lane per pair, radial + Y_lm (+ gradients) in registers
-// bounded by LB (max l) and NB (radial functions).
+// Half-pair loop: one lane per half pair (i, j) of the whole list, radial +
+// Y_lm (+ gradients) in registers bounded by LB (max l) and NB (radial
+// functions). A lane finds its atom i from the half-list offsets; the forces
+// on i and j go out as atomics per lane, the energy and stress with one
+// atomic per wave when the wave's pairs belong to one structure.
 // ---------------------------------------------------------------------------
 template <class T, int LB, int NB>
-__global__ void __launch_bounds__(64) pair_kernel(ModelDev md, EvalDev ed, BatchDev bd,
-                                                  const int* __restrict__ atom_st,
-                                                  const int* __restrict__ hb_off,
-                                                  const int* __restrict__ hb_j,
-                                                  const double* __restrict__ hb_d,
-                                                  const Cpx<T>* __restrict__ ps_e,
-                                                  const Cpx<T>* __restrict__ ps_f, int ps_stride,
-                                                  double* __restrict__ energy,
-                                                  double* __restrict__ forces,
-                                                  double* __restrict__ stress, int pair_model) {
+__global__ void __launch_bounds__(256) pair_kernel(ModelDev md, EvalDev ed, BatchDev bd,
+                                                   const int* __restrict__ atom_st,
+                                                   const int* __restrict__ hb_off,
+                                                   const int* __restrict__ hb_j,
+                                                   const double* __restrict__ hb_d, int n_half,
+                                                   const Cpx<T>* __restrict__ ps_e,
+                                                   const Cpx<T>* __restrict__ ps_f, int ps_stride,
+                                                   double* __restrict__ energy,
+                                                   double* __restrict__ forces,
+                                                   double* __restrict__ stress, int pair_model) {
     constexpr int NLMH = (LB + 1) * (LB + 2) / 2;
-    const int i = blockIdx.x;
-    if (i >= bd.n_atoms_total) return;
-    const int t1 = bd.types[i];
-    const EvalTypeDev& E = ed.t[t1];
-    const int p0 = hb_off[i], p1 = hb_off[i + 1];
-    double e_acc = 0.0, s_acc[6] = {0, 0, 0, 0, 0, 0};
-    double fi_acc[3] = {0, 0, 0};
-    const Cpx<T>* Pi_e = ps_e + (size_t)i * ps_stride;
-    const Cpx<T>* Pi_f = ps_f + (size_t)i * ps_stride;
-    for (int k = p0 + threadIdx.x; k < p1; k += blockDim.x) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double eij = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    int st = -1;
+    if (k < n_half) {
+        // atom i of pair k: hb_off[i] <= k < hb_off[i + 1]
+        int lo = 0, hi = bd.n_atoms_total;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (hb_off[mid] <= k) lo = mid;
+            else hi = mid;
+        }
+        const int i = lo;
+        st = atom_st[i];
+        const int t1 = bd.types[i];
+        const EvalTypeDev& E = ed.t[t1];
+        const Cpx<T>* Pi_e = ps_e + (size_t)i * ps_stride;
+        const Cpx<T>* Pi_f = ps_f + (size_t)i * ps_stride;
         const int j = hb_j[k];
         // diff_ij: r_i - r_j - T
-        const double dx = -hb_d[3 * (size_t)k], dy = -hb_d[3 * (size_t)k + 1],
-                     dz = -hb_d[3 * (size_t)k + 2];
+        dx = -hb_d[3 * (size_t)k], dy = -hb_d[3 * (size_t)k + 1], dz = -hb_d[3 * (size_t)k + 2];
         const double dis = sqrt(dx * dx + dy * dy + dz * dz);
-        if (dis >= md.cutoff) continue;
-        const int tp = md.tp_of[t1 * md.n_type + bd.types[j]];
-        T fn[NB], fd[NB];
-        radial<T>(T(dis), T(md.cutoff), md.rad + (size_t)tp * md.n_fn * 2, md.rad_count[tp], fn, fd);
-        const Cpx<T>* Pj_e = ps_e + (size_t)j * ps_stride;
-        const Cpx<T>* Pj_f = ps_f + (size_t)j * ps_stride;
-        double eij = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-        const int h0 = E.hd_off[tp], h1 = E.hd_off[tp + 1];
-        if (pair_model) {
-            T ef = T(0), ff = T(0);
-            for (int h = h0; h < h1; ++h) {
-                const int nid = E.hd_nid[h];
-                const int hi = E.hd_i[h], hj = E.hd_j[h];
-                ef += fn[nid] * (Pi_e[hi].re + Pj_e[hj].re);
-                ff += fd[nid] * (Pi_f[hi].re + Pj_f[hj].re);
+        if (dis < md.cutoff) {
+            const int tp = md.tp_of[t1 * md.n_type + bd.types[j]];
+            T fn[NB], fd[NB];
+            radial<T>(T(dis), T(md.cutoff), md.rad + (size_t)tp * md.n_fn * 2, md.rad_count[tp], fn, fd);
+            const Cpx<T>* Pj_e = ps_e + (size_t)j * ps_stride;
+            const Cpx<T>* Pj_f = ps_f + (size_t)j * ps_stride;
+            const int h0 = E.hd_off[tp], h1 = E.hd_off[tp + 1];
+            if (pair_model) {
+                T ef = T(0), ff = T(0);
+                for (int h = h0; h < h1; ++h) {
+                    const int nid = E.hd_nid[h];
+                    const int hi = E.hd_i[h], hj = E.hd_j[h];
+                    ef += fn[nid] * (Pi_e[hi].re + Pj_e[hj].re);
+                    ff += fd[nid] * (Pi_f[hi].re + Pj_f[hj].re);
+                }
+                eij = double(ef);
+                const double fij = -double(ff) / dis;
+                fx = fij * dx, fy = fij * dy, fz = fij * dz;
+            } else {
+                Cpx<T> Y[NLMH], dY[3 * NLMH];
+                T pbuf[NLMH], qbuf[NLMH];
+                ylm<T>(md.max_l, T(dx), T(dy), T(dz), T(dis), Y, dY, pbuf, qbuf);
+                const T ir = T(1) / T(dis);
+                T ea = T(0), fxa = T(0), fya = T(0), fza = T(0);
+                for (int h = h0; h < h1; ++h) {
+                    const int nid = E.hd_nid[h];
+                    const T fv = fn[nid];
+                    if (fv < T(1e-20)) continue;
+                    const int key = E.hd_ylm[h];
+                    const int hi = E.hd_i[h], hj = E.hd_j[h];
+                    const T sj = T(E.hd_signj[h]);
+                    const T w = E.hd_m0[h] ? T(0.5) : T(1);
+                    const Cpx<T> se = {(Pi_e[hi].re + Pj_e[hj].re * sj) * w,
+                                       (Pi_e[hi].im + Pj_e[hj].im * sj) * w};
+                    const Cpx<T> sf = {(Pi_f[hi].re + Pj_f[hj].re * sj) * w,
+                                       (Pi_f[hi].im + Pj_f[hj].im * sj) * w};
+                    const Cpx<T> y = Y[key];
+                    ea += fv * (y.re * se.re - y.im * se.im);
+                    const T d1r = fd[nid] * y.re * ir, d1i = fd[nid] * y.im * ir;
+                    // val_a = -(d1 * d_a + fn * dY_a)
+                    const Cpx<T> vx = {-(d1r * T(dx) + fv * dY[3 * key].re), -(d1i * T(dx) + fv * dY[3 * key].im)};
+                    const Cpx<T> vy = {-(d1r * T(dy) + fv * dY[3 * key + 1].re), -(d1i * T(dy) + fv * dY[3 * key + 1].im)};
+                    const Cpx<T> vz = {-(d1r * T(dz) + fv * dY[3 * key + 2].re), -(d1i * T(dz) + fv * dY[3 * key + 2].im)};
+                    fxa += re_mul(vx, sf);
+                    fya += re_mul(vy, sf);
+                    fza += re_mul(vz, sf);
+                }
+                eij = double(ea), fx = double(fxa), fy = double(fya), fz = double(fza);
             }
-            eij = double(ef);
-            const double fij = -double(ff) / dis;
-            fx = fij * dx, fy = fij * dy, fz = fij * dz;
-        } else {
-            Cpx<T> Y[NLMH], dY[3 * NLMH];
-            T pbuf[NLMH], qbuf[NLMH];
-            ylm<T>(md.max_l, T(dx), T(dy), T(dz), T(dis), Y, dY, pbuf, qbuf);
-            const T ir = T(1) / T(dis);
-            T ea = T(0), fxa = T(0), fya = T(0), fza = T(0);
-            for (int h = h0; h < h1; ++h) {
-                const int nid = E.hd_nid[h];
-                const T fv = fn[nid];
-                if (fv < T(1e-20)) continue;
-                const int key = E.hd_ylm[h];
-                const int hi = E.hd_i[h], hj = E.hd_j[h];
-                const T sj = T(E.hd_signj[h]);
-                const T w = E.hd_m0[h] ? T(0.5) : T(1);
-                const Cpx<T> se = {(Pi_e[hi].re + Pj_e[hj].re * sj) * w,
-                                   (Pi_e[hi].im + Pj_e[hj].im * sj) * w};
-                const Cpx<T> sf = {(Pi_f[hi].re + Pj_f[hj].re * sj) * w,
-                                   (Pi_f[hi].im + Pj_f[hj].im * sj) * w};
-                const Cpx<T> y = Y[key];
-                ea += fv * (y.re * se.re - y.im * se.im);
-                const T d1r = fd[nid] * y.re * ir, d1i = fd[nid] * y.im * ir;
-                // val_a = -(d1 * d_a + fn * dY_a)
-                const Cpx<T> vx = {-(d1r * T(dx) + fv * dY[3 * key].re), -(d1i * T(dx) + fv * dY[3 * key].im)};
-                const Cpx<T> vy = {-(d1r * T(dy) + fv * dY[3 * key + 1].re), -(d1i * T(dy) + fv * dY[3 * key + 1].im)};
-                const Cpx<T> vz = {-(d1r * T(dz) + fv * dY[3 * key + 2].re), -(d1i * T(dz) + fv * dY[3 * key + 2].im)};
-                fxa += re_mul(vx, sf);
-                fya += re_mul(vy, sf);
-                fza += re_mul(vz, sf);
-            }
-            eij = double(ea), fx = double(fxa), fy = double(fya), fz = double(fza);
+            atomicAdd(forces + 3 * (size_t)i, fx);
+            atomicAdd(forces + 3 * (size_t)i + 1, fy);
+            atomicAdd(forces + 3 * (size_t)i + 2, fz);
+            atomicAdd(forces + 3 * (size_t)j, -fx);
+            atomicAdd(forces + 3 * (size_t)j + 1, -fy);
+            atomicAdd(forces + 3 * (size_t)j + 2, -fz);
         }
-        e_acc += eij;
-        fi_acc[0] += fx, fi_acc[1] += fy, fi_acc[2] += fz;
-        atomicAdd(forces + 3 * (size_t)j, -fx);
-        atomicAdd(forces + 3 * (size_t)j + 1, -fy);
-        atomicAdd(forces + 3 * (size_t)j + 2, -fz);
-        s_acc[0] += dx * fx, s_acc[1] += dy * fy, s_acc[2] += dz * fz;
-        s_acc[3] += dx * fy, s_acc[4] += dy * fz, s_acc[5] += dz * fx;
     }
-    // wave reduction (64 lanes)
-    for (int off = 32; off > 0; off >>= 1) {
-        e_acc += __shfl_down(e_acc, off);
-        for (int a = 0; a < 3; ++a) fi_acc[a] += __shfl_down(fi_acc[a], off);
-        for (int a = 0; a < 6; ++a) s_acc[a] += __shfl_down(s_acc[a], off);
-    }
-    if (threadIdx.x == 0) {
-        const int s = atom_st[i];
-        atomicAdd(energy + s, e_acc);
-        for (int a = 0; a < 3; ++a) atomicAdd(forces + 3 * (size_t)i + a, fi_acc[a]);
-        for (int a = 0; a < 6; ++a) atomicAdd(stress + 6 * (size_t)s + a, s_acc[a]);
+    double sv[7] = {eij, dx * fx, dy * fy, dz * fz, dx * fy, dy * fz, dz * fx};
+    const int s0 = __builtin_amdgcn_readfirstlane(st);
+    if (s0 < 0) return;  // lane 0 past the end: so is the whole wave
+    if (__all(st == s0 || st < 0)) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int a = 0; a < 7; ++a) sv[a] += __shfl_down(sv[a], off);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(energy + s0, sv[0]);
+            for (int a = 0; a < 6; ++a) atomicAdd(stress + 6 * (size_t)s0 + a, sv[1 + a]);
+        }
+    } else if (st >= 0) {
+        atomicAdd(energy + st, sv[0]);
+        for (int a = 0; a < 6; ++a) atomicAdd(stress + 6 * (size_t)st + a, sv[1 + a]);
     }
 }
 
@@ -193,10 +255,12 @@ template <class T>
 void launch_prodsum_t(hipStream_t st, const ModelDev& md, const EvalDev& ed, const BatchDev& bd,
                       const void* a_full, int a_stride, void* ps_e, void* ps_f, int ps_stride,
                       int max_lin, int pair_model) {
-    const size_t sm = (size_t)a_stride * 2 * sizeof(T) + (size_t)std::max(1, max_lin) * sizeof(T);
+    const size_t sm = 128 * (4 * sizeof(double) + sizeof(int)) + (size_t)a_stride * 2 * sizeof(T) +
+                      (size_t)std::max(1, max_lin) * sizeof(T);
     if (sm > 160 * 1024) throw std::invalid_argument("per-atom evaluation data exceed LDS");
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&prodsum_kernel<T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&prodsum_kernel<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm),
+              "prodsum shared memory");
     prodsum_kernel<T><<<bd.n_atoms_total, 64, sm, st>>>(
         md, ed, bd, reinterpret_cast<const Cpx<T>*>(a_full), a_stride,
         reinterpret_cast<Cpx<T>*>(ps_e), reinterpret_cast<Cpx<T>*>(ps_f), ps_stride, pair_model);
@@ -216,28 +280,29 @@ void launch_prodsum(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev
 template <class T, int LB, int NB>
 void launch_pair_lb(hipStream_t st, const ModelDev& md, const EvalDev& ed, const BatchDev& bd,
                     const int* atom_st, const int* hb_off, const int* hb_j, const double* hb_d,
-                    const void* ps_e, const void* ps_f, int ps_stride, double* energy,
+                    int n_half, const void* ps_e, const void* ps_f, int ps_stride, double* energy,
                     double* forces, double* stress, int pair_model) {
-    pair_kernel<T, LB, NB><<<bd.n_atoms_total, 64, 0, st>>>(
-        md, ed, bd, atom_st, hb_off, hb_j, hb_d, reinterpret_cast<const Cpx<T>*>(ps_e),
+    if (n_half <= 0) return;
+    pair_kernel<T, LB, NB><<<(n_half + 255) / 256, 256, 0, st>>>(
+        md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, reinterpret_cast<const Cpx<T>*>(ps_e),
         reinterpret_cast<const Cpx<T>*>(ps_f), ps_stride, energy, forces, stress, pair_model);
 }
 
 template <class T>
 void launch_pair_t(hipStream_t st, const ModelDev& md, const EvalDev& ed, const BatchDev& bd,
                    const int* atom_st, const int* hb_off, const int* hb_j, const double* hb_d,
-                   const void* ps_e, const void* ps_f, int ps_stride, double* energy,
+                   int n_half, const void* ps_e, const void* ps_f, int ps_stride, double* energy,
                    double* forces, double* stress, int pair_model) {
     const int L = md.max_l;
     if (md.n_fn > 32) throw std::invalid_argument("evaluation supports at most 32 radial functions");
     if (L <= 4)
-        launch_pair_lb<T, 4, 32>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, ps_e, ps_f, ps_stride,
+        launch_pair_lb<T, 4, 32>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f, ps_stride,
                                  energy, forces, stress, pair_model);
     else if (L <= 8)
-        launch_pair_lb<T, 8, 32>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, ps_e, ps_f, ps_stride,
+        launch_pair_lb<T, 8, 32>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f, ps_stride,
                                  energy, forces, stress, pair_model);
     else if (L <= 12)
-        launch_pair_lb<T, 12, 32>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, ps_e, ps_f, ps_stride,
+        launch_pair_lb<T, 12, 32>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f, ps_stride,
                                   energy, forces, stress, pair_model);
     else
         throw std::invalid_argument("evaluation supports max_l <= 12");
@@ -245,13 +310,13 @@ void launch_pair_t(hipStream_t st, const ModelDev& md, const EvalDev& ed, const 
 
 void launch_pair(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,
                  const BatchDev& bd, const int* atom_st, const int* hb_off, const int* hb_j,
-                 const double* hb_d, const void* ps_e, const void* ps_f, int ps_stride,
+                 const double* hb_d, int n_half, const void* ps_e, const void* ps_f, int ps_stride,
                  double* energy, double* forces, double* stress, int pair_model) {
     if (fp64)
-        launch_pair_t<double>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, ps_e, ps_f, ps_stride,
+        launch_pair_t<double>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f, ps_stride,
                               energy, forces, stress, pair_model);
     else
-        launch_pair_t<float>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, ps_e, ps_f, ps_stride,
+        launch_pair_t<float>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f, ps_stride,
                              energy, forces, stress, pair_model);
 }
 
