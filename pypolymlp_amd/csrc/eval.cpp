@@ -83,8 +83,14 @@ void upload_potential(ModelHandle& m) {
         E.pq_iters = (int)iters;
         E.pq_key = static_cast<const int4*>(upv(pq_key));
         E.pq_c = static_cast<const double2*>(upv(pq_c));
-        E.hd_off = upv(hoff), E.hd_nid = upv(nid), E.hd_ylm = upv(ylm);
-        E.hd_i = upv(hi), E.hd_j = upv(hj), E.hd_signj = upv(sj), E.hd_m0 = upv(m0);
+        std::vector<int4> rec(std::max<size_t>(1, nid.size()), int4{0, 0, 0, 0});
+        for (size_t h = 0; h < nid.size(); ++h) {
+            if (hj[h] < 0 || hj[h] >= (1 << 28)) throw std::invalid_argument("head index range");
+            rec[h] = int4{nid[h], ylm[h], hi[h],
+                          hj[h] | (sj[h] < 0.f ? 1 << 28 : 0) | (m0[h] ? 1 << 29 : 0)};
+        }
+        E.hd_off = upv(hoff);
+        E.hd_rec = static_cast<const int4*>(upv(rec));
     }
     m.pot_uploaded = true;
     ++m.pot_version;

@@ -20,12 +20,10 @@ struct EvalTypeDev {
     const double2* pq_c;
     // half-pair contraction: heads of this type grouped by type pair
     const int* hd_off;      // [n_tp+1]
-    const int* hd_nid;
-    const int* hd_ylm;
-    const int* hd_i;        // ilocal_noconj of the head (atom i)
-    const int* hd_j;        // jlocal_noconj (same component seen from atom j)
-    const float* hd_signj;  // (-1)^l
-    const unsigned char* hd_m0;  // m == 0 (weight 1/2)
+    // per head, one 16-byte record: (radial n_id, ylm key, ilocal_noconj of the
+    // head (atom i), jlocal_noconj (the same component seen from atom j) with
+    // bit 28 set where (-1)^l = -1 and bit 29 where m == 0 (weight 1/2))
+    const int4* hd_rec;
 };
 
 struct EvalDev {

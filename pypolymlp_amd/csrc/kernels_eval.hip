@@ -186,8 +186,8 @@ __global__ void __launch_bounds__(256) pair_kernel(ModelDev md, EvalDev ed, Batc
             if (pair_model) {
                 T ef = T(0), ff = T(0);
                 for (int h = h0; h < h1; ++h) {
-                    const int nid = E.hd_nid[h];
-                    const int hi = E.hd_i[h], hj = E.hd_j[h];
+                    const int4 hr = E.hd_rec[h];
+                    const int nid = hr.x, hi = hr.z, hj = hr.w & 0x0fffffff;
                     ef += fn[nid] * (Pi_e[hi].re + Pj_e[hj].re);
                     ff += fd[nid] * (Pi_f[hi].re + Pj_f[hj].re);
                 }
@@ -201,13 +201,14 @@ __global__ void __launch_bounds__(256) pair_kernel(ModelDev md, EvalDev ed, Batc
                 const T ir = T(1) / T(dis);
                 T ea = T(0), fxa = T(0), fya = T(0), fza = T(0);
                 for (int h = h0; h < h1; ++h) {
-                    const int nid = E.hd_nid[h];
+                    const int4 hr = E.hd_rec[h];
+                    const int nid = hr.x;
                     const T fv = fn[nid];
                     if (fv < T(1e-20)) continue;
-                    const int key = E.hd_ylm[h];
-                    const int hi = E.hd_i[h], hj = E.hd_j[h];
-                    const T sj = T(E.hd_signj[h]);
-                    const T w = E.hd_m0[h] ? T(0.5) : T(1);
+                    const int key = hr.y;
+                    const int hi = hr.z, hj = hr.w & 0x0fffffff;
+                    const T sj = (hr.w >> 28) & 1 ? T(-1) : T(1);
+                    const T w = (hr.w >> 29) & 1 ? T(0.5) : T(1);
                     const Cpx<T> se = {(Pi_e[hi].re + Pj_e[hj].re * sj) * w,
                                        (Pi_e[hi].im + Pj_e[hj].im * sj) * w};
                     const Cpx<T> sf = {(Pi_f[hi].re + Pj_f[hj].re * sj) * w,
