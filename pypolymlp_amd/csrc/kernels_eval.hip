@@ -10,12 +10,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 
 #include "device_math.hpp"
 #include "device_tables.hpp"
 #include "eval_tables.hpp"
 #include "kernels.hpp"
+#include "pair_data.hpp"
 
 namespace plm {
 
@@ -136,14 +139,155 @@ __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, Ba
 }
 
 // ---------------------------------------------------------------------------
-// Half-pair loop: one lane per half pair (i, j) of the whole list, radial +
-// Y_lm (+ gradients) in registers bounded by LB (max l) and NB (radial
-// functions). A lane finds its atom i from the half-list offsets; the forces
-// on i and j go out as atomics per lane, the energy and stress with one
-// atomic per wave when the wave's pairs belong to one structure.
+// Half-pair loop, record form: one 64-lane workgroup per atom i. The records of up to
+// kEvPairs of i's half pairs at a time (radial functions and derivatives,
+// Y_lm and its gradient for the reversed vector r_i - r_j - T) are computed
+// cooperatively in LDS (pairs_coop, the same arithmetic as the per-lane
+// radial / ylm); then 8 lanes per pair split the pair's heads (lane slice s
+// takes every 8th head) and the pair's sums are reduced over its 8 lanes.
+// Nothing is indexed per lane at run time outside LDS, so no private arrays.
+// ---------------------------------------------------------------------------
+constexpr int kEvPairs = 8;  // pairs per chunk, 8 lanes each
+
+template <class T>
+__global__ void __launch_bounds__(64) pair_rec_kernel(ModelDev md, EvalDev ed, BatchDev bd,
+                                                  const int* __restrict__ atom_st,
+                                                  const int* __restrict__ hb_off,
+                                                  const int* __restrict__ hb_j,
+                                                  const double* __restrict__ hb_d,
+                                                  const Cpx<T>* __restrict__ ps_e,
+                                                  const Cpx<T>* __restrict__ ps_f, int ps_stride,
+                                                  double* __restrict__ energy,
+                                                  double* __restrict__ forces,
+                                                  double* __restrict__ stress, int pair_model) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    T* pairs = reinterpret_cast<T*>(smem_raw);
+    const int i = blockIdx.x;
+    if (i >= bd.n_atoms_total) return;
+    const int p0 = hb_off[i], p1 = hb_off[i + 1];
+    if (p0 == p1) return;
+    const int t1 = bd.types[i];
+    const EvalTypeDev& E = ed.t[t1];
+    const PairLayout<T> L(md.n_fn, md.n_lmh);
+    BatchDev hb = bd;  // the records are built from the half list
+    hb.nb_j = hb_j;
+    hb.nb_d = hb_d;
+    const int lane = threadIdx.x, pl = lane >> 3, sl = lane & 7;
+    const Cpx<T>* Pi_e = ps_e + (size_t)i * ps_stride;
+    const Cpx<T>* Pi_f = ps_f + (size_t)i * ps_stride;
+    double e_acc = 0.0, fi_acc[3] = {0, 0, 0}, s_acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int c0 = p0; c0 < p1; c0 += kEvPairs) {
+        const int np = min(kEvPairs, p1 - c0);
+        __syncthreads();  // the previous chunk's records are read
+        pairs_coop<T>(md, L, pairs, np, c0, hb, t1, 0, true, -1.0, pair_model ? -1 : md.max_l);
+        const int k = c0 + pl;
+        T* slot = pairs + (size_t)min(pl, np - 1) * L.stride;
+        const T* ms = L.misc(slot);
+        const bool live = pl < np && ms[4] != T(0);
+        // pair model: ea = energy sum, fxa = radial-derivative sum
+        T ea = T(0), fxa = T(0), fya = T(0), fza = T(0);
+        int j = 0;
+        if (live) {
+            const int tp = int(ms[5]);
+            j = hb_j[k];
+            const T* fn = L.fn(slot);
+            const T* fd = L.fd(slot);
+            const Cpx<T>* Pj_e = ps_e + (size_t)j * ps_stride;
+            const Cpx<T>* Pj_f = ps_f + (size_t)j * ps_stride;
+            const int h0 = E.hd_off[tp], h1 = E.hd_off[tp + 1];
+            if (pair_model) {
+                for (int h = h0 + sl; h < h1; h += 8) {
+                    const int4 hr = E.hd_rec[h];
+                    const int nid = hr.x, hi = hr.z, hj = hr.w & 0x0fffffff;
+                    ea += fn[nid] * (Pi_e[hi].re + Pj_e[hj].re);
+                    fxa += fd[nid] * (Pi_f[hi].re + Pj_f[hj].re);
+                }
+            } else {
+                const Cpx<T>* Y = L.Y(slot);
+                const Cpx<T>* dY = L.dY(slot);
+                const T dx = ms[0], dy = ms[1], dz = ms[2];
+                const T ir = T(1) / ms[3];
+                for (int h = h0 + sl; h < h1; h += 8) {
+                    const int4 hr = E.hd_rec[h];
+                    const int nid = hr.x;
+                    const T fv = fn[nid];
+                    if (fv < T(1e-20)) continue;
+                    const int key = hr.y;
+                    const int hi = hr.z, hj = hr.w & 0x0fffffff;
+                    const T sj = (hr.w >> 28) & 1 ? T(-1) : T(1);
+                    const T w = (hr.w >> 29) & 1 ? T(0.5) : T(1);
+                    const Cpx<T> se = {(Pi_e[hi].re + Pj_e[hj].re * sj) * w,
+                                       (Pi_e[hi].im + Pj_e[hj].im * sj) * w};
+                    const Cpx<T> sf = {(Pi_f[hi].re + Pj_f[hj].re * sj) * w,
+                                       (Pi_f[hi].im + Pj_f[hj].im * sj) * w};
+                    const Cpx<T> y = Y[key];
+                    ea += fv * (y.re * se.re - y.im * se.im);
+                    const T d1r = fd[nid] * y.re * ir, d1i = fd[nid] * y.im * ir;
+                    // val_a = -(d1 * d_a + fn * dY_a)
+                    const Cpx<T> vx = {-(d1r * dx + fv * dY[3 * key].re), -(d1i * dx + fv * dY[3 * key].im)};
+                    const Cpx<T> vy = {-(d1r * dy + fv * dY[3 * key + 1].re), -(d1i * dy + fv * dY[3 * key + 1].im)};
+                    const Cpx<T> vz = {-(d1r * dz + fv * dY[3 * key + 2].re), -(d1i * dz + fv * dY[3 * key + 2].im)};
+                    fxa += re_mul(vx, sf);
+                    fya += re_mul(vy, sf);
+                    fza += re_mul(vz, sf);
+                }
+            }
+        }
+        // the pair's sums over its 8 lanes
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1) {
+            ea += __shfl_xor(ea, off);
+            fxa += __shfl_xor(fxa, off);
+            fya += __shfl_xor(fya, off);
+            fza += __shfl_xor(fza, off);
+        }
+        if (live && sl == 0) {
+            // diff_ij: r_i - r_j - T, in FP64 for the forces' directions and the stress
+            const double dx = -hb_d[3 * (size_t)k], dy = -hb_d[3 * (size_t)k + 1],
+                         dz = -hb_d[3 * (size_t)k + 2];
+            const double eij = double(ea);
+            double fx, fy, fz;
+            if (pair_model) {
+                const double fij = -double(fxa) / sqrt(dx * dx + dy * dy + dz * dz);
+                fx = fij * dx, fy = fij * dy, fz = fij * dz;
+            } else {
+                fx = double(fxa), fy = double(fya), fz = double(fza);
+            }
+            e_acc += eij;
+            fi_acc[0] += fx, fi_acc[1] += fy, fi_acc[2] += fz;
+            atomicAdd(forces + 3 * (size_t)j, -fx);
+            atomicAdd(forces + 3 * (size_t)j + 1, -fy);
+            atomicAdd(forces + 3 * (size_t)j + 2, -fz);
+            s_acc[0] += dx * fx, s_acc[1] += dy * fy, s_acc[2] += dz * fz;
+            s_acc[3] += dx * fy, s_acc[4] += dy * fz, s_acc[5] += dz * fx;
+        }
+    }
+    // wave reduction (64 lanes)
+    for (int off = 32; off > 0; off >>= 1) {
+        e_acc += __shfl_down(e_acc, off);
+        for (int a = 0; a < 3; ++a) fi_acc[a] += __shfl_down(fi_acc[a], off);
+        for (int a = 0; a < 6; ++a) s_acc[a] += __shfl_down(s_acc[a], off);
+    }
+    if (lane == 0) {
+        const int s = atom_st[i];
+        atomicAdd(energy + s, e_acc);
+        for (int a = 0; a < 3; ++a) atomicAdd(forces + 3 * (size_t)i + a, fi_acc[a]);
+        for (int a = 0; a < 6; ++a) atomicAdd(stress + 6 * (size_t)s + a, s_acc[a]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Half-pair loop, per-lane form: one lane per half pair (i, j) of the whole
+// list, radial + Y_lm (+ gradients) in private arrays bounded by LB (max l) and
+// NB (radial functions). A lane finds its atom i from the half-list offsets;
+// the forces on i and j go out as atomics per lane, the energy and stress with
+// one atomic per wave when the wave's pairs belong to one structure. Faster
+// than the record form (below) where the private arrays are small enough to
+// stay in the L1 cache (FP32, max_l <= 4: 496 bytes per lane); in FP64 they
+// take 1.7 KB per lane and the record form is faster.
 // ---------------------------------------------------------------------------
 template <class T, int LB, int NB>
-__global__ void __launch_bounds__(256) pair_kernel(ModelDev md, EvalDev ed, BatchDev bd,
+__global__ void __launch_bounds__(256) pair_lane_kernel(ModelDev md, EvalDev ed, BatchDev bd,
                                                    const int* __restrict__ atom_st,
                                                    const int* __restrict__ hb_off,
                                                    const int* __restrict__ hb_j,
@@ -278,24 +422,41 @@ void launch_prodsum(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev
                                 pair_model);
 }
 
+template <class T>
+void launch_pair_rec_t(hipStream_t st, const ModelDev& md, const EvalDev& ed, const BatchDev& bd,
+                   const int* atom_st, const int* hb_off, const int* hb_j, const double* hb_d,
+                   const void* ps_e, const void* ps_f, int ps_stride, double* energy,
+                   double* forces, double* stress, int pair_model) {
+    const size_t sm = (size_t)kEvPairs * PairLayout<T>(md.n_fn, md.n_lmh).stride * sizeof(T);
+    if (sm > 160 * 1024) throw std::invalid_argument("evaluation pair records exceed LDS");
+    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&pair_rec_kernel<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm),
+              "pair shared memory");
+    pair_rec_kernel<T><<<bd.n_atoms_total, 64, sm, st>>>(
+        md, ed, bd, atom_st, hb_off, hb_j, hb_d, reinterpret_cast<const Cpx<T>*>(ps_e),
+        reinterpret_cast<const Cpx<T>*>(ps_f), ps_stride, energy, forces, stress, pair_model);
+}
+
 template <class T, int LB, int NB>
 void launch_pair_lb(hipStream_t st, const ModelDev& md, const EvalDev& ed, const BatchDev& bd,
                     const int* atom_st, const int* hb_off, const int* hb_j, const double* hb_d,
                     int n_half, const void* ps_e, const void* ps_f, int ps_stride, double* energy,
                     double* forces, double* stress, int pair_model) {
     if (n_half <= 0) return;
-    pair_kernel<T, LB, NB><<<(n_half + 255) / 256, 256, 0, st>>>(
+    pair_lane_kernel<T, LB, NB><<<(n_half + 255) / 256, 256, 0, st>>>(
         md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, reinterpret_cast<const Cpx<T>*>(ps_e),
         reinterpret_cast<const Cpx<T>*>(ps_f), ps_stride, energy, forces, stress, pair_model);
 }
 
 template <class T>
-void launch_pair_t(hipStream_t st, const ModelDev& md, const EvalDev& ed, const BatchDev& bd,
-                   const int* atom_st, const int* hb_off, const int* hb_j, const double* hb_d,
-                   int n_half, const void* ps_e, const void* ps_f, int ps_stride, double* energy,
-                   double* forces, double* stress, int pair_model) {
+void launch_pair_lane_t(hipStream_t st, const ModelDev& md, const EvalDev& ed,
+                        const BatchDev& bd, const int* atom_st, const int* hb_off,
+                        const int* hb_j, const double* hb_d, int n_half, const void* ps_e,
+                        const void* ps_f, int ps_stride, double* energy, double* forces,
+                        double* stress, int pair_model) {
     const int L = md.max_l;
-    if (md.n_fn > 32) throw std::invalid_argument("evaluation supports at most 32 radial functions");
+    if (md.n_fn > 32)
+        throw std::invalid_argument("the per-lane pair loop supports at most 32 radial functions");
     if (L <= 4)
         launch_pair_lb<T, 4, 32>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f, ps_stride,
                                  energy, forces, stress, pair_model);
@@ -306,19 +467,35 @@ void launch_pair_t(hipStream_t st, const ModelDev& md, const EvalDev& ed, const 
         launch_pair_lb<T, 12, 32>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f, ps_stride,
                                   energy, forces, stress, pair_model);
     else
-        throw std::invalid_argument("evaluation supports max_l <= 12");
+        throw std::invalid_argument("the per-lane pair loop supports max_l <= 12");
 }
 
+// The record form, except in FP32 with max_l <= 4 (the per-lane form);
+// PLM_EVAL_PAIR=lane / records forces one (the per-lane form needs max_l <= 12
+// and at most 32 radial functions).
 void launch_pair(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,
                  const BatchDev& bd, const int* atom_st, const int* hb_off, const int* hb_j,
                  const double* hb_d, int n_half, const void* ps_e, const void* ps_f, int ps_stride,
                  double* energy, double* forces, double* stress, int pair_model) {
-    if (fp64)
-        launch_pair_t<double>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f, ps_stride,
-                              energy, forces, stress, pair_model);
+    if (n_half <= 0) return;
+    bool lane = !fp64 && md.max_l <= 4;
+    if (const char* e = getenv("PLM_EVAL_PAIR")) {
+        if (!strcmp(e, "lane")) lane = true;
+        else if (!strcmp(e, "records")) lane = false;
+        else throw std::invalid_argument("PLM_EVAL_PAIR: lane or records");
+    }
+    if (lane && fp64)
+        launch_pair_lane_t<double>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f,
+                                   ps_stride, energy, forces, stress, pair_model);
+    else if (lane)
+        launch_pair_lane_t<float>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f,
+                                  ps_stride, energy, forces, stress, pair_model);
+    else if (fp64)
+        launch_pair_rec_t<double>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, ps_e, ps_f,
+                                  ps_stride, energy, forces, stress, pair_model);
     else
-        launch_pair_t<float>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f, ps_stride,
-                             energy, forces, stress, pair_model);
+        launch_pair_rec_t<float>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, ps_e, ps_f,
+                                 ps_stride, energy, forces, stress, pair_model);
 }
 
 __global__ void __launch_bounds__(256) fill_zero_kernel(double* __restrict__ x, size_t n) {

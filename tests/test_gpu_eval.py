@@ -24,6 +24,19 @@ def say(msg):
     sys.stdout.flush()
 
 
+# the half-pair loop's two forms (records in LDS, per-lane arrays); "auto" is
+# the library's choice
+PAIR_FORMS = ["auto", "lane", "records"]
+
+
+def _pair_form(form, monkeypatch):
+    if form == "auto":
+        monkeypatch.delenv("PLM_EVAL_PAIR", raising=False)
+    else:
+        monkeypatch.setenv("PLM_EVAL_PAIR", form)
+
+
+@pytest.mark.parametrize("form", PAIR_FORMS)
 @pytest.mark.parametrize(
     "model,e_true,f_true,s_true",
     [
@@ -31,12 +44,13 @@ def say(msg):
         ("polymlp.yaml.gtinv.MgO", -40.223320043232334, MGO_GTINV_F, MGO_GTINV_S),
     ],
 )
-def test_eval_mgo_golden(model, e_true, f_true, s_true):
+def test_eval_mgo_golden(model, e_true, f_true, s_true, form, monkeypatch):
+    _pair_form(form, monkeypatch)
     spec, coeffs = load_model(model)
     m = capi.Model(spec.as_dict())
     m.set_coeffs(coeffs)
     st = poscar("POSCAR.RS.MgO")
-    say(f"eval model={model}")
+    say(f"eval model={model} pair form={form}")
     e, f, s = capi.eval_structure(m, st.axis, st.positions_cartesian, st.types)
     assert e == pytest.approx(e_true, rel=1e-8)
     np.testing.assert_allclose(f, f_true, atol=1e-6)
@@ -50,8 +64,10 @@ def test_eval_mgo_golden(model, e_true, f_true, s_true):
 
 
 @pytest.mark.skipif(not ref.available(), reason="oracle library not built")
+@pytest.mark.parametrize("form", PAIR_FORMS)
 @pytest.mark.parametrize("cfg", ["cfg1", "cfg2"])
-def test_eval_configs_vs_oracle(cfg, tmp_path):
+def test_eval_configs_vs_oracle(cfg, form, tmp_path, monkeypatch):
+    _pair_form(form, monkeypatch)
     if cfg == "cfg1":
         spec, ds = datasets.config1_model(), datasets.config1_dataset(3)
     else:
@@ -65,7 +81,7 @@ def test_eval_configs_vs_oracle(cfg, tmp_path):
     sts = ds.structures
     b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
                    [s.types for s in sts])
-    say(f"eval batch {cfg} FP64")
+    say(f"eval batch {cfg} FP64, pair form {form}")
     E, F, S = capi.eval_batch(m, b, capi.PLM_FP64)
     say(f"eval batch {cfg} MIXED")
     Em, Fm, Sm = capi.eval_batch(m, b, capi.PLM_MIXED)
