@@ -14,6 +14,8 @@
 
 namespace plm {
 
+void check_hip(hipError_t e, const char* what);
+
 // ---------------------------------------------------------------------------
 // Neighbour list: one thread per atom i, brute force over (j, T) in the
 // reference's loop order (j ascending, translations in host order). The
@@ -189,6 +191,92 @@ __global__ void __launch_bounds__(256) anlm_kernel(ModelDev md, BatchDev bd,
     }
 }
 
+// Order parameters with register sums (evaluation, at most 64 Q heads per
+// type): thread t owns the entries t + 64 q (q < Q) of the atom type's list
+// grouped by type pair, holds their type pair, radial index, Y key and head
+// in registers for the whole atom, and adds f_n Y_lm of every pair of its
+// entries' type pair into a register sum; the sums go to their heads at the
+// end. Pairs are taken in the same order as anlm_kernel, so the sums are the
+// same; the entry tables are read once per atom instead of once per pair and
+// the LDS read-modify-write per pair and entry is gone.
+template <class T, int Q>
+__global__ void __launch_bounds__(64) anlm_reg_kernel(ModelDev md, BatchDev bd,
+                                                      Cpx<T>* __restrict__ a_full, int a_stride,
+                                                      int atom0, int n_atoms, int negate) {
+    extern __shared__ unsigned char smem_raw[];
+    const int ia = blockIdx.x;
+    if (ia >= n_atoms) return;
+    const int i = atom0 + ia;
+    const int t1 = bd.types[i];
+    const TypeDev& TT = md.t[t1];
+    const PairLayout<T> L(md.n_fn, md.n_lmh);
+    const int NPS = kAnlmPairs;
+    T* pairs = reinterpret_cast<T*>(smem_raw);
+    Cpx<T>* acc = reinterpret_cast<Cpx<T>*>(pairs + (size_t)NPS * L.stride);
+    const int tid = threadIdx.x;
+    const int n_nc = TT.nc_tp_off[md.n_tp];
+    int etp[Q], enid[Q], ekey[Q], ehd[Q];
+    Cpx<T> a[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int h = tid + 64 * q;
+        etp[q] = -1, enid[q] = 0, ekey[q] = 0, ehd[q] = 0;
+        a[q] = {T(0), T(0)};
+        if (h < n_nc) {
+            int tp = 0;
+            while (TT.nc_tp_off[tp + 1] <= h) ++tp;
+            etp[q] = tp, enid[q] = TT.nc_tp_nid[h], ekey[q] = TT.nc_tp_ylm[h];
+            ehd[q] = TT.nc_tp_head[h];
+        }
+    }
+    for (int h = tid; h < TT.n_noconj; h += blockDim.x) acc[h] = {T(0), T(0)};
+    const int p0 = bd.nb_off[i], p1 = bd.nb_off[i + 1];
+    const int lmax = md.max_l;
+    for (int c0 = p0; c0 < p1; c0 += NPS) {
+        const int np = min(NPS, p1 - c0);
+        __syncthreads();
+        // X path: d = r_j - r_i + T (local.cpp:149); evaluation: the reversed
+        // vector r_i - r_j - T (polymlp_eval.cpp:376-379)
+        pairs_coop<T>(md, L, pairs, np, c0, bd, t1, 0, false, negate ? -1.0 : 1.0, lmax);
+        for (int p = 0; p < np; ++p) {
+            T* slot = pairs + (size_t)p * L.stride;
+            const T* m = L.misc(slot);
+            if (m[4] == T(0)) continue;
+            const int tp = int(m[5]);
+            const T* fn = L.fn(slot);
+            const Cpx<T>* Y = L.Y(slot);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if (etp[q] != tp) continue;
+                const T f = fn[enid[q]];
+                if (f < T(1e-20)) continue;
+                if (ekey[q] < 0) {
+                    a[q].re += f;
+                } else {
+                    const Cpx<T> y = Y[ekey[q]];
+                    a[q].re += f * y.re;
+                    a[q].im += f * y.im;
+                }
+            }
+        }
+    }
+    __syncthreads();  // the zeroed heads, before the entries' sums land on theirs
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (etp[q] >= 0) acc[ehd[q]] = a[q];
+    __syncthreads();
+    Cpx<T>* out = a_full + (size_t)ia * a_stride;
+    for (int k = threadIdx.x; k < TT.n_nlm; k += blockDim.x) {
+        const Cpx<T> v = acc[TT.nlm_src[k]];
+        if (TT.nlm_conj[k]) {
+            const T cc = T(TT.nlm_sign[k]);
+            out[k] = {cc * v.re, -cc * v.im};
+        } else {
+            out[k] = v;
+        }
+    }
+}
+
 // explicit instantiation via launchers
 void launch_neighbor_full(hipStream_t st, int n, const int* atom_st, const int* st_atom_off,
                           const double* pos, const int* tr_off, const double* trans, double cut2,
@@ -215,13 +303,33 @@ size_t anlm_smem(const ModelDev& md, int elem, int nthreads, int max_noconj) {
     return (size_t)nthreads * L.stride * elem + (size_t)max_noconj * 2 * elem;
 }
 
+template <class T, int Q>
+void launch_anlm_reg(hipStream_t st, const ModelDev& md, const BatchDev& bd, void* a_full,
+                     int a_stride, int atom0, int n_atoms, size_t sm, int negate) {
+    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_reg_kernel<T, Q>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm),
+              "anlm shared memory");
+    anlm_reg_kernel<T, Q><<<n_atoms, 64, sm, st>>>(md, bd, reinterpret_cast<Cpx<T>*>(a_full),
+                                                   a_stride, atom0, n_atoms, negate);
+}
+
+// register sums up to 512 heads per type (8 per thread), LDS sums beyond
 template <class T>
 void launch_anlm_t(hipStream_t st, const ModelDev& md, const BatchDev& bd, void* a_full,
                    int a_stride, int atom0, int n_atoms, int max_noconj, int negate) {
     const int B = 64;
     const size_t sm = anlm_smem(md, sizeof(T), kAnlmPairs, max_noconj);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_kernel<T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (max_noconj <= 4 * 64) {
+        launch_anlm_reg<T, 4>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+        return;
+    }
+    if (max_noconj <= 8 * 64) {
+        launch_anlm_reg<T, 8>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+        return;
+    }
+    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_kernel<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm),
+              "anlm shared memory");
     anlm_kernel<T><<<n_atoms, B, sm, st>>>(md, bd, reinterpret_cast<Cpx<T>*>(a_full), a_stride,
                                            atom0, n_atoms, negate);
 }
