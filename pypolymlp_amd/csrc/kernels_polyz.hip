@@ -172,7 +172,20 @@ __global__ void __launch_bounds__(kRowThreads) poly_rows_kernel(PolyDev pd, Batc
     for (int e0 = 0; e0 < pd.E; e0 += CH) {
         const int e1 = min(pd.E, e0 + CH);
         __syncthreads();
-        for (int e = e0 + tid; e < e1; e += kRowThreads) zr[e - e0] = zs[e];
+        // the row of Z into LDS, eight loads in flight per thread
+        for (int e = e0 + tid; e < e1; e += 8 * kRowThreads) {
+            T v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int ee = e + k * kRowThreads;
+                v[k] = ee < e1 ? zs[ee] : T(0);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int ee = e + k * kRowThreads;
+                if (ee < e1) zr[ee - e0] = v[k];
+            }
+        }
         __syncthreads();
         auto part = [&](const int4 p) {
             T v = T(0);
