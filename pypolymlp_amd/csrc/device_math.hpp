@@ -28,6 +28,34 @@ __device__ __forceinline__ Cpx<T> cmul(Cpx<T> a, Cpx<T> b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
+// dst[k] = src[k] for k < n complex values, strided over the workgroup, eight
+// loads in flight per thread (as the vector type of two T)
+template <class T> struct Vec2Of;
+template <> struct Vec2Of<float> { using type = float2; };
+template <> struct Vec2Of<double> { using type = double2; };
+template <class T> struct Vec4Of;
+template <> struct Vec4Of<float> { using type = float4; };
+template <> struct Vec4Of<double> { using type = double4; };
+template <class T>
+__device__ __forceinline__ void stage_cpx(Cpx<T>* dst, const Cpx<T>* __restrict__ src, int n) {
+    using V = typename Vec2Of<T>::type;
+    const V* s = reinterpret_cast<const V*>(src);
+    V* d = reinterpret_cast<V*>(dst);
+    for (int k = threadIdx.x; k < n; k += 8 * blockDim.x) {
+        V v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = k + u * blockDim.x;
+            v[u] = kk < n ? s[kk] : V{};
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = k + u * blockDim.x;
+            if (kk < n) d[kk] = v[u];
+        }
+    }
+}
+
 template <class T> struct MathConst;
 template <> struct MathConst<double> {
     static constexpr double pi = 3.1415926535897932384626433832795;

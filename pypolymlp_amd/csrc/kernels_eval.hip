@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, Ba
     const TypeDev& TT = md.t[t1];
     const EvalTypeDev& E = ed.t[t1];
     const Cpx<T>* ag = a_full + (size_t)i * a_stride;
-    for (int k = threadIdx.x; k < TT.n_nlm; k += blockDim.x) a[k] = ag[k];
+    stage_cpx(a, ag, TT.n_nlm);
     __syncthreads();
     for (int f = threadIdx.x; f < TT.n_lin; f += blockDim.x) {
         T v = T(0);

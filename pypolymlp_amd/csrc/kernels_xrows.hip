@@ -157,10 +157,22 @@ __global__ void __launch_bounds__(256) anlm_rec_kernel(ModelDev md, BatchDev bd,
         for (int c0 = to[tp]; c0 < to[tp + 1]; c0 += kArPairs) {
             const int np = min(kArPairs, to[tp + 1] - c0);
             __syncthreads();
-            for (int e = tid; e < np * W; e += blockDim.x) {
-                const int p = e / W, o = e % W;
-                const T* rec = pd + (size_t)(c0 + p - pair_base) * R.stride;
-                fL[e] = o < nfn ? rec[R.fn() + o] : rec[R.ylm((o - nfn) >> 1) + ((o - nfn) & 1)];
+            // eight loads in flight per thread
+            for (int e = tid; e < np * W; e += 8 * blockDim.x) {
+                T v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int ee = e + u * blockDim.x;
+                    if (ee >= np * W) break;
+                    const int p = ee / W, o = ee % W;
+                    const T* rec = pd + (size_t)(c0 + p - pair_base) * R.stride;
+                    v[u] = o < nfn ? rec[R.fn() + o] : rec[R.ylm((o - nfn) >> 1) + ((o - nfn) & 1)];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int ee = e + u * blockDim.x;
+                    if (ee < np * W) fL[ee] = v[u];
+                }
             }
             __syncthreads();
             for (int h = h0 + tid; h < h1; h += blockDim.x) {
@@ -238,7 +250,7 @@ __global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
     const TypeDev& TT = md.t[ti];
     const Cpx<T>* ag = a_full + (size_t)ia * a_stride;
     const int one = TT.n_nlm;
-    for (int k = threadIdx.x; k < TT.n_nlm; k += blockDim.x) a[k] = ag[k];
+    stage_cpx(a, ag, TT.n_nlm);
     if (threadIdx.x == 0) a[one] = {T(1), T(0)};
     __syncthreads();
     Cpx<T>* wo = w + (size_t)ia * w_stride;
@@ -251,10 +263,19 @@ __global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
         for (int bk = threadIdx.x >> 6; bk < TT.wq_nblk; bk += nwave) {
             const int g = TT.wq_item[bk * 64 + lane];
             T vr = T(0), vi = T(0);
-            for (int r = TT.wq_boff[bk] + lane; r < TT.wq_boff[bk + 1]; r += 64) {
-                const Cpx<T> pr = prod_keys<T, NF>(a, k4[r], one);
-                vr += cc[2 * r] * pr.re;
-                vi += cc[2 * r + 1] * pr.im;
+            // the next row's key record and coefficients load while this
+            // row's order parameters are gathered
+            const int r1 = TT.wq_boff[bk + 1];
+            int r = TT.wq_boff[bk] + lane;
+            int4 kn = r < r1 ? k4[r] : int4{-1, -1, -1, -1};
+            T cn0 = r < r1 ? cc[2 * r] : T(0), cn1 = r < r1 ? cc[2 * r + 1] : T(0);
+            for (; r < r1; r += 64) {
+                const int4 kk = kn;
+                const T c0 = cn0, c1 = cn1;
+                if (r + 64 < r1) kn = k4[r + 64], cn0 = cc[2 * (r + 64)], cn1 = cc[2 * (r + 64) + 1];
+                const Cpx<T> pr = prod_keys<T, NF>(a, kk, one);
+                vr += c0 * pr.re;
+                vi += c1 * pr.im;
             }
             if (g >= 0) wo[g] = Cpx<T>{vr, vi};
         }
@@ -268,10 +289,17 @@ __global__ void __launch_bounds__(256) wprime_kernel(ModelDev md, BatchDev bd,
         for (int bk = threadIdx.x >> 6; bk < TT.fq_nblk; bk += nwave) {
             const int f = TT.fq_item[bk * 64 + lane];
             T v = T(0);
-            for (int r = TT.fq_boff[bk] + lane; r < TT.fq_boff[bk + 1]; r += 64) {
-                const T pr = prod_keys<T, NV>(a, k4[r], one).re;
+            const int r1 = TT.fq_boff[bk + 1];
+            int r = TT.fq_boff[bk] + lane;
+            int4 kn = r < r1 ? k4[r] : int4{-1, -1, -1, -1};
+            T cn = r < r1 ? c[r] : T(0);
+            for (; r < r1; r += 64) {
+                const int4 kk = kn;
+                const T c0 = cn;
+                if (r + 64 < r1) kn = k4[r + 64], cn = c[r + 64];
+                const T pr = prod_keys<T, NV>(a, kk, one).re;
                 if (TT.fv_skip && fabs(pr) <= T(1e-20)) continue;
-                v += c[r] * pr;
+                v += c0 * pr;
             }
             if (f >= 0) xo[f] = v;
         }
