@@ -83,16 +83,29 @@ __global__ void __launch_bounds__(256) poly_deriv_kernel(ModelDev md, PolyDev pd
             T s0 = T(0), s1 = T(0), s2 = T(0);
             T st[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
             const int p0 = bd.nb_off[i], p1 = bd.nb_off[i + 1];
+            // record headers (d, r | valid, tp, j) as two 4-element loads; the
+            // next pair's header loads while this pair is processed
+            using V4 = typename Vec4Of<T>::type;
+            V4 h0n{}, h1n{};
+            if (p0 + sl < p1) {
+                const V4* hr = reinterpret_cast<const V4*>(prec + (size_t)(p0 + sl - pair_base) * R.stride);
+                h0n = hr[0], h1n = hr[1];
+            }
             for (int p = p0 + sl; p < p1; p += S) {
                 const T* rec = prec + (size_t)(p - pair_base) * R.stride;
-                if (rec[4] == T(0)) continue;
-                const int tp = int(rec[5]);
+                const V4 h0 = h0n, h1 = h1n;
+                if (p + S < p1) {
+                    const V4* hr = reinterpret_cast<const V4*>(rec + (size_t)S * R.stride);
+                    h0n = hr[0], h1n = hr[1];
+                }
+                if (h1.x == T(0)) continue;
+                const int tp = int(h1.y);
                 const int ga = TT.wz_off[fl * ntp + tp], gb = TT.wz_off[fl * ntp + tp + 1];
                 if (ga == gb) continue;
                 const T* fn = rec + R.fn();
                 const T* fdr = rec + R.fd();
-                const T ir = T(1) / rec[3];
-                const T ux = rec[0] * ir, uy = rec[1] * ir, uz = rec[2] * ir;
+                const T ir = T(1) / h0.w;
+                const T ux = h0.x * ir, uy = h0.y * ir, uz = h0.z * ir;
                 T q0 = T(0), q1 = T(0), q2 = T(0);
                 for (int g = ga; g < gb; ++g) {
                     const int nid = TT.wz_nid[g];
@@ -113,11 +126,11 @@ __global__ void __launch_bounds__(256) poly_deriv_kernel(ModelDev md, PolyDev pd
                     }
                 }
                 s0 += q0, s1 += q1, s2 += q2;
-                const int jl = int(rec[6]);
+                const int jl = int(h1.z);
                 atomicAdd(a + 6 + 3 * jl, -q0);
                 atomicAdd(a + 6 + 3 * jl + 1, -q1);
                 atomicAdd(a + 6 + 3 * jl + 2, -q2);
-                const T dx = rec[0], dy = rec[1], dz = rec[2];
+                const T dx = h0.x, dy = h0.y, dz = h0.z;
                 st[0] -= q0 * dx, st[1] -= q1 * dy, st[2] -= q2 * dz;
                 st[3] -= q0 * dy, st[4] -= q1 * dz, st[5] -= q2 * dx;
             }
