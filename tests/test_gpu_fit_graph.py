@@ -6,9 +6,13 @@ of the X build, the polynomial GEMMs included) on 12 structures, split into
 several sub-batches by a small X budget. Creation leaves the accumulator at
 zero; each replay adds one pass. Replays run with other eager accumulations
 and synchronisations between them, and every replay is checked against the
-eager sums: FP64 to 1e-10 and mixed to 1e-7 (relative, normwise; the X build
-adds per-pair terms with atomics, so the last bits of X, and in the mixed form
-an occasional quantisation step, depend on their order).
+eager sums: FP64 to 1e-10 and mixed to 1e-6 (relative, normwise). The X build
+adds per-pair terms with atomics (in the mixed form FP32 adds into the factor
+derivatives), so the last bits of X, and in the mixed form an occasional
+quantisation digit, depend on their order: two eager mixed accumulations of
+this batch differ by up to 5e-8 in X^T X and 1.1e-7 in X^T y (measured), FP64
+ones by 2e-16. A replay that missed a sub-batch or read a stale buffer would be
+off by 1e-3 or more.
 """
 
 import os
@@ -22,7 +26,7 @@ from pypolymlp_amd import capi, datasets
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision,tol", [(capi.PLM_FP64, 1e-10), (capi.PLM_MIXED, 1e-7)])
+@pytest.mark.parametrize("precision,tol", [(capi.PLM_FP64, 1e-10), (capi.PLM_MIXED, 1e-6)])
 def test_fit_graph_replays_match_eager(precision, tol, monkeypatch):
     print(f"fit graph: binary alloy model, 12 structures, precision {precision}", flush=True)
     sys.stdout.flush()
