@@ -6,6 +6,7 @@
 #include <map>
 #include <tuple>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -117,6 +118,10 @@ struct ModelHandle {
         int candidates = 0;
     };
     std::map<std::tuple<int, int, int>, GroupChoice> group_tuning;
+    // work buffers of eager evaluations (order parameters, head sums, results),
+    // kept and grown across calls; eval_mutex serialises the calls that use them
+    DevBuf ev_a, ev_pse, ev_psf, ev_out;
+    std::mutex eval_mutex;
 };
 
 struct BatchHandle {

@@ -174,11 +174,12 @@ void eval_batch(ModelHandle& m, BatchHandle& b, bool fp64, double* energies, dou
                 double* stresses) {
     prepare_eval(m, b);
     hipStream_t st = device().stream;
-    DevBuf a_buf, pse, psf, out;
+    std::lock_guard<std::mutex> lock(m.eval_mutex);
     size_t n_out = 0;
-    alloc_eval(m, b, fp64, a_buf, pse, psf, out, n_out);
-    enqueue_eval(m, b, fp64, a_buf.ptr, pse.ptr, psf.ptr, (double*)out.ptr, n_out, device().stream);
-    copy_out(b, out, n_out, st, energies, forces, stresses);
+    alloc_eval(m, b, fp64, m.ev_a, m.ev_pse, m.ev_psf, m.ev_out, n_out);
+    enqueue_eval(m, b, fp64, m.ev_a.ptr, m.ev_pse.ptr, m.ev_psf.ptr, (double*)m.ev_out.ptr, n_out,
+                 st);
+    copy_out(b, m.ev_out, n_out, st, energies, forces, stresses);
 }
 
 EvalGraph::~EvalGraph() {
