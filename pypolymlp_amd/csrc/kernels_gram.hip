@@ -144,10 +144,18 @@ __global__ void __launch_bounds__(256) colexp_kernel(const T* __restrict__ X,
     for (int u = 0; u < 8; ++u) m[u] = T(0);
     if (c < ncols) {
         int r = r0 + ty;
+        // the next batch's row indices load while this batch's rows of X do
+        int rn[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rn[u] = r + 28 < r1 ? rmap[r + 4 * u] : -1;
         for (; r + 28 < r1; r += 32) {
             int rr[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) rr[u] = rmap[r + 4 * u];
+            for (int u = 0; u < 8; ++u) rr[u] = rn[u];
+            if (r + 60 < r1) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) rn[u] = rmap[r + 32 + 4 * u];
+            }
             T v[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) v[u] = rr[u] >= 0 ? X[(size_t)rr[u] * ld + c] : T(0);
