@@ -146,7 +146,7 @@ def eval_leg(steps: int, graph_steps: int, cpu: bool):
     graph steps replay the evaluation kernels over the resident lists.
 
     Roofline of the dominant evaluation kernel, the half-pair loop
-    (pair_rec_kernel, or pair_lane_kernel in FP32 for max_l <= 4): algorithmic
+    (pair_lane_kernel): algorithmic
     flops = 40 n_fn n_lm_half per half pair (radial functions, Y_lm and their
     gradients, and the contraction with both atoms' prod_sum), against the
     vector peak of the precision it computes in (FP64 78.6, FP32 157.3 TFLOP/s)."""
@@ -197,8 +197,7 @@ def eval_leg(steps: int, graph_steps: int, cpu: bool):
         ms_p, n_p = kern["eval_pair"]
         peak = FP64_PEAK_TFLOPS if name == "fp64" else FP32_PEAK_TFLOPS
         ach = 40.0 * n_fn * n_lmh * n_half / (ms_p / n_p * 1e-3) / 1e12
-        form = os.environ.get("PLM_EVAL_PAIR") or (
-            "lane" if name == "mixed" and spec.max_l <= 4 else "records")
+        form = os.environ.get("PLM_EVAL_PAIR") or "lane"
         out[f"eval_roofline_{name}"] = {
             "bound": "valu", "unit": "TFLOP/s",
             "kernel": "pair_lane_kernel" if form == "lane" else "pair_rec_kernel",

@@ -1,6 +1,7 @@
 // Evaluation (PolymlpEval) orchestration: potential tables, half neighbour
 // list, order parameters, per-atom prod_sum and the half-pair loop.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -47,14 +48,27 @@ void upload_potential(ModelHandle& m) {
             for (size_t k = 0; k < T.ntp.size(); ++k)
                 by_tp[T.ntp[k][2]].push_back({T.ntp[k][1], -1, (int)k, T.ntp[k][3], 1, 0});
         }
-        std::vector<int> hoff(ntp + 1, 0), nid, ylm, hi, hj;
+        // heads of each type pair ordered by Y key, with the first head of every
+        // key (the per-lane pair loop generates Y_lm key by key)
+        const int nkey = m.md.n_lmh;
+        std::vector<int> hoff(ntp + 1, 0), koff((size_t)ntp * (nkey + 1), 0), nid, ylm, hi, hj;
         std::vector<float> sj;
         std::vector<unsigned char> m0;
         for (int tp = 0; tp < ntp; ++tp) {
+            if (gt)
+                std::stable_sort(by_tp[tp].begin(), by_tp[tp].end(),
+                                 [](const std::array<int, 6>& a, const std::array<int, 6>& b) {
+                                     return a[1] < b[1];
+                                 });
+            int* ko = koff.data() + (size_t)tp * (nkey + 1);
             for (const auto& e : by_tp[tp]) {
+                if (gt && (e[1] < 0 || e[1] >= nkey)) throw std::invalid_argument("head Y key range");
+                if (gt) ++ko[e[1] + 1];
                 nid.push_back(e[0]), ylm.push_back(e[1]), hi.push_back(e[2]), hj.push_back(e[3]);
                 sj.push_back((float)e[4]), m0.push_back((unsigned char)e[5]);
             }
+            ko[0] = hoff[tp];
+            for (int k = 0; k < nkey; ++k) ko[k + 1] += ko[k];
             hoff[tp + 1] = (int)nid.size();
         }
         auto upv = [&](const auto& v) {
@@ -90,6 +104,7 @@ void upload_potential(ModelHandle& m) {
                           hj[h] | (sj[h] < 0.f ? 1 << 28 : 0) | (m0[h] ? 1 << 29 : 0)};
         }
         E.hd_off = upv(hoff);
+        E.hd_koff = upv(koff);
         E.hd_rec = static_cast<const int4*>(upv(rec));
     }
     m.pot_uploaded = true;

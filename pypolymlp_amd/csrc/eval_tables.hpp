@@ -20,6 +20,9 @@ struct EvalTypeDev {
     const double2* pq_c;
     // half-pair contraction: heads of this type grouped by type pair
     const int* hd_off;      // [n_tp+1]
+    // gtinv: the heads of each type pair are ordered by Y key; hd_koff[tp (n_lmh + 1) + k]
+    // is the first head of key k of type pair tp
+    const int* hd_koff;
     // per head, one 16-byte record: (radial n_id, ylm key, ilocal_noconj of the
     // head (atom i), jlocal_noconj (the same component seen from atom j) with
     // bit 28 set where (-1)^l = -1 and bit 29 where m == 0 (weight 1/2))

@@ -278,13 +278,12 @@ __global__ void __launch_bounds__(64) pair_rec_kernel(ModelDev md, EvalDev ed, B
 
 // ---------------------------------------------------------------------------
 // Half-pair loop, per-lane form: one lane per half pair (i, j) of the whole
-// list, radial + Y_lm (+ gradients) in private arrays bounded by LB (max l) and
-// NB (radial functions). A lane finds its atom i from the half-list offsets;
-// the forces on i and j go out as atomics per lane, the energy and stress with
-// one atomic per wave when the wave's pairs belong to one structure. Faster
-// than the record form (below) where the private arrays are small enough to
-// stay in the L1 cache (FP32, max_l <= 4: 496 bytes per lane); in FP64 they
-// take 1.7 KB per lane and the record form is faster.
+// list, the radial functions in registers (NB bounds their number) and Y_lm
+// with its gradient generated column by column (LB bounds max l), each key
+// used at once on its heads (the heads of a type pair are ordered by key). A
+// lane finds its atom i from the half-list offsets; the forces on i and j go
+// out as atomics per lane, the energy and stress with one atomic per wave
+// when the wave's pairs belong to one structure.
 // ---------------------------------------------------------------------------
 template <class T, int LB, int NB>
 __global__ void __launch_bounds__(256) pair_lane_kernel(ModelDev md, EvalDev ed, BatchDev bd,
@@ -297,7 +296,6 @@ __global__ void __launch_bounds__(256) pair_lane_kernel(ModelDev md, EvalDev ed,
                                                    double* __restrict__ energy,
                                                    double* __restrict__ forces,
                                                    double* __restrict__ stress, int pair_model) {
-    constexpr int NLMH = (LB + 1) * (LB + 2) / 2;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double eij = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
     double dx = 0.0, dy = 0.0, dz = 0.0;
@@ -339,34 +337,128 @@ __global__ void __launch_bounds__(256) pair_lane_kernel(ModelDev md, EvalDev ed,
                 const double fij = -double(ff) / dis;
                 fx = fij * dx, fy = fij * dy, fz = fij * dz;
             } else {
-                Cpx<T> Y[NLMH], dY[3 * NLMH];
-                T pbuf[NLMH], qbuf[NLMH];
-                ylm<T>(md.max_l, T(dx), T(dy), T(dz), T(dis), Y, dY, pbuf, qbuf);
+                // Y_l^{-m} and its gradient generated column by column, m = max_l
+                // down to 0 (column m needs Q(l, m + 1) of the column before), each
+                // key used at once on its heads: the recurrences and operation
+                // order of ylm(), with no array indexed at run time
+                using C = MathConst<T>;
+                const T x = T(dx), yy = T(dy), z = T(dz), r = T(dis);
+                const T ct = z / r;
+                const T rho = dhypot(x, yy);
+                T cp = T(1), sp = T(0);
+                if (rho > T(0)) cp = x / rho, sp = yy / rho;
+                const T st = dsqrt(T(1) - ct * ct);
+                const T invr = T(1) / r;
                 const T ir = T(1) / T(dis);
+                T cm[LB + 1], sm[LB + 1];  // cos, sin (m phi) by the Chebyshev recurrence
+                cm[0] = T(1), sm[0] = T(0);
+                {
+                    T c1 = T(1), c2 = cp, s1 = T(0), s2 = -sp;
+                    const T tc = T(2) * c2;
+#pragma unroll
+                    for (int mp = 1; mp <= LB; ++mp) {
+                        const T sv = tc * s1 - s2;
+                        const T cv = tc * c1 - c2;
+                        c2 = c1, c1 = cv, s2 = s1, s1 = sv;
+                        cm[mp] = cv, sm[mp] = sv;
+                    }
+                }
+                const int lmax = md.max_l;
+                const int* ko = E.hd_koff + (size_t)tp * (md.n_lmh + 1);
                 T ea = T(0), fxa = T(0), fya = T(0), fza = T(0);
-                for (int h = h0; h < h1; ++h) {
-                    const int4 hr = E.hd_rec[h];
-                    const int nid = hr.x;
-                    const T fv = fn[nid];
-                    if (fv < T(1e-20)) continue;
-                    const int key = hr.y;
-                    const int hi = hr.z, hj = hr.w & 0x0fffffff;
-                    const T sj = (hr.w >> 28) & 1 ? T(-1) : T(1);
-                    const T w = (hr.w >> 29) & 1 ? T(0.5) : T(1);
-                    const Cpx<T> se = {(Pi_e[hi].re + Pj_e[hj].re * sj) * w,
-                                       (Pi_e[hi].im + Pj_e[hj].im * sj) * w};
-                    const Cpx<T> sf = {(Pi_f[hi].re + Pj_f[hj].re * sj) * w,
-                                       (Pi_f[hi].im + Pj_f[hj].im * sj) * w};
-                    const Cpx<T> y = Y[key];
-                    ea += fv * (y.re * se.re - y.im * se.im);
-                    const T d1r = fd[nid] * y.re * ir, d1i = fd[nid] * y.im * ir;
-                    // val_a = -(d1 * d_a + fn * dY_a)
-                    const Cpx<T> vx = {-(d1r * T(dx) + fv * dY[3 * key].re), -(d1i * T(dx) + fv * dY[3 * key].im)};
-                    const Cpx<T> vy = {-(d1r * T(dy) + fv * dY[3 * key + 1].re), -(d1i * T(dy) + fv * dY[3 * key + 1].im)};
-                    const Cpx<T> vz = {-(d1r * T(dz) + fv * dY[3 * key + 2].re), -(d1i * T(dz) + fv * dY[3 * key + 2].im)};
-                    fxa += re_mul(vx, sf);
-                    fya += re_mul(vy, sf);
-                    fza += re_mul(vz, sf);
+                auto heads = [&](int key, Cpx<T> y, Cpx<T> gx, Cpx<T> gy, Cpx<T> gz) {
+                    for (int h = ko[key]; h < ko[key + 1]; ++h) {
+                        const int4 hr = E.hd_rec[h];
+                        const int nid = hr.x;
+                        const T fv = fn[nid];
+                        if (fv < T(1e-20)) continue;
+                        const int hi = hr.z, hj = hr.w & 0x0fffffff;
+                        const T sj = (hr.w >> 28) & 1 ? T(-1) : T(1);
+                        const T w = (hr.w >> 29) & 1 ? T(0.5) : T(1);
+                        const Cpx<T> se = {(Pi_e[hi].re + Pj_e[hj].re * sj) * w,
+                                           (Pi_e[hi].im + Pj_e[hj].im * sj) * w};
+                        const Cpx<T> sf = {(Pi_f[hi].re + Pj_f[hj].re * sj) * w,
+                                           (Pi_f[hi].im + Pj_f[hj].im * sj) * w};
+                        ea += fv * (y.re * se.re - y.im * se.im);
+                        const T d1r = fd[nid] * y.re * ir, d1i = fd[nid] * y.im * ir;
+                        // val_a = -(d1 * d_a + fn * dY_a)
+                        const Cpx<T> vx = {-(d1r * x + fv * gx.re), -(d1i * x + fv * gx.im)};
+                        const Cpx<T> vy = {-(d1r * yy + fv * gy.re), -(d1i * yy + fv * gy.im)};
+                        const Cpx<T> vz = {-(d1r * z + fv * gz.re), -(d1i * z + fv * gz.im)};
+                        fxa += re_mul(vx, sf);
+                        fya += re_mul(vy, sf);
+                        fza += re_mul(vz, sf);
+                    }
+                };
+                T qn[LB + 1];  // Q(l, m + 1)
+#pragma unroll
+                for (int l = 0; l <= LB; ++l) qn[l] = T(0);
+#pragma unroll
+                for (int m = LB; m >= 0; --m) {
+                    if (m > lmax) continue;
+                    // normalised associated Legendre column m: P and Q = P / sin(theta)
+                    T pc[LB + 1], qc[LB + 1];
+                    T pmm = C::sqrt_inv_2pi, qmm = T(0);
+                    if (m >= 1) {
+                        pmm = -st * C::sqrt3div2 * C::sqrt_inv_2pi;
+                        qmm = -C::sqrt3div2 * C::sqrt_inv_2pi;
+#pragma unroll
+                        for (int l = 2; l <= m; ++l) {
+                            const T c1 = -dsqrt(T(1) + T(0.5) / T(l)) * st;
+                            pmm = c1 * pmm;
+                            qmm = c1 * qmm;
+                        }
+                    }
+#pragma unroll
+                    for (int l = 0; l <= LB; ++l) pc[l] = T(0), qc[l] = T(0);
+                    pc[m] = pmm, qc[m] = qmm;
+                    if (m + 1 <= LB && m + 1 <= lmax) {
+                        if (m == 0) {
+                            pc[1] = ct * C::sqrt3 * C::sqrt_inv_2pi;
+                            qc[1] = T(0);
+                        } else {
+                            const T c2 = dsqrt(T(2) * (T(m + 1) - T(1)) + T(3)) * ct;
+                            pc[m + 1] = c2 * pmm;
+                            qc[m + 1] = c2 * qmm;
+                        }
+#pragma unroll
+                        for (int l = m + 2; l <= LB; ++l) {
+                            if (l > lmax) break;
+                            const T ls = T(l * l), lm1s = T((l - 1) * (l - 1)), msq = T(m * m);
+                            const T alm = dsqrt((T(4) * ls - T(1)) / (ls - msq));
+                            const T blm = -dsqrt((lm1s - msq) / (T(4) * lm1s - T(1)));
+                            pc[l] = alm * (ct * pc[l - 1] + blm * pc[l - 2]);
+                            qc[l] = alm * (ct * qc[l - 1] + blm * qc[l - 2]);
+                        }
+                    }
+#pragma unroll
+                    for (int l = m; l <= LB; ++l) {
+                        if (l > lmax) break;
+                        if (m == 0) {
+                            const T common =
+                                (l >= 1) ? qn[l] * st * invr * dsqrt(T(0.5) * T(l) * T(l + 1)) : T(0);
+                            heads(lm2i(l, 0) + l, Cpx<T>{pc[l] * C::half_sqrt2, T(0)},
+                                  Cpx<T>{common * ct * cp, T(0)}, Cpx<T>{common * ct * sp, T(0)},
+                                  Cpx<T>{-common * st, T(0)});
+                        } else {
+                            const T c = cm[m], sv = sm[m];
+                            const T sign = (m % 2 == 1) ? T(-1) : T(1);
+                            const T tmp = sign * pc[l] * C::half_sqrt2;
+                            const T cre = c * C::half_sqrt2 * invr, cim = sv * C::half_sqrt2 * invr;
+                            T dth = T(m) * ct * qc[l];
+                            if (m != l) dth += dsqrt(T((l - m) * (l + m + 1))) * qn[l] * st;
+                            const T dph = T(m) * qc[l];
+                            const T ure = dth * ct * cp, uim = -dph * sp;
+                            const T vre = dth * ct * sp, vim = dph * cp;
+                            const T wre = -dth * st;
+                            heads(lm2i(l, -m) + l, Cpx<T>{tmp * c, -tmp * sv},
+                                  Cpx<T>{sign * (cre * ure - cim * uim), -sign * (cre * uim + cim * ure)},
+                                  Cpx<T>{sign * (cre * vre - cim * vim), -sign * (cre * vim + cim * vre)},
+                                  Cpx<T>{sign * (cre * wre), -sign * (cim * wre)});
+                        }
+                    }
+#pragma unroll
+                    for (int l = 0; l <= LB; ++l) qn[l] = qc[l];
                 }
                 eij = double(ea), fx = double(fxa), fy = double(fya), fz = double(fza);
             }
@@ -470,15 +562,14 @@ void launch_pair_lane_t(hipStream_t st, const ModelDev& md, const EvalDev& ed,
         throw std::invalid_argument("the per-lane pair loop supports max_l <= 12");
 }
 
-// The record form, except in FP32 with max_l <= 4 (the per-lane form);
-// PLM_EVAL_PAIR=lane / records forces one (the per-lane form needs max_l <= 12
-// and at most 32 radial functions).
+// The per-lane form where it applies (max_l <= 12, at most 32 radial
+// functions), the record form beyond; PLM_EVAL_PAIR=lane / records forces one.
 void launch_pair(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,
                  const BatchDev& bd, const int* atom_st, const int* hb_off, const int* hb_j,
                  const double* hb_d, int n_half, const void* ps_e, const void* ps_f, int ps_stride,
                  double* energy, double* forces, double* stress, int pair_model) {
     if (n_half <= 0) return;
-    bool lane = !fp64 && md.max_l <= 4;
+    bool lane = md.max_l <= 12 && md.n_fn <= 32;
     if (const char* e = getenv("PLM_EVAL_PAIR")) {
         if (!strcmp(e, "lane")) lane = true;
         else if (!strcmp(e, "records")) lane = false;
