@@ -191,16 +191,17 @@ __global__ void __launch_bounds__(256) anlm_kernel(ModelDev md, BatchDev bd,
     }
 }
 
-// Order parameters with register sums (evaluation, at most 64 Q heads per
-// type): thread t owns the entries t + 64 q (q < Q) of the atom type's list
+// Order parameters with register sums (evaluation, at most NT Q heads per
+// type, NT threads per atom): thread t owns the entries t + NT q (q < Q) of the
+// atom type's list
 // grouped by type pair, holds their type pair, radial index, Y key and head
 // in registers for the whole atom, and adds f_n Y_lm of every pair of its
 // entries' type pair into a register sum; the sums go to their heads at the
 // end. Pairs are taken in the same order as anlm_kernel, so the sums are the
 // same; the entry tables are read once per atom instead of once per pair and
 // the LDS read-modify-write per pair and entry is gone.
-template <class T, int Q>
-__global__ void __launch_bounds__(64) anlm_reg_kernel(ModelDev md, BatchDev bd,
+template <class T, int Q, int NT>
+__global__ void __launch_bounds__(NT) anlm_reg_kernel(ModelDev md, BatchDev bd,
                                                       Cpx<T>* __restrict__ a_full, int a_stride,
                                                       int atom0, int n_atoms, int negate) {
     extern __shared__ unsigned char smem_raw[];
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(64) anlm_reg_kernel(ModelDev md, BatchDev bd,
     Cpx<T> a[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int h = tid + 64 * q;
+        const int h = tid + NT * q;
         etp[q] = -1, enid[q] = 0, ekey[q] = 0, ehd[q] = 0;
         a[q] = {T(0), T(0)};
         if (h < n_nc) {
@@ -303,28 +304,29 @@ size_t anlm_smem(const ModelDev& md, int elem, int nthreads, int max_noconj) {
     return (size_t)nthreads * L.stride * elem + (size_t)max_noconj * 2 * elem;
 }
 
-template <class T, int Q>
+template <class T, int Q, int NT>
 void launch_anlm_reg(hipStream_t st, const ModelDev& md, const BatchDev& bd, void* a_full,
                      int a_stride, int atom0, int n_atoms, size_t sm, int negate) {
-    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_reg_kernel<T, Q>),
+    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_reg_kernel<T, Q, NT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm),
               "anlm shared memory");
-    anlm_reg_kernel<T, Q><<<n_atoms, 64, sm, st>>>(md, bd, reinterpret_cast<Cpx<T>*>(a_full),
-                                                   a_stride, atom0, n_atoms, negate);
+    anlm_reg_kernel<T, Q, NT><<<n_atoms, NT, sm, st>>>(md, bd, reinterpret_cast<Cpx<T>*>(a_full),
+                                                       a_stride, atom0, n_atoms, negate);
 }
 
-// register sums up to 512 heads per type (8 per thread), LDS sums beyond
+// register sums: up to 256 heads per type with 64 threads per atom, up to 2048
+// with 256 threads (8 per thread); LDS sums beyond
 template <class T>
 void launch_anlm_t(hipStream_t st, const ModelDev& md, const BatchDev& bd, void* a_full,
                    int a_stride, int atom0, int n_atoms, int max_noconj, int negate) {
     const int B = 64;
     const size_t sm = anlm_smem(md, sizeof(T), kAnlmPairs, max_noconj);
     if (max_noconj <= 4 * 64) {
-        launch_anlm_reg<T, 4>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+        launch_anlm_reg<T, 4, 64>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
         return;
     }
-    if (max_noconj <= 8 * 64) {
-        launch_anlm_reg<T, 8>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+    if (max_noconj <= 8 * 256) {
+        launch_anlm_reg<T, 8, 256>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
         return;
     }
     check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_kernel<T>),

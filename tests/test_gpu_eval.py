@@ -227,7 +227,10 @@ def test_eval_graph_replays(prec):
         np.testing.assert_allclose(e, e0, rtol=1e-12)
         for fk, f0k in zip(f, f0):
             np.testing.assert_allclose(fk, f0k, rtol=0, atol=1e-12 * max(1.0, np.abs(f0k).max()))
-        np.testing.assert_allclose(s, s0, rtol=0, atol=1e-12 * np.abs(s0).max())
+        # stresses are per-wave partial sums added by FP64 atomics in no fixed
+        # order, and the pair terms cancel to about 1e-3 of their size: runs
+        # differ by up to about 1e-12 of the largest component
+        np.testing.assert_allclose(s, s0, rtol=0, atol=1e-11 * np.abs(s0).max())
         e1, _, _ = capi.eval_batch(m, b, prec)
         np.testing.assert_allclose(e1, e0, rtol=1e-12)
     assert e[0] == pytest.approx(-40.223320043232334, rel=1e-8 if prec == capi.PLM_FP64 else 1e-6)

@@ -4,8 +4,10 @@ device array, no allocation in the call) against plm_eval, through the C-ABI.
 Two calls in a row, the second on a smaller structure (the array keeps the
 first call's contents), in FP64 and in mixed precision, each compared with
 plm_eval on the same structure: the kernels are the same and the neighbour
-lists are the reference's in the same order, so the results agree to 1e-12
-(relative, normwise) -- tighter than the FP64 parity bound of 1e-10. The
+lists are the reference's in the same order, so energies and forces agree to
+1e-12 (relative, normwise) -- tighter than the FP64 parity bound of 1e-10 --
+and stresses to 1e-11 (they are per-wave partial sums added by FP64 atomics in
+no fixed order, of pair terms that cancel to about 1e-3 of their size). The
 FP64 energy of the reference's MgO cell also matches the reference's golden
 value. Structures whose buffers exceed the array are refused with an error.
 """
@@ -53,7 +55,7 @@ def test_eval_noalloc_matches_eval_two_calls(mgo, precision):
         e0, f0, s0 = capi.eval_structure(mgo, axis, pos, types, precision)
         e1, f1, s1 = capi.eval_structure_noalloc(mgo, axis, pos, types, precision)
         assert abs(e1 - e0) <= 1e-12 * abs(e0)
-        assert _close(f1, f0) and _close(s1, s0)
+        assert _close(f1, f0) and _close(s1, s0, 1e-11)
         if precision == capi.PLM_FP64 and n == 8:
             assert e1 == pytest.approx(-40.223320043232334, rel=1e-8)
 
