@@ -2,6 +2,7 @@
 // list, order parameters, per-atom prod_sum and the half-pair loop.
 #include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -18,6 +19,22 @@ void upload_potential(ModelHandle& m) {
     const int nt = M.n_type(), ntp = M.n_tp();
     const bool gt = M.input().gtinv;
     m.max_heads = 0, m.max_lin = 0;
+    // the contraction deals each atom's terms to one wave, or to four waves
+    // (sharing the atom's LDS data) when a wave would hold more than 128 terms
+    // per lane; PLM_PRODSUM_LANES=64|256 forces one
+    long max_terms = 0;
+    for (int t = 0; t < nt; ++t) {
+        long n = 0;
+        for (const auto& h : M.type(t).pot) n += (long)h.size();
+        max_terms = std::max(max_terms, n);
+    }
+    int lanes = max_terms > 128L * 64 ? 256 : 64;
+    if (const char* e = std::getenv("PLM_PRODSUM_LANES")) {
+        if (!std::strcmp(e, "64")) lanes = 64;
+        else if (!std::strcmp(e, "256")) lanes = 256;
+        else throw std::invalid_argument("PLM_PRODSUM_LANES must be 64 or 256");
+    }
+    m.ed.pq_lanes = lanes;
     for (int t = 0; t < nt; ++t) {
         const auto& T = M.type(t);
         EvalTypeDev& E = m.ed.t[t];
@@ -79,8 +96,8 @@ void upload_potential(ModelHandle& m) {
         // interleaved lane chunks of the contraction (keys and heads fit 16 bits:
         // the order parameters and features they index are held in LDS)
         const long n_terms = (long)ce.size();
-        const long iters = (n_terms + 63) / 64;
-        std::vector<int4> pq_key((size_t)std::max(1L, iters) * 64, int4{-1, -1, -1, 0});
+        const long iters = (n_terms + lanes - 1) / lanes;
+        std::vector<int4> pq_key((size_t)std::max(1L, iters) * lanes, int4{-1, -1, -1, 0});
         std::vector<double2> pq_c(pq_key.size(), double2{0.0, 0.0});
         auto k16 = [](int v) -> unsigned {
             if (v >= 0xffff) throw std::invalid_argument("evaluation table index exceeds 16 bits");
@@ -88,7 +105,7 @@ void upload_potential(ModelHandle& m) {
         };
         for (int h = 0; h < E.n_heads; ++h)
             for (int e = off[h]; e < off[h + 1]; ++e) {
-                const size_t r = (size_t)(e % iters) * 64 + e / iters;
+                const size_t r = (size_t)(e % iters) * lanes + e / iters;
                 pq_key[r] = int4{(int)(k16(pd[3 * e]) | k16(pd[3 * e + 1]) << 16),
                                  (int)(k16(pd[3 * e + 2]) | k16(h) << 16),
                                  (int)(k16(pf[2 * e]) | k16(pf[2 * e + 1]) << 16), 0};

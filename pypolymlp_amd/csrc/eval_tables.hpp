@@ -9,10 +9,10 @@ struct EvalTypeDev {
     int n_heads;            // non-conjugate components (gtinv) or pair components
     const int* pot_off;     // [n_heads+1] first term of each head
     // The terms (coefficients and the order-parameter and feature keys of their
-    // products, ordered by head) for the per-atom contraction, dealt to the 64
-    // lanes of a wave in equal contiguous chunks of pq_iters terms and
-    // interleaved: record i * 64 + l is the i-th term of lane l's chunk (past
-    // the chunk: padding, head 0xffff). Keys and head in 16-bit fields,
+    // products, ordered by head) for the per-atom contraction, dealt to the
+    // pq_lanes lanes of a workgroup in equal contiguous chunks of pq_iters terms
+    // and interleaved: record i * pq_lanes + l is the i-th term of lane l's
+    // chunk (past the chunk: padding, head 0xffff). Keys and head in 16-bit fields,
     // 0xffff: none:
     //   (pd0 | pd1 << 16, pd2 | head << 16, pf0 | pf1 << 16, 0), coefficients (e, f).
     int pq_iters;
@@ -31,6 +31,7 @@ struct EvalTypeDev {
 
 struct EvalDev {
     EvalTypeDev t[kMaxTypes];
+    int pq_lanes;           // lanes of the contraction's workgroup: 64 or 256
 };
 
 }  // namespace plm

@@ -27,17 +27,17 @@ void check_hip(hipError_t e, const char* what);
 // ---------------------------------------------------------------------------
 // Per atom: features, products of features, and
 //   prod_sum_{e,f}[h] = sum_terms coeff_{e,f} * prod_features * prod_anlm
-// One 64-lane workgroup per atom; the atom's order parameters and features
-// staged in LDS (the term products gather from them at random keys). The
-// terms of all heads are dealt to the lanes in equal contiguous chunks
-// (every lane does the same number of terms, whatever the heads' lengths),
-// read as one 16-byte key record and one coefficient pair per term, one
-// coalesced load per wave and iteration. A lane writes the heads that begin
-// and end in its chunk; the heads that cross a chunk boundary are summed from
-// the lanes' boundary slots in LDS.
+// One workgroup of NL lanes (one wave, or four for models with many terms)
+// per atom; the atom's order parameters and features staged in LDS (the term
+// products gather from them at random keys). The terms of all heads are dealt
+// to the lanes in equal contiguous chunks (every lane does the same number of
+// terms, whatever the heads' lengths), read as one 16-byte key record and one
+// coefficient pair per term, one coalesced load per wave and iteration. A
+// lane writes the heads that begin and end in its chunk; the heads that cross
+// a chunk boundary are summed from the lanes' boundary slots in LDS.
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, BatchDev bd,
+template <class T, int NL>
+__global__ void __launch_bounds__(NL) prodsum_kernel(ModelDev md, EvalDev ed, BatchDev bd,
                                                      const Cpx<T>* __restrict__ a_full,
                                                      int a_stride, Cpx<T>* __restrict__ ps_e,
                                                      Cpx<T>* __restrict__ ps_f, int ps_stride,
@@ -45,9 +45,10 @@ __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, Ba
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     // two slots per lane of head sums (4 values each) and their heads, the
     // atom's order parameters (a_stride complex values), then its features
+    constexpr int NS = 2 * NL;
     double* sv = reinterpret_cast<double*>(smem_raw);
-    int* sh = reinterpret_cast<int*>(sv + 4 * 128);
-    Cpx<T>* a = reinterpret_cast<Cpx<T>*>(sh + 128);
+    int* sh = reinterpret_cast<int*>(sv + 4 * NS);
+    Cpx<T>* a = reinterpret_cast<Cpx<T>*>(sh + NS);
     T* xs = reinterpret_cast<T*>(a + a_stride);
     const int i = blockIdx.x;
     if (i >= bd.n_atoms_total) return;
@@ -87,8 +88,8 @@ __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, Ba
     for (int h = lane; h < E.n_heads; h += blockDim.x)
         if (E.pot_off[h + 1] == E.pot_off[h]) oe[h] = {T(0), T(0)}, of[h] = {T(0), T(0)};
     for (int it = 0; it < E.pq_iters; ++it) {
-        const int4 k = E.pq_key[(size_t)it * 64 + lane];
-        const double2 c = E.pq_c[(size_t)it * 64 + lane];
+        const int4 k = E.pq_key[(size_t)it * NL + lane];
+        const double2 c = E.pq_c[(size_t)it * NL + lane];
         const int head = (unsigned)k.y >> 16;
         if (head == 0xffff) break;  // the rest of this lane's chunk is padding
         if (head != cur) {
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(64) prodsum_kernel(ModelDev md, EvalDev ed, Ba
         while (p >= 0 && sh[p] < 0) --p;
         if (p >= 0 && sh[p] == h) continue;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        for (int q = sl; q < 128 && (sh[q] == h || sh[q] < 0); ++q) {
+        for (int q = sl; q < NS && (sh[q] == h || sh[q] < 0); ++q) {
             if (sh[q] < 0) continue;
             s0 += sv[4 * q], s1 += sv[4 * q + 1], s2 += sv[4 * q + 2], s3 += sv[4 * q + 3];
         }
@@ -488,30 +489,45 @@ __global__ void __launch_bounds__(256) pair_lane_kernel(ModelDev md, EvalDev ed,
     }
 }
 
-template <class T>
+template <class T, int NL>
 void launch_prodsum_t(hipStream_t st, const ModelDev& md, const EvalDev& ed, const BatchDev& bd,
                       const void* a_full, int a_stride, void* ps_e, void* ps_f, int ps_stride,
                       int max_lin, int pair_model) {
-    const size_t sm = 128 * (4 * sizeof(double) + sizeof(int)) + (size_t)a_stride * 2 * sizeof(T) +
-                      (size_t)std::max(1, max_lin) * sizeof(T);
+    const size_t sm = 2 * NL * (4 * sizeof(double) + sizeof(int)) +
+                      (size_t)a_stride * 2 * sizeof(T) + (size_t)std::max(1, max_lin) * sizeof(T);
     if (sm > 160 * 1024) throw std::invalid_argument("per-atom evaluation data exceed LDS");
-    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&prodsum_kernel<T>),
+    check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&prodsum_kernel<T, NL>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm),
               "prodsum shared memory");
-    prodsum_kernel<T><<<bd.n_atoms_total, 64, sm, st>>>(
+    prodsum_kernel<T, NL><<<bd.n_atoms_total, NL, sm, st>>>(
         md, ed, bd, reinterpret_cast<const Cpx<T>*>(a_full), a_stride,
         reinterpret_cast<Cpx<T>*>(ps_e), reinterpret_cast<Cpx<T>*>(ps_f), ps_stride, pair_model);
+}
+
+template <class T>
+void launch_prodsum_lanes(hipStream_t st, const ModelDev& md, const EvalDev& ed,
+                          const BatchDev& bd, const void* a_full, int a_stride, void* ps_e,
+                          void* ps_f, int ps_stride, int max_lin, int pair_model) {
+    // the term tables were laid out for ed.pq_lanes lanes
+    if (ed.pq_lanes == 256)
+        launch_prodsum_t<T, 256>(st, md, ed, bd, a_full, a_stride, ps_e, ps_f, ps_stride, max_lin,
+                                 pair_model);
+    else if (ed.pq_lanes == 64)
+        launch_prodsum_t<T, 64>(st, md, ed, bd, a_full, a_stride, ps_e, ps_f, ps_stride, max_lin,
+                                pair_model);
+    else
+        throw std::invalid_argument("contraction lanes must be 64 or 256");
 }
 
 void launch_prodsum(hipStream_t st, bool fp64, const ModelDev& md, const EvalDev& ed,
                     const BatchDev& bd, const void* a_full, int a_stride, void* ps_e, void* ps_f,
                     int ps_stride, int max_lin, int pair_model) {
     if (fp64)
-        launch_prodsum_t<double>(st, md, ed, bd, a_full, a_stride, ps_e, ps_f, ps_stride, max_lin,
-                                 pair_model);
+        launch_prodsum_lanes<double>(st, md, ed, bd, a_full, a_stride, ps_e, ps_f, ps_stride,
+                                     max_lin, pair_model);
     else
-        launch_prodsum_t<float>(st, md, ed, bd, a_full, a_stride, ps_e, ps_f, ps_stride, max_lin,
-                                pair_model);
+        launch_prodsum_lanes<float>(st, md, ed, bd, a_full, a_stride, ps_e, ps_f, ps_stride,
+                                    max_lin, pair_model);
 }
 
 template <class T>

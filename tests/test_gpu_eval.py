@@ -24,16 +24,19 @@ def say(msg):
     sys.stdout.flush()
 
 
-# the half-pair loop's two forms (records in LDS, per-lane arrays); "auto" is
-# the library's choice
-PAIR_FORMS = ["auto", "lane", "records"]
+# the half-pair loop's two forms (one lane per pair, pair records in LDS) and
+# the contraction's two workgroup sizes (terms dealt to 64 or 256 lanes);
+# "auto" is the library's choice of both
+PAIR_FORMS = ["auto", "lane", "records", "terms64", "terms256"]
 
 
 def _pair_form(form, monkeypatch):
-    if form == "auto":
-        monkeypatch.delenv("PLM_EVAL_PAIR", raising=False)
-    else:
+    monkeypatch.delenv("PLM_EVAL_PAIR", raising=False)
+    monkeypatch.delenv("PLM_PRODSUM_LANES", raising=False)
+    if form in ("lane", "records"):
         monkeypatch.setenv("PLM_EVAL_PAIR", form)
+    elif form.startswith("terms"):
+        monkeypatch.setenv("PLM_PRODSUM_LANES", form[5:])
 
 
 @pytest.mark.parametrize("form", PAIR_FORMS)
