@@ -314,19 +314,29 @@ void launch_anlm_reg(hipStream_t st, const ModelDev& md, const BatchDev& bd, voi
                                                        a_stride, atom0, n_atoms, negate);
 }
 
-// register sums: up to 256 heads per type with 64 threads per atom, up to 2048
-// with 256 threads (8 per thread); LDS sums beyond
+// register sums with 2-4 entries per thread (measured: 128 x 2 for 165 heads
+// 0.96 ms per 32,000 atoms against 1.37 ms for 64 x 4 and 1.06 ms for 256 x 1;
+// 512 x 3 for 1,350 heads 2.20 ms per 8,192 atoms against 2.73 ms for 256 x 8),
+// up to 2,048 heads per type; LDS sums beyond
 template <class T>
 void launch_anlm_t(hipStream_t st, const ModelDev& md, const BatchDev& bd, void* a_full,
                    int a_stride, int atom0, int n_atoms, int max_noconj, int negate) {
     const int B = 64;
     const size_t sm = anlm_smem(md, sizeof(T), kAnlmPairs, max_noconj);
-    if (max_noconj <= 4 * 64) {
-        launch_anlm_reg<T, 4, 64>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+    if (max_noconj <= 2 * 128) {
+        launch_anlm_reg<T, 2, 128>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
         return;
     }
-    if (max_noconj <= 8 * 256) {
-        launch_anlm_reg<T, 8, 256>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+    if (max_noconj <= 3 * 256) {
+        launch_anlm_reg<T, 3, 256>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+        return;
+    }
+    if (max_noconj <= 3 * 512) {
+        launch_anlm_reg<T, 3, 512>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+        return;
+    }
+    if (max_noconj <= 4 * 512) {
+        launch_anlm_reg<T, 4, 512>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
         return;
     }
     check_hip(hipFuncSetAttribute(reinterpret_cast<const void*>(&anlm_kernel<T>),
