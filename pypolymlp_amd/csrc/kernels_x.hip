@@ -7,6 +7,10 @@
 //   Local::compute_anlmtp_d (values, conjugate fill)  local.cpp:116-217
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+
 #include "device_math.hpp"
 #include "device_tables.hpp"
 #include "kernels.hpp"
@@ -318,24 +322,48 @@ void launch_anlm_reg(hipStream_t st, const ModelDev& md, const BatchDev& bd, voi
 // 0.96 ms per 32,000 atoms against 1.37 ms for 64 x 4 and 1.06 ms for 256 x 1;
 // 512 x 3 for 1,350 heads 2.20 ms per 8,192 atoms against 2.73 ms for 256 x 8),
 // up to 2,048 heads per type; LDS sums beyond
+// PLM_ANLM_SHAPE=2x128|3x256|3x512|4x512|lds forces one form (an error where
+// the type's heads do not fit it)
 template <class T>
 void launch_anlm_t(hipStream_t st, const ModelDev& md, const BatchDev& bd, void* a_full,
                    int a_stride, int atom0, int n_atoms, int max_noconj, int negate) {
     const int B = 64;
     const size_t sm = anlm_smem(md, sizeof(T), kAnlmPairs, max_noconj);
-    if (max_noconj <= 2 * 128) {
+    if (const char* e = getenv("PLM_ANLM_SHAPE")) {
+        auto forced = [&](int cap) {
+            if (max_noconj > cap)
+                throw std::invalid_argument("PLM_ANLM_SHAPE: the model's heads exceed this form");
+        };
+        if (!strcmp(e, "2x128")) {
+            forced(2 * 128);
+            launch_anlm_reg<T, 2, 128>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+            return;
+        } else if (!strcmp(e, "3x256")) {
+            forced(3 * 256);
+            launch_anlm_reg<T, 3, 256>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+            return;
+        } else if (!strcmp(e, "3x512")) {
+            forced(3 * 512);
+            launch_anlm_reg<T, 3, 512>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+            return;
+        } else if (!strcmp(e, "4x512")) {
+            forced(4 * 512);
+            launch_anlm_reg<T, 4, 512>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
+            return;
+        } else if (strcmp(e, "lds") != 0) {
+            throw std::invalid_argument("PLM_ANLM_SHAPE: 2x128, 3x256, 3x512, 4x512 or lds");
+        }
+        // lds: the LDS sums below
+    } else if (max_noconj <= 2 * 128) {
         launch_anlm_reg<T, 2, 128>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
         return;
-    }
-    if (max_noconj <= 3 * 256) {
+    } else if (max_noconj <= 3 * 256) {
         launch_anlm_reg<T, 3, 256>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
         return;
-    }
-    if (max_noconj <= 3 * 512) {
+    } else if (max_noconj <= 3 * 512) {
         launch_anlm_reg<T, 3, 512>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
         return;
-    }
-    if (max_noconj <= 4 * 512) {
+    } else if (max_noconj <= 4 * 512) {
         launch_anlm_reg<T, 4, 512>(st, md, bd, a_full, a_stride, atom0, n_atoms, sm, negate);
         return;
     }

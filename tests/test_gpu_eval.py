@@ -24,19 +24,34 @@ def say(msg):
     sys.stdout.flush()
 
 
-# the half-pair loop's two forms (one lane per pair, pair records in LDS) and
-# the contraction's two workgroup sizes (terms dealt to 64 or 256 lanes);
-# "auto" is the library's choice of both
-PAIR_FORMS = ["auto", "lane", "records", "terms64", "terms256"]
+# the half-pair loop's two forms (one lane per pair, pair records in LDS), the
+# contraction's two workgroup sizes (terms dealt to 64 or 256 lanes) and the
+# order parameters' forms (register sums of Q entries per thread over NT
+# threads, QxNT, or sums in LDS); "auto" is the library's choice of all three
+PAIR_FORMS = ["auto", "lane", "records", "terms64", "terms256",
+              "anlm2x128", "anlm3x256", "anlm3x512", "anlm4x512", "anlmlds"]
 
 
 def _pair_form(form, monkeypatch):
-    monkeypatch.delenv("PLM_EVAL_PAIR", raising=False)
-    monkeypatch.delenv("PLM_PRODSUM_LANES", raising=False)
+    for var in ("PLM_EVAL_PAIR", "PLM_PRODSUM_LANES", "PLM_ANLM_SHAPE"):
+        monkeypatch.delenv(var, raising=False)
     if form in ("lane", "records"):
         monkeypatch.setenv("PLM_EVAL_PAIR", form)
     elif form.startswith("terms"):
         monkeypatch.setenv("PLM_PRODSUM_LANES", form[5:])
+    elif form.startswith("anlm"):
+        monkeypatch.setenv("PLM_ANLM_SHAPE", form[4:])
+
+
+# heads each forced register form holds per type (Q x NT)
+ANLM_CAPACITY = {"anlm2x128": 256, "anlm3x256": 768, "anlm3x512": 1536, "anlm4x512": 2048}
+
+
+def _refused(form, m, n_type):
+    """True where a forced order-parameter form cannot hold the model's heads:
+    the evaluation must then fail with an error before any result."""
+    heads = max(m.type_sizes(t)[1] for t in range(n_type))
+    return heads > ANLM_CAPACITY.get(form, 1 << 30)
 
 
 @pytest.mark.parametrize("form", PAIR_FORMS)
@@ -53,6 +68,11 @@ def test_eval_mgo_golden(model, e_true, f_true, s_true, form, monkeypatch):
     m = capi.Model(spec.as_dict())
     m.set_coeffs(coeffs)
     st = poscar("POSCAR.RS.MgO")
+    if _refused(form, m, spec.n_type):
+        say(f"eval model={model}: form {form} refused")
+        with pytest.raises(RuntimeError, match="exceed"):
+            capi.eval_structure(m, st.axis, st.positions_cartesian, st.types)
+        return
     say(f"eval model={model} pair form={form}")
     e, f, s = capi.eval_structure(m, st.axis, st.positions_cartesian, st.types)
     assert e == pytest.approx(e_true, rel=1e-8)
@@ -84,6 +104,11 @@ def test_eval_configs_vs_oracle(cfg, form, tmp_path, monkeypatch):
     sts = ds.structures
     b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
                    [s.types for s in sts])
+    if _refused(form, m, spec.n_type):
+        say(f"eval batch {cfg}: form {form} refused")
+        with pytest.raises(RuntimeError, match="exceed"):
+            capi.eval_batch(m, b, capi.PLM_FP64)
+        return
     say(f"eval batch {cfg} FP64, pair form {form}")
     E, F, S = capi.eval_batch(m, b, capi.PLM_FP64)
     say(f"eval batch {cfg} MIXED")
