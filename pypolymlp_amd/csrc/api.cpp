@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <utility>
 #include <vector>
 
 #include "../../include/polymlp_amd.h"
@@ -197,9 +198,16 @@ int plm_api_sum_of_prod(plm_model* mp, int type, const double* anlm_re, const do
         launch_prodsum(st, fp64, m.md, m.ed, one.bd, one.a.ptr, m.max_nlm, pe.ptr, pf.ptr, ps,
                        m.max_lin, 0);
         check_hip(hipGetLastError(), "prodsum launch");
+        // the sums are stored by head position; returned in the potential's order
         const int nh = m.ed.t[type].n_heads;
-        if (prod_e) download_cpx(prod_e, pe, (size_t)nh, fp64, st);
-        if (prod_f) download_cpx(prod_f, pf, (size_t)nh, fp64, st);
+        const std::vector<int>& rank = m.head_rank.at(type);
+        std::vector<double> tmp(2 * (size_t)nh);
+        for (auto [dst, buf] : {std::pair<double*, const DevBuf*>{prod_e, &pe}, {prod_f, &pf}}) {
+            if (!dst) continue;
+            download_cpx(tmp.data(), *buf, (size_t)nh, fp64, st);
+            for (int k = 0; k < nh; ++k)
+                dst[2 * k] = tmp[2 * (size_t)rank[k]], dst[2 * k + 1] = tmp[2 * (size_t)rank[k] + 1];
+        }
     });
 }
 

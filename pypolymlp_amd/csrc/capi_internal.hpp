@@ -98,6 +98,9 @@ struct ModelHandle {
     unsigned pot_version = 0;  // counts uploads of the potential tables
     EvalDev ed{};
     int max_heads = 0, max_lin = 0;
+    // per type, the position of each head (in the potential's order) in the
+    // per-atom head sums, which follow the half-pair loop's order
+    std::vector<std::vector<int>> head_rank;
     explicit ModelHandle(const ModelInput& in);
     void build_device();
     // column-block tables for blocks of CB columns over columns [0, cover)

@@ -35,26 +35,20 @@ void upload_potential(ModelHandle& m) {
         else throw std::invalid_argument("PLM_PRODSUM_LANES must be 64 or 256");
     }
     m.ed.pq_lanes = lanes;
+    // The heads of each type in the order of the half-pair loop: grouped by type
+    // pair, within one ordered by Y key (the per-lane pair loop generates Y_lm
+    // key by key). The contraction writes head k of an atom at its position
+    // rank[t][k] in that order, so the pair loop reads both atoms' sums in
+    // address order. Entries: (radial n_id, ylm key, head k, head seen from
+    // atom j, sign, m == 0).
+    using HeadEntry = std::array<int, 6>;
+    std::vector<std::vector<std::vector<HeadEntry>>> by_tp_t(nt);
+    std::vector<std::vector<int>> order(nt);
+    m.head_rank.assign(nt, {});
     for (int t = 0; t < nt; ++t) {
         const auto& T = M.type(t);
-        EvalTypeDev& E = m.ed.t[t];
-        E.n_heads = (int)T.pot.size();
-        m.max_heads = std::max(m.max_heads, E.n_heads);
-        m.max_lin = std::max(m.max_lin, (int)T.features.size());
-        std::vector<int> off(E.n_heads + 1, 0), pd, pf;
-        std::vector<double> ce, cf;
-        for (int h = 0; h < E.n_heads; ++h) {
-            for (const auto& pt : T.pot[h]) {
-                const auto& k3 = T.prod_deriv[pt.prod];
-                const auto& k2 = T.prod_features[pt.prod_features];
-                if (k3.size() > 3 || k2.size() > 2) throw std::invalid_argument("term order too high");
-                for (int a = 0; a < 3; ++a) pd.push_back(a < (int)k3.size() ? k3[a] : -1);
-                for (int a = 0; a < 2; ++a) pf.push_back(a < (int)k2.size() ? k2[a] : -1);
-                ce.push_back(pt.coeff_e), cf.push_back(pt.coeff_f);
-            }
-            off[h + 1] = (int)ce.size();
-        }
-        std::vector<std::vector<std::array<int, 6>>> by_tp(ntp);
+        auto& by_tp = by_tp_t[t];
+        by_tp.assign(ntp, {});
         if (gt) {
             for (size_t k = 0; k < T.noconj.size(); ++k) {
                 const auto& a = T.nlm[T.noconj[k]];
@@ -65,23 +59,71 @@ void upload_potential(ModelHandle& m) {
             for (size_t k = 0; k < T.ntp.size(); ++k)
                 by_tp[T.ntp[k][2]].push_back({T.ntp[k][1], -1, (int)k, T.ntp[k][3], 1, 0});
         }
-        // heads of each type pair ordered by Y key, with the first head of every
-        // key (the per-lane pair loop generates Y_lm key by key)
+        const int n_heads = (int)T.pot.size();
+        auto& rank = m.head_rank[t];
+        rank.assign(n_heads, -1);
+        for (int tp = 0; tp < ntp; ++tp) {
+            if (gt)
+                std::stable_sort(by_tp[tp].begin(), by_tp[tp].end(),
+                                 [](const HeadEntry& a, const HeadEntry& b) { return a[1] < b[1]; });
+            for (const auto& e : by_tp[tp]) {
+                if (e[2] < 0 || e[2] >= n_heads || rank[e[2]] >= 0)
+                    throw std::invalid_argument("evaluation heads do not match the potential");
+                rank[e[2]] = (int)order[t].size();
+                order[t].push_back(e[2]);
+            }
+        }
+        if ((int)order[t].size() != n_heads)
+            throw std::invalid_argument("evaluation heads do not match the potential");
+    }
+    // the type of atom j for an atom of type t and a type pair
+    auto partner = [&](int t, int tp) {
+        int found = -1;
+        for (int t2 = 0; t2 < nt; ++t2)
+            if (M.type_pair(t, t2) == tp) {
+                if (found >= 0) throw std::invalid_argument("type pair with two partner types");
+                found = t2;
+            }
+        return found;
+    };
+    for (int t = 0; t < nt; ++t) {
+        const auto& T = M.type(t);
+        EvalTypeDev& E = m.ed.t[t];
+        E.n_heads = (int)T.pot.size();
+        m.max_heads = std::max(m.max_heads, E.n_heads);
+        m.max_lin = std::max(m.max_lin, (int)T.features.size());
+        // the terms by head position
+        std::vector<int> off(E.n_heads + 1, 0), pd, pf;
+        std::vector<double> ce, cf;
+        for (int r = 0; r < E.n_heads; ++r) {
+            for (const auto& pt : T.pot[order[t][r]]) {
+                const auto& k3 = T.prod_deriv[pt.prod];
+                const auto& k2 = T.prod_features[pt.prod_features];
+                if (k3.size() > 3 || k2.size() > 2) throw std::invalid_argument("term order too high");
+                for (int a = 0; a < 3; ++a) pd.push_back(a < (int)k3.size() ? k3[a] : -1);
+                for (int a = 0; a < 2; ++a) pf.push_back(a < (int)k2.size() ? k2[a] : -1);
+                ce.push_back(pt.coeff_e), cf.push_back(pt.coeff_f);
+            }
+            off[r + 1] = (int)ce.size();
+        }
+        // per type pair its heads and the first head of every Y key; hi and hj
+        // are head positions of atoms i and j
         const int nkey = m.md.n_lmh;
         std::vector<int> hoff(ntp + 1, 0), koff((size_t)ntp * (nkey + 1), 0), nid, ylm, hi, hj;
         std::vector<float> sj;
         std::vector<unsigned char> m0;
         for (int tp = 0; tp < ntp; ++tp) {
-            if (gt)
-                std::stable_sort(by_tp[tp].begin(), by_tp[tp].end(),
-                                 [](const std::array<int, 6>& a, const std::array<int, 6>& b) {
-                                     return a[1] < b[1];
-                                 });
+            const auto& heads = by_tp_t[t][tp];
+            const int t2 = heads.empty() ? -1 : partner(t, tp);
+            if (!heads.empty() && t2 < 0) throw std::invalid_argument("type pair without partner type");
             int* ko = koff.data() + (size_t)tp * (nkey + 1);
-            for (const auto& e : by_tp[tp]) {
+            for (const auto& e : heads) {
                 if (gt && (e[1] < 0 || e[1] >= nkey)) throw std::invalid_argument("head Y key range");
+                if (e[3] < 0 || e[3] >= (int)m.head_rank[t2].size())
+                    throw std::invalid_argument("head of atom j out of range");
                 if (gt) ++ko[e[1] + 1];
-                nid.push_back(e[0]), ylm.push_back(e[1]), hi.push_back(e[2]), hj.push_back(e[3]);
+                nid.push_back(e[0]), ylm.push_back(e[1]);
+                hi.push_back(m.head_rank[t][e[2]]), hj.push_back(m.head_rank[t2][e[3]]);
                 sj.push_back((float)e[4]), m0.push_back((unsigned char)e[5]);
             }
             ko[0] = hoff[tp];
