@@ -86,22 +86,35 @@ def test_eval_mgo_golden(model, e_true, f_true, s_true, form, monkeypatch):
         assert np.linalg.norm(s - sr) <= 1e-10 * np.linalg.norm(sr)
 
 
+# oracle results per configuration (the same seeded coefficients in every form)
+_ORACLE_EVAL = {}
+
+
 @pytest.mark.skipif(not ref.available(), reason="oracle library not built")
 @pytest.mark.parametrize("form", PAIR_FORMS)
-@pytest.mark.parametrize("cfg", ["cfg1", "cfg2"])
+@pytest.mark.parametrize("cfg", ["cfg1", "cfg2", "cfg3"])
 def test_eval_configs_vs_oracle(cfg, form, tmp_path, monkeypatch):
+    """cfg1 (one type), cfg2 (two types) and cfg3 (three types, 1,755 heads and
+    8,268 linear features per type; its 275- and 82-atom structures) against
+    the oracle's PolymlpEval in every kernel form."""
     _pair_form(form, monkeypatch)
     if cfg == "cfg1":
         spec, ds = datasets.config1_model(), datasets.config1_dataset(3)
-    else:
+        sts = ds.structures
+    elif cfg == "cfg2":
         spec, ds = datasets.config2_model(), datasets.config2_dataset(2, targets=False)
+        sts = ds.structures
+    else:
+        spec, ds = datasets.config3_model(), datasets.config3_dataset(3, targets=False)
+        sts = ds.structures[1:]
     m = capi.Model(spec.as_dict())
     rng = np.random.default_rng(7)
     coeffs = rng.normal(size=m.n_features) * 1e-3
     m.set_coeffs(coeffs)
-    path = write_model_yaml(spec, str(tmp_path / "m.yaml"), coeffs)
-    R = ref.RefModel(path, with_eval=True)
-    sts = ds.structures
+    if cfg not in _ORACLE_EVAL:
+        path = write_model_yaml(spec, str(tmp_path / "m.yaml"), coeffs)
+        R = ref.RefModel(path, with_eval=True)
+        _ORACLE_EVAL[cfg] = [R.eval(st.axis, st.positions_cartesian, st.types) for st in sts]
     b = capi.Batch(m, [s.axis for s in sts], [s.positions_cartesian for s in sts],
                    [s.types for s in sts])
     if _refused(form, m, spec.n_type):
@@ -114,7 +127,7 @@ def test_eval_configs_vs_oracle(cfg, form, tmp_path, monkeypatch):
     say(f"eval batch {cfg} MIXED")
     Em, Fm, Sm = capi.eval_batch(m, b, capi.PLM_MIXED)
     for k, st in enumerate(sts):
-        er, fr, sr = R.eval(st.axis, st.positions_cartesian, st.types)
+        er, fr, sr = _ORACLE_EVAL[cfg][k]
         assert E[k] == pytest.approx(er, rel=1e-10)
         assert np.linalg.norm(F[k] - fr) <= 1e-10 * np.linalg.norm(fr)
         assert np.linalg.norm(S[k] - sr) <= 1e-10 * np.linalg.norm(sr)
