@@ -563,16 +563,16 @@ void launch_pair_lane_t(hipStream_t st, const ModelDev& md, const EvalDev& ed,
                         const void* ps_f, int ps_stride, double* energy, double* forces,
                         double* stress, int pair_model) {
     const int L = md.max_l;
-    // radial arrays of 16 entries keep FP64 values in registers (32 spill
-    // 528 bytes per lane to scratch); measured faster in FP64 at max_l <= 4
-    // only (2.26 against 2.53 ms per 32,000 atoms; at max_l 8 the register
-    // pressure costs more, and FP32 never spills)
-    if constexpr (sizeof(T) == 8) {
-        if (L <= 4 && md.n_fn <= 16) {
-            launch_pair_lb<T, 4, 16>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f,
-                                     ps_stride, energy, forces, stress, pair_model);
-            return;
-        }
+    // radial arrays of 16 entries where the model has at most 16 radial
+    // functions and max_l <= 4: in FP64 they stay in registers (32 spill 528
+    // bytes per lane to scratch; 2.26 against 2.53 ms per 32,000 atoms), in
+    // FP32 the lower register count holds 4 waves per SIMD instead of 3 (1.60
+    // against 1.66 ms); at max_l 8 the 16-entry form's registers cost more than
+    // the spill (4.40 against 3.90 ms), so it keeps 32
+    if (L <= 4 && md.n_fn <= 16) {
+        launch_pair_lb<T, 4, 16>(st, md, ed, bd, atom_st, hb_off, hb_j, hb_d, n_half, ps_e, ps_f,
+                                 ps_stride, energy, forces, stress, pair_model);
+        return;
     }
     if (md.n_fn > 32)
         throw std::invalid_argument("the per-lane pair loop supports at most 32 radial functions");
