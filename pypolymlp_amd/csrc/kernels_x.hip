@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(256) anlm_kernel(ModelDev md, BatchDev bd,
     const int i = atom0 + ia;
     const int t1 = bd.types[i];
     const TypeDev& TT = md.t[t1];
-    const PairLayout<T> L(md.n_fn, md.n_lmh);
+    const PairLayout<T> L(md.n_fn, md.n_lmh, false);  // values only
     const int NPS = kAnlmPairs;  // pairs per chunk, one lane each
     T* pairs = reinterpret_cast<T*>(smem_raw);
     Cpx<T>* acc = reinterpret_cast<Cpx<T>*>(pairs + (size_t)NPS * L.stride);
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(NT) anlm_reg_kernel(ModelDev md, BatchDev bd,
     const int i = atom0 + ia;
     const int t1 = bd.types[i];
     const TypeDev& TT = md.t[t1];
-    const PairLayout<T> L(md.n_fn, md.n_lmh);
+    const PairLayout<T> L(md.n_fn, md.n_lmh, false);  // values only
     const int NPS = kAnlmPairs;
     T* pairs = reinterpret_cast<T*>(smem_raw);
     Cpx<T>* acc = reinterpret_cast<Cpx<T>*>(pairs + (size_t)NPS * L.stride);
@@ -304,7 +304,7 @@ void launch_neighbor_half(hipStream_t st, int n, const int* atom_st, const int* 
 }
 
 size_t anlm_smem(const ModelDev& md, int elem, int nthreads, int max_noconj) {
-    const PairLayout<double> L(md.n_fn, md.n_lmh);
+    const PairLayout<double> L(md.n_fn, md.n_lmh, false);
     return (size_t)nthreads * L.stride * elem + (size_t)max_noconj * 2 * elem;
 }
 

@@ -16,19 +16,23 @@ namespace plm {
 template <class T>
 struct PairLayout {
     int nfn, nlmh;
+    int dyw;     // elements of dY: 6 nlmh, or 0 in the values-only layout
     int stride;  // elements of T per pair slot
-    __device__ __host__ PairLayout(int nfn_, int nlmh_) : nfn(nfn_), nlmh(nlmh_) {
-        // fn[nfn] fd[nfn] Y[2*nlmh] dY[6*nlmh] p[nlmh] q[nlmh] misc[8]
-        stride = 2 * nfn + 10 * nlmh + 8;
+    // fn[nfn] fd[nfn] Y[2*nlmh] dY[6*nlmh] p[nlmh] q[nlmh] misc[8]; without
+    // derivatives (deriv false: pairs_coop then writes no fd and no dY) the
+    // slot has no dY
+    __device__ __host__ PairLayout(int nfn_, int nlmh_, bool deriv = true)
+        : nfn(nfn_), nlmh(nlmh_), dyw(deriv ? 6 * nlmh_ : 0) {
+        stride = 2 * nfn + 4 * nlmh + dyw + 8;
     }
     __device__ T* fn(T* b) const { return b; }
     __device__ T* fd(T* b) const { return b + nfn; }
     __device__ Cpx<T>* Y(T* b) const { return reinterpret_cast<Cpx<T>*>(b + 2 * nfn); }
     __device__ Cpx<T>* dY(T* b) const { return reinterpret_cast<Cpx<T>*>(b + 2 * nfn + 2 * nlmh); }
-    __device__ T* p(T* b) const { return b + 2 * nfn + 8 * nlmh; }
-    __device__ T* q(T* b) const { return b + 2 * nfn + 9 * nlmh; }
-    __device__ T* misc(T* b) const { return b + 2 * nfn + 10 * nlmh; }
-    __device__ const T* misc(const T* b) const { return b + 2 * nfn + 10 * nlmh; }
+    __device__ T* p(T* b) const { return b + 2 * nfn + 2 * nlmh + dyw; }
+    __device__ T* q(T* b) const { return b + 2 * nfn + 3 * nlmh + dyw; }
+    __device__ T* misc(T* b) const { return b + 2 * nfn + 4 * nlmh + dyw; }
+    __device__ const T* misc(const T* b) const { return b + 2 * nfn + 4 * nlmh + dyw; }
 };
 // misc: [0..2] d, [3] r, [4] valid (1/0), [5] tp, [6] j (atom index within
 // its structure)
